@@ -1,0 +1,153 @@
+/*
+ * lmff_hip.h -- C ABI of the MI355X LMFF force path (Tersoff + ILP/graphene/hBN).
+ *
+ * Plain C types only: pointers, sizes and status codes.  Every function
+ * returns LMFF_OK (0) or a positive LMFF_ERR_* code; lmff_last_error() gives
+ * the message.  A context is not thread-safe: drive it from one host thread.
+ * Arrays are host pointers, atom-major ([n][3] = x0 y0 z0 x1 ...), in the
+ * caller's atom order; the library keeps its own cell-sorted copy in HBM.
+ *
+ * Reference interfaces replaced (paths relative to the LAMMPS-LMFF tree):
+ *   lmff_settings        PairHybrid::settings -> PairILPGrapheneHBN::settings
+ *                        (Baseline/USER-MISC/pair_ilp_graphene_hbn.cpp:132-149)
+ *   lmff_coeff_tersoff   PairTersoff::coeff + read_file + setup_params
+ *                        (Baseline/MANYBODY/pair_tersoff.cpp:293-358, 392-580)
+ *   lmff_coeff_ilp       PairILPGrapheneHBN::coeff + read_file (pair_ilp_graphene_hbn.cpp:155-219, 246-387)
+ *   lmff_neighbor        "neighbor <skin> bin" + "neigh_modify every/delay/check"
+ *                        (LMFF/neighbor.cpp:87-91, 240-317, 1947-2035)
+ *   lmff_pair_compute    Pair::compute(eflag, vflag) of "hybrid/overlay tersoff ilp/graphene/hbn"
+ *                        (LMFF/pair_hybrid.cpp:93-210; pair_tersoff.cpp:86-266;
+ *                        pair_ilp_graphene_hbn.cpp:428-675), with the SWLMFF
+ *                        host->device precedent of SWLMFF/pair_lmff.cpp:37-43,376-445
+ *   lmff_run             Verlet::run with fix nve / fix langevin on device
+ *                        (LMFF/verlet.cpp:220-350, LMFF/fix_nve.cpp:64-143,
+ *                        LMFF/fix_langevin.cpp:447-760; SWLMFF/fix_nve_sw64.c:7-161)
+ */
+#ifndef LMFF_HIP_H
+#define LMFF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lmff_ctx lmff_ctx;
+
+enum {
+  LMFF_OK = 0,
+  LMFF_ERR_ARG = 1,      /* bad argument or call order */
+  LMFF_ERR_FILE = 2,     /* potential file missing or malformed */
+  LMFF_ERR_PARAM = 3,    /* illegal / duplicate / missing potential entry */
+  LMFF_ERR_HIP = 4,      /* HIP runtime failure */
+  LMFF_ERR_NORMAL = 5,   /* >3 normal neighbours or zero normal (ilp.cpp:739,887,1030,1070) */
+  LMFF_ERR_SHORT = 6,    /* too many Tersoff neighbours within cutmax */
+  LMFF_ERR_NONFINITE = 7,/* non-numeric coordinates (domain.cpp:514) */
+  LMFF_ERR_MEMORY = 8,   /* HBM budget exceeded */
+  LMFF_ERR_BOX = 9,      /* periodic side not above twice the neighbor cutoff */
+  LMFF_ERR_STATE = 10    /* no such system / not set up */
+};
+
+enum { LMFF_FP64 = 0, LMFF_MIXED = 1 };
+enum { LMFF_NVE = 0, LMFF_LANGEVIN = 1 };
+
+/* Context: owns one HIP stream on `device` and all HBM state. */
+int lmff_create(lmff_ctx **ctx, int device);
+void lmff_destroy(lmff_ctx *ctx);
+const char *lmff_last_error(const lmff_ctx *ctx);
+const char *lmff_version(void);
+
+/* pair_style hybrid/overlay tersoff ilp/graphene/hbn <cut_global> [tap_flag]
+ * ntypes = number of LAMMPS atom types (1-based types in all arrays). */
+int lmff_settings(lmff_ctx *ctx, int ntypes, double ilp_cut_global, int tap_flag);
+/* pair_coeff * * tersoff <file> e1 .. e_ntypes   ("NULL" leaves a type out) */
+int lmff_coeff_tersoff(lmff_ctx *ctx, const char *file, const char *const *elements);
+/* pair_coeff * * ilp/graphene/hbn <file> e1 .. e_ntypes */
+int lmff_coeff_ilp(lmff_ctx *ctx, const char *file, const char *const *elements);
+/* neighbor <skin> bin ; neigh_modify every <every> delay <delay> check <yes=1|no=0> */
+int lmff_neighbor(lmff_ctx *ctx, double skin, int every, int delay, int check);
+/* LMFF_FP64 or LMFF_MIXED (FP32 pair/bond-order math, FP64 displacements and accumulation) */
+int lmff_precision(lmff_ctx *ctx, int mode);
+/* mass <type> <value> for all types (index type-1) */
+int lmff_masses(lmff_ctx *ctx, const double *mass_per_type);
+
+/* ---- Pair plugin entry: one force evaluation on host arrays ---------------------------
+ * Semantics of Pair::compute(eflag, vflag) for hybrid/overlay tersoff ilp/graphene/hbn on
+ * nlocal owned atoms of an orthogonal periodic box (no ghosts: images are handled inside).
+ * eflag bit0 = global energy, bit1 = per-atom energy; vflag%4 != 0 = global virial,
+ * vflag&4 = per-atom virial.  Outputs ACCUMULATE (+=) into f[n][3], eatom[n], vatom[n][6]
+ * as LAMMPS pair styles do; eng_vdwl and virial[6] (xx yy zz xy xz yz) and pvector[2]
+ * (ILP vdW, ILP repulsion energies) are overwritten.  new_list != 0 forces a neighbor
+ * rebuild (the caller's neighbor->ago == 0); the first call always builds.
+ * Any output pointer may be NULL. */
+int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int32_t *type,
+                      const int64_t *tag, const int64_t *mol, const double boxlo[3],
+                      const double boxhi[3], const int periodic[3], int new_list, int eflag,
+                      int vflag, double *f, double *eng_vdwl, double virial[6], double *eatom,
+                      double *vatom, double pvector[2]);
+
+/* ---- Device-resident systems (standalone runs and the batched job server) ------------
+ * A context holds any number of independent systems packed into one atom array; every
+ * kernel launch covers all of them.  x, v are [n][3]; v may be NULL (zero). */
+int lmff_system_add(lmff_ctx *ctx, int64_t natoms, const double boxlo[3], const double boxhi[3],
+                    const int periodic[3], const int32_t *type, const int64_t *tag,
+                    const int64_t *mol, const double *x, const double *v, int *sys_id);
+int lmff_system_remove(lmff_ctx *ctx, int sys_id);
+/* fix nve (style LMFF_NVE) or fix langevin T T damp seed + fix nve (LMFF_LANGEVIN) */
+int lmff_system_integrator(lmff_ctx *ctx, int sys_id, int style, double dt, double t_target,
+                           double damp, uint64_t seed);
+int lmff_system_natoms(lmff_ctx *ctx, int sys_id, int64_t *natoms);
+/* copies the system's state back in the caller's original atom order (any may be NULL) */
+int lmff_system_get(lmff_ctx *ctx, int sys_id, double *x, double *v, double *f);
+int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *x, const double *v);
+/* per-atom energy [n] and virial [n][6] of the last evaluation with those flags */
+int lmff_system_peratom(lmff_ctx *ctx, int sys_id, double *eatom, double *vatom);
+
+/* Thermo record of one system (metal units). */
+typedef struct {
+  int64_t step;
+  double pe;           /* eng_vdwl of the pair: Tersoff + ILP */
+  double ke;
+  double e_vdw;        /* pvector[0] */
+  double e_rep;        /* pvector[1] */
+  double e_tersoff;
+  double temp;         /* 2 ke / ((3N-3) kB) */
+  double virial[6];    /* pair virial, F.r convention */
+} lmff_thermo;
+
+/* Evaluate forces (and energy/virial per flags) of every resident system at the current
+ * positions, rebuilding neighbor lists if positions were set since the last build. */
+int lmff_evaluate(lmff_ctx *ctx, int eflag, int vflag);
+int lmff_system_thermo(lmff_ctx *ctx, int sys_id, lmff_thermo *out);
+
+/* Advance every resident system by nsteps Velocity-Verlet steps (forces from the previous
+ * evaluation are reused as LAMMPS does after setup).  Energies are evaluated every
+ * thermo_every steps (0 = only at the end); records go to out[] in system-major order,
+ * out[s * nrec + r], nrec = nsteps / thermo_every (+1 if nsteps is not a multiple);
+ * out may be NULL.  use_graph != 0 replays the steps between neighbor rebuilds as a
+ * captured hipGraph. */
+int lmff_run(lmff_ctx *ctx, int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out,
+             int64_t out_capacity);
+
+/* Device-side timing of the last lmff_run: total ms and per-kernel-family averages
+ * (index: 0 list build, 1 normals, 2 ILP pair, 3 Tersoff zeta, 4 gather+Tersoff force,
+ *  5 integrate, 6 reductions).  Requires lmff_profile(ctx, 1) before the run. */
+int lmff_profile(lmff_ctx *ctx, int enable);
+int lmff_profile_read(lmff_ctx *ctx, double *ms_total, double *ms_per_family, int64_t *launches_per_family);
+
+/* HBM accounting: bytes currently held and the per-atom estimate used for admission. */
+int lmff_memory(lmff_ctx *ctx, int64_t *bytes_in_use, double *bytes_per_atom_estimate);
+/* Bytes a new system of natoms would add (lists sized from the current density estimate). */
+int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint);
+
+/* ILP list of one atom (caller order index) as sorted (caller-order j, sx, sy, sz) rows, for
+ * neighbor-set parity checks; returns the count in *n (rows up to capacity). */
+int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, int capacity, int *n);
+/* Normals (caller order): normal[n][3], count[n]. */
+int lmff_debug_normals(lmff_ctx *ctx, int sys_id, double *normal, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMFF_HIP_H */

@@ -1,0 +1,1371 @@
+// Host engine and C ABI of the LMFF force path on MI355X (see include/lmff_hip.h).
+//
+// One context = one HIP stream + every resident system packed into one cell-sorted atom
+// array.  Per MD step (Verlet::run, LMFF/verlet.cpp:220-350):
+//   initial_integrate -> [rebuild: pbc wrap, cell sort, permute, list build]
+//   -> normals -> ILP pair -> Tersoff zeta -> gather (normal-derivative + Tersoff forces)
+//   -> post_force/final_integrate -> [thermo reductions]
+// The steps between rebuilds can be replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <array>
+#include <functional>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lmff_hip.h"
+#include "lmff_kernels.hpp"
+#include "potentials.hpp"
+
+using namespace lmff;
+
+namespace {
+
+constexpr int CHUNK = 4096;
+constexpr int BS = 256;
+
+struct HipError {
+  hipError_t e;
+  const char *what;
+};
+
+#define HIPCHK(call)                                      \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) throw HipError{e_, #call};      \
+  } while (0)
+
+struct LmffError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw LmffError{code, buf};
+}
+
+// Device buffer with capacity growth (contents are not preserved on growth).
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  size_t *tally = nullptr;
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    release();
+    size_t b = std::max<size_t>(need, 256);
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+    if (tally) *tally += b;
+  }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      if (tally) *tally -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T *as() const { return (T *)p; }
+  ~DBuf() { release(); }
+};
+
+struct SysHost {
+  int id;
+  bool live;
+  int64_t n;
+  double lo[3], hi[3];
+  int per[3];
+  int nc[3];
+  double glo[3], bs[3];
+  int cell_off, ncells;
+  int begin, end;
+  int style = LMFF_NVE;
+  double dt = 0.001, t_target = 300.0, damp = 1.0;
+  unsigned long long seed = 1;
+  long long step = 0;
+  std::vector<int> chunk_ids;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t bytes_in_use = 0;
+  // settings
+  int ntypes = 0;
+  double ilp_cut = 16.0;
+  int tap_flag = 1;
+  bool have_settings = false, have_tersoff = false, have_ilp = false;
+  TersoffTable ters;
+  IlpTable ilp;
+  double skin = 2.0;
+  int every = 1, delay = 0, check = 1;
+  int precision = LMFF_FP64;
+  std::vector<double> mass;  // per type
+  bool params_dirty = true;
+  // systems
+  std::vector<SysHost> sys;  // in packed order
+  int next_id = 0;
+  int64_t n = 0;             // total atoms
+  int ncells = 0;
+  bool layout_dirty = true;  // packed arrays must be rebuilt from host staging
+  bool need_rebuild = true;
+  bool forces_valid = false;
+  int ago = 0;
+  long long global_step = 0;
+  // staging for systems added since the last layout (host copies, caller order)
+  struct Staged {
+    int id;
+    std::vector<double> x, v;
+    std::vector<int> type, tag, mol;
+  };
+  std::vector<Staged> staged;
+  // device state
+  DBuf x, v, f, xhold, mol, tag, sysid, orig, key, key2, val, perm, tmp4, tmpi;
+  DBuf slist, scnt, ilist, icnt;
+  DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, fz, pref, eb, fil, S, T, th, vatom;
+  DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
+  DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag;
+  int scap = 64, icap = 0;
+  size_t sort_tmp_bytes = 0;
+  int end_bit = 1;
+  int nchunks = 0;
+  // derived cutoffs
+  double cutneigh = 18.0, short_cut = 4.1;
+  // graph
+  hipGraphExec_t graph = nullptr;
+  int graph_len = 0;
+  long long graph_gen = -1;
+  long long gen = 0;
+  int graph_prec = -1;
+  // last evaluation flags (for per-atom outputs)
+  int last_eflag = 0, last_vflag = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks;
+  double prof_total = 0.0;
+  double prof_ms[8] = {0};
+  long long prof_cnt[8] = {0};
+  // pair-plugin bookkeeping
+  int pair_sys = -1;
+  int64_t pair_n = -1;
+
+  Ctx() {
+    DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
+                   &slist, &scnt, &ilist, &icnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &fz, &pref, &eb,
+                   &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
+                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag};
+    for (DBuf *b : all) b->tally = &bytes_in_use;
+  }
+  ~Ctx() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  SysHost *find(int id) {
+    for (auto &s : sys)
+      if (s.id == id && s.live) return &s;
+    return nullptr;
+  }
+
+  // ---------------------------------------------------------------- parameters
+  template <typename R>
+  void fill_params(std::vector<IlpPair<R>> &ipv, std::vector<TersP<R>> &tpv, std::vector<int> &t3v) {
+    ipv.assign(MAXT * MAXT, IlpPair<R>{});
+    for (int a = 1; a <= ntypes; a++)
+      for (int b = 1; b <= ntypes; b++) {
+        IlpPair<R> &q = ipv[(a - 1) * MAXT + (b - 1)];
+        const int ea = ilp.emap.map[a], eb_ = ilp.emap.map[b];
+        if (ea < 0 || eb_ < 0) continue;
+        const int ne = ilp.nelements();
+        const IlpEntry &p = ilp.params[ilp.elem2param[ea * ne + eb_]];
+        const IlpEntry &r = ilp.params[ilp.elem2param[eb_ * ne + ea]];
+        q.z0 = (R)p.z0;
+        q.lambda = (R)p.lambda;
+        q.half_eps = (R)(0.5 * p.epsilon);
+        q.C = (R)p.C;
+        q.d2inv = (R)p.delta2inv;
+        q.C6 = (R)p.C6;
+        q.d = (R)p.d;
+        q.inv_seff = (R)(1.0 / p.seff);
+        q.C6_d_seff = (R)(p.C6 * p.d / p.seff);
+        q.rcutsq = ilp.cutILPsq[ea * ne + eb_];
+        q.mapped = 1;
+        q.vsym = (p.C6 == r.C6 && p.d == r.d && p.seff == r.seff) ? 1 : 0;
+        q.esym = (p.lambda == r.lambda && p.z0 == r.z0) ? 1 : 0;
+      }
+    tpv.clear();
+    for (const auto &p : ters.params) {
+      TersP<R> q{};
+      q.lam1 = (R)p.lam1; q.biga = (R)p.biga; q.lam2 = (R)p.lam2; q.bigb = (R)p.bigb;
+      q.bigr = (R)p.bigr; q.bigd = (R)p.bigd; q.cc = (R)(p.c * p.c); q.dd = (R)(p.d * p.d);
+      q.h = (R)p.h; q.gamma = (R)p.gamma; q.powern = (R)p.powern; q.beta = (R)p.beta;
+      q.c1 = (R)p.c1; q.c2 = (R)p.c2; q.c3 = (R)p.c3; q.c4 = (R)p.c4; q.lam3 = (R)p.lam3;
+      q.cutsq = p.cutsq;
+      q.powermint = p.powermint;
+      tpv.push_back(q);
+    }
+    if (tpv.empty()) tpv.push_back(TersP<R>{});
+    t3v.assign(MAXT * MAXT * MAXT, -1);
+    for (int a = 1; a <= ntypes; a++)
+      for (int b = 1; b <= ntypes; b++)
+        for (int c = 1; c <= ntypes; c++) {
+          const int ea = ters.emap.map[a], eb_ = ters.emap.map[b], ec = ters.emap.map[c];
+          if (ea < 0 || eb_ < 0 || ec < 0) continue;
+          t3v[((a - 1) * MAXT + (b - 1)) * MAXT + (c - 1)] = ters.index(ea, eb_, ec);
+        }
+  }
+
+  void upload_params() {
+    if (!params_dirty) return;
+    if (!have_settings || !have_tersoff || !have_ilp) fail(LMFF_ERR_ARG, "pair_style/pair_coeff not complete");
+    if ((int)mass.size() != ntypes) fail(LMFF_ERR_ARG, "masses not set for all atom types");
+    std::vector<IlpPair<double>> ipd;
+    std::vector<IlpPair<float>> ipf;
+    std::vector<TersP<double>> tpd;
+    std::vector<TersP<float>> tpf;
+    std::vector<int> t3v;
+    fill_params(ipd, tpd, t3v);
+    fill_params(ipf, tpf, t3v);
+    ip_d.ensure(sizeof(IlpPair<double>) * ipd.size());
+    ip_f.ensure(sizeof(IlpPair<float>) * ipf.size());
+    tp_d.ensure(sizeof(TersP<double>) * tpd.size());
+    tp_f.ensure(sizeof(TersP<float>) * tpf.size());
+    t3.ensure(sizeof(int) * t3v.size());
+    rmass.ensure(sizeof(double) * MAXT);
+    errw.ensure(sizeof(int) * 4);
+    ovf.ensure(sizeof(int) * 4);
+    std::vector<double> m(MAXT, 1.0);
+    for (int t = 0; t < ntypes; t++) m[t] = mass[t];
+    HIPCHK(hipMemcpy(ip_d.p, ipd.data(), sizeof(IlpPair<double>) * ipd.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ip_f.p, ipf.data(), sizeof(IlpPair<float>) * ipf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tp_d.p, tpd.data(), sizeof(TersP<double>) * tpd.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tp_f.p, tpf.data(), sizeof(TersP<float>) * tpf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t3.p, t3v.data(), sizeof(int) * t3v.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(rmass.p, m.data(), sizeof(double) * MAXT, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(errw.p, 0, sizeof(int) * 4));
+    double rc_max = 0.0;
+    for (double c : ilp.cutILPsq) rc_max = std::max(rc_max, std::sqrt(c));
+    const double cutforce = std::max(ilp_cut, ters.cutmax);
+    cutneigh = cutforce + skin;
+    short_cut = std::max(ters.cutmax, rc_max) + skin;
+    params_dirty = false;
+    gen++;
+  }
+
+  // ---------------------------------------------------------------- layout
+  // Builds the packed device arrays from the resident systems (kept in their current
+  // sorted order) plus staged new systems, in system order.
+  void relayout() {
+    if (!layout_dirty) return;
+    upload_params();
+    // gather resident systems' device data (contiguous [begin,end) per live system)
+    struct Blob {
+      std::vector<double4> x, v;
+      std::vector<int> mol, tag, orig;
+    };
+    std::vector<SysHost> keep;
+    std::vector<Blob> blobs;
+    for (auto &s : sys) {
+      if (!s.live) continue;
+      Blob b;
+      const size_t m = (size_t)(s.end - s.begin);
+      b.x.resize(m); b.v.resize(m); b.mol.resize(m); b.tag.resize(m); b.orig.resize(m);
+      if (m) {
+        HIPCHK(hipMemcpy(b.x.data(), x.as<double4>() + s.begin, m * sizeof(double4), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.v.data(), v.as<double4>() + s.begin, m * sizeof(double4), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.mol.data(), mol.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.tag.data(), tag.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.orig.data(), orig.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
+      }
+      keep.push_back(s);
+      blobs.push_back(std::move(b));
+    }
+    for (auto &st : staged) {
+      SysHost *s = nullptr;
+      for (auto &q : sys)
+        if (q.id == st.id) s = &q;
+      if (!s || !s->live) continue;
+      Blob b;
+      const size_t m = (size_t)s->n;
+      b.x.resize(m); b.v.resize(m); b.mol.resize(m); b.tag.resize(m); b.orig.resize(m);
+      for (size_t i = 0; i < m; i++) {
+        b.x[i] = {st.x[3 * i], st.x[3 * i + 1], st.x[3 * i + 2], (double)st.type[i]};
+        b.v[i] = {st.v[3 * i], st.v[3 * i + 1], st.v[3 * i + 2], 0.0};
+        b.mol[i] = st.mol[i];
+        b.tag[i] = st.tag[i];
+        b.orig[i] = (int)i;
+      }
+      SysHost copy = *s;
+      bool found = false;
+      for (size_t k = 0; k < keep.size(); k++)
+        if (keep[k].id == st.id) {
+          blobs[k] = std::move(b);
+          found = true;
+        }
+      if (!found) {
+        keep.push_back(copy);
+        blobs.push_back(std::move(b));
+      }
+    }
+    staged.clear();
+    sys = keep;
+    // atom ranges, cell grids
+    n = 0;
+    ncells = 0;
+    for (auto &s : sys) {
+      s.begin = (int)n;
+      n += s.n;
+      s.end = (int)n;
+      s.cell_off = ncells;
+      ncells += s.ncells;
+    }
+    if (n >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)n);
+    alloc_atoms((size_t)n);
+    std::vector<double4> hx((size_t)n), hv((size_t)n);
+    std::vector<int> hm((size_t)n), ht((size_t)n), ho((size_t)n), hs((size_t)n);
+    for (size_t k = 0; k < sys.size(); k++) {
+      auto &s = sys[k];
+      auto &b = blobs[k];
+      std::copy(b.x.begin(), b.x.end(), hx.begin() + s.begin);
+      std::copy(b.v.begin(), b.v.end(), hv.begin() + s.begin);
+      std::copy(b.mol.begin(), b.mol.end(), hm.begin() + s.begin);
+      std::copy(b.tag.begin(), b.tag.end(), ht.begin() + s.begin);
+      std::copy(b.orig.begin(), b.orig.end(), ho.begin() + s.begin);
+      std::fill(hs.begin() + s.begin, hs.begin() + s.end, (int)k);
+    }
+    if (n) {
+      HIPCHK(hipMemcpy(x.p, hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(v.p, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(mol.p, hm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(tag.p, ht.data(), n * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(orig.p, ho.data(), n * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(sysid.p, hs.data(), n * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemset(f.p, 0, n * sizeof(double4)));
+    }
+    upload_systab();
+    // reduction chunks: never straddle a system
+    std::vector<Chunk> ch;
+    std::vector<int> cb;
+    for (size_t k = 0; k < sys.size(); k++) {
+      cb.push_back((int)ch.size());
+      for (int b = sys[k].begin; b < sys[k].end; b += CHUNK) ch.push_back(Chunk{b, std::min(b + CHUNK, sys[k].end), (int)k, 0});
+    }
+    cb.push_back((int)ch.size());
+    nchunks = (int)ch.size();
+    chunks.ensure(sizeof(Chunk) * std::max<size_t>(ch.size(), 1));
+    cbeg.ensure(sizeof(int) * cb.size());
+    partial.ensure(sizeof(double2) * std::max(nchunks, 1) * 16);
+    sums.ensure(sizeof(double) * std::max<size_t>(sys.size(), 1) * 16);
+    if (!ch.empty()) HIPCHK(hipMemcpy(chunks.p, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(cbeg.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+    cell_start.ensure(sizeof(int) * (size_t)(ncells + 1));
+    end_bit = 1;
+    while ((1ll << end_bit) <= (long long)ncells) end_bit++;
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr, (int *)nullptr,
+                                              (int *)nullptr, (int)std::max<int64_t>(n, 1), 0, end_bit, stream));
+    sort_tmp.ensure(tb);
+    sort_tmp_bytes = tb;
+    layout_dirty = false;
+    need_rebuild = true;
+    forces_valid = false;
+    gen++;
+  }
+
+  void alloc_atoms(size_t m) {
+    m = std::max<size_t>(m, 1);
+    x.ensure(m * sizeof(double4)); v.ensure(m * sizeof(double4)); f.ensure(m * sizeof(double4));
+    xhold.ensure(m * sizeof(double4)); tmp4.ensure(m * sizeof(double4));
+    mol.ensure(m * sizeof(int)); tag.ensure(m * sizeof(int)); sysid.ensure(m * sizeof(int));
+    orig.ensure(m * sizeof(int)); tmpi.ensure(m * sizeof(int));
+    key.ensure(m * sizeof(uint32_t)); key2.ensure(m * sizeof(uint32_t));
+    val.ensure(m * sizeof(int)); perm.ensure(m * sizeof(int));
+    scnt.ensure(m * sizeof(int)); icnt.ensure(m * sizeof(int));
+    slist.ensure(m * scap * sizeof(uint32_t));
+    nrm.ensure(m * sizeof(double4));
+    dn.ensure(m * 27 * sizeof(double)); dndri.ensure(m * 9 * sizeof(double)); vet.ensure(m * 9 * sizeof(double));
+    nnb.ensure(m * 4 * sizeof(uint32_t)); tsh.ensure(m * MAXSH * sizeof(uint32_t)); tcnt.ensure(m * sizeof(int));
+    fz.ensure(m * MAXSH * sizeof(double)); pref.ensure(m * MAXSH * sizeof(double)); eb.ensure(m * MAXSH * sizeof(double));
+    fil.ensure(m * sizeof(double4)); S.ensure(m * sizeof(double4)); T.ensure(m * 6 * sizeof(double));
+    th.ensure(m * NTH * sizeof(double)); vatom.ensure(m * 6 * sizeof(double));
+  }
+
+  void upload_systab() {
+    std::vector<SysDev> t(sys.size());
+    for (size_t k = 0; k < sys.size(); k++) {
+      const auto &s = sys[k];
+      SysDev &d = t[k];
+      for (int a = 0; a < 3; a++) {
+        d.lo[a] = s.lo[a];
+        d.prd[a] = s.hi[a] - s.lo[a];
+        d.per[a] = s.per[a];
+        d.nc[a] = s.nc[a];
+        d.glo[a] = s.glo[a];
+        d.inv_bs[a] = 1.0 / s.bs[a];
+      }
+      d.cell_off = s.cell_off;
+      d.begin = s.begin;
+      d.end = s.end;
+      d.dtv = s.dt;
+      d.dtf = 0.5 * s.dt * FTM2V;
+      d.style = s.style;
+      d.t_target = s.t_target;
+      d.damp = s.damp;
+      d.seed = s.seed;
+      d.step = s.step;
+    }
+    systab.ensure(sizeof(SysDev) * std::max<size_t>(t.size(), 1));
+    if (!t.empty()) HIPCHK(hipMemcpy(systab.p, t.data(), sizeof(SysDev) * t.size(), hipMemcpyHostToDevice));
+  }
+
+  void sync_steps_from_device() {
+    if (sys.empty()) return;
+    std::vector<SysDev> t(sys.size());
+    HIPCHK(hipMemcpy(t.data(), systab.p, sizeof(SysDev) * t.size(), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < sys.size(); k++) sys[k].step = t[k].step;
+  }
+
+  // ---------------------------------------------------------------- device pointers
+  template <typename R>
+  DevPtrs<R> ptrs() {
+    DevPtrs<R> P{};
+    P.n = (int)n;
+    P.x = x.as<double4>(); P.v = v.as<double4>(); P.f = f.as<double4>();
+    P.mol = mol.as<int>(); P.tag = tag.as<int>(); P.sys = sysid.as<int>();
+    P.sys_tab = systab.as<SysDev>();
+    P.slist = slist.as<uint32_t>(); P.scnt = scnt.as<int>(); P.scap = scap;
+    P.ilist = ilist.as<uint32_t>(); P.icnt = icnt.as<int>(); P.icap = icap;
+    P.nrm = (typename std::conditional<sizeof(R) == 8, double4, float4>::type *)nrm.p;
+    P.dn = dn.as<R>(); P.dndri = dndri.as<R>(); P.vet = vet.as<double>();
+    P.nnb = nnb.as<uint32_t>(); P.tsh = tsh.as<uint32_t>(); P.tcnt = tcnt.as<int>();
+    P.fz = fz.as<double>(); P.pref = pref.as<double>(); P.eb = eb.as<double>();
+    P.fil = fil.as<double4>(); P.S = S.as<double4>(); P.T = T.as<double>();
+    P.th = th.as<double>(); P.vatom = vatom.as<double>();
+    P.ip = (const IlpPair<R> *)(sizeof(R) == 8 ? ip_d.p : ip_f.p);
+    P.tp = (const TersP<R> *)(sizeof(R) == 8 ? tp_d.p : tp_f.p);
+    P.t3 = t3.as<int>();
+    P.err = errw.as<int>();
+    P.ovf = ovf.as<int>();
+    P.cutneighsq = cutneigh * cutneigh;
+    P.shortsq = short_cut * short_cut;
+    P.tersoff_cutmaxsq = ters.cutmax * ters.cutmax;
+    P.ilp_cutsq = ilp_cut * ilp_cut;
+    P.rcut = std::sqrt(ilp_cut * ilp_cut);
+    P.inv_rcut = 1.0 / P.rcut;
+    P.tap_flag = tap_flag;
+    return P;
+  }
+
+  // ---------------------------------------------------------------- profiling helpers
+  void mark_begin(int fam, size_t &slot) {
+    if (!prof) return;
+    if (ev_used + 2 > ev_pool.size()) {
+      size_t add = std::max<size_t>(ev_pool.size(), 256);
+      for (size_t k = 0; k < add; k++) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ev_pool.push_back(e);
+      }
+    }
+    slot = ev_used;
+    ev_used += 2;
+    HIPCHK(hipEventRecord(ev_pool[slot], stream));
+    (void)fam;
+  }
+  void mark_end(int fam, size_t slot) {
+    if (!prof) return;
+    HIPCHK(hipEventRecord(ev_pool[slot + 1], stream));
+    ev_marks.push_back({fam, {slot, slot + 1}});
+  }
+
+  // ---------------------------------------------------------------- rebuild
+  void rebuild() {
+    if (n == 0) {
+      need_rebuild = false;
+      return;
+    }
+    size_t mk = 0;
+    mark_begin(0, mk);
+    const int nb = (int)((n + BS - 1) / BS);
+    hipLaunchKernelGGL(k_wrap_key, dim3(nb), dim3(BS), 0, stream, (int)n, x.as<double4>(), sysid.as<int>(),
+                       systab.as<SysDev>(), key.as<uint32_t>(), val.as<int>(), errw.as<int>());
+    size_t tb = sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(), val.as<int>(),
+                                              perm.as<int>(), (int)n, 0, end_bit, stream));
+    permute4(x);
+    permute4(v);
+    permutei(mol);
+    permutei(tag);
+    permutei(orig);
+    hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
+                       key2.as<uint32_t>(), cell_start.as<int>());
+    HIPCHK(hipMemcpyAsync(xhold.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+    build_lists();
+    mark_end(0, mk);
+    need_rebuild = false;
+    ago = 0;
+  }
+
+  void permute4(DBuf &b) {
+    hipLaunchKernelGGL(k_permute<double4>, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, perm.as<int>(),
+                       b.as<double4>(), tmp4.as<double4>());
+    HIPCHK(hipMemcpyAsync(b.p, tmp4.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+  }
+  void permutei(DBuf &b) {
+    hipLaunchKernelGGL(k_permute<int>, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, perm.as<int>(),
+                       b.as<int>(), tmpi.as<int>());
+    HIPCHK(hipMemcpyAsync(b.p, tmpi.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+  }
+
+  void build_lists() {
+    for (int attempt = 0; attempt < 4; attempt++) {
+      slist.ensure((size_t)n * scap * sizeof(uint32_t));
+      ilist.ensure((size_t)n * std::max(icap, 1) * sizeof(uint32_t));
+      HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int) * 4, stream));
+      auto P = ptrs<double>();
+      const int nb = (int)((n + 3) / 4);
+      hipLaunchKernelGGL(k_build_lists<double>, dim3(nb), dim3(BS), 0, stream, P, (const int *)cell_start.as<int>());
+      int o[4];
+      HIPCHK(hipMemcpyAsync(o, ovf.p, sizeof o, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      bool grow = false;
+      if (o[0] > 0) {
+        scap = ((int)(o[0] * 1.25) + 63) / 64 * 64;
+        grow = true;
+      }
+      if (o[1] > 0 || icap == 0) {
+        const int need = std::max(o[1], o[2]);
+        icap = ((int)(need * 1.15) + 64 + 63) / 64 * 64;
+        grow = true;
+      }
+      if (!grow) return;
+      gen++;  // list pointers / capacities change: captured graphs are stale
+    }
+    fail(LMFF_ERR_MEMORY, "neighbor list capacity did not converge");
+  }
+
+  // ---------------------------------------------------------------- force evaluation
+  template <typename R>
+  void force_impl(int ef, int vf, int va) {
+    auto P = ptrs<R>();
+    const int nb1 = (int)((n + BS - 1) / BS);
+    const int nbw = (int)((n + 3) / 4);
+    size_t mk = 0;
+    mark_begin(1, mk);
+    if (vf || va) hipLaunchKernelGGL((k_normals<R, true>), dim3(nb1), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_normals<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    mark_end(1, mk);
+    mark_begin(2, mk);
+    if (va) hipLaunchKernelGGL((k_ilp<R, true, true, true>), dim3(nbw), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_ilp<R, true, true, false>), dim3(nbw), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_ilp<R, true, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_ilp<R, false, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
+    mark_end(2, mk);
+    mark_begin(3, mk);
+    if (ef || vf || va) hipLaunchKernelGGL((k_tersoff_zeta<R, true>), dim3(nb1), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_tersoff_zeta<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    mark_end(3, mk);
+    mark_begin(4, mk);
+    if (va) hipLaunchKernelGGL((k_gather<R, true, true, true>), dim3(nb1), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_gather<R, true, true, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_gather<R, false, false, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    mark_end(4, mk);
+  }
+
+  void force(int eflag, int vflag) {
+    if (n == 0) return;
+    const int va = (vflag & 4) ? 1 : 0;
+    const int vf = ((vflag % 4) != 0 || va) ? 1 : 0;
+    const int ef = (eflag != 0 || vf) ? 1 : 0;
+    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va);
+    else force_impl<double>(ef, vf, va);
+    last_eflag = ef ? (eflag | 1) : 0;
+    last_vflag = va ? 4 | 2 : (vf ? 2 : 0);
+    forces_valid = true;
+  }
+
+  void check_errors() {
+    int e[4];
+    HIPCHK(hipMemcpyAsync(e, errw.p, sizeof e, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (e[0] == 0) return;
+    HIPCHK(hipMemset(errw.p, 0, sizeof(int) * 4));
+    if (e[0] & EB_NONFINITE) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords - simulation unstable");
+    if (e[0] & EB_NORMAL_COUNT)
+      fail(LMFF_ERR_NORMAL, "There are too many neighbors for some atoms, please check your configuration");
+    if (e[0] & EB_NORMAL_ZERO) fail(LMFF_ERR_NORMAL, "The magnitude of the normal vector is zero");
+    if (e[0] & EB_SHORT) fail(LMFF_ERR_SHORT, "more than %d Tersoff neighbours within cutmax", MAXSH);
+  }
+
+  // ---------------------------------------------------------------- thermo
+  // columns: 0 e_vdw 1 e_rep 2 e_tersoff 3-8 virial 9 ke
+  void reduce(int ncols, double *dst_dev) {
+    if (nchunks == 0) return;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(BS), 0, stream, (const double *)th.p, (size_t)n, ncols,
+                       chunks.as<Chunk>(), partial.as<double2>());
+    hipLaunchKernelGGL(k_reduce_systems, dim3((unsigned)sys.size()), dim3(64), 0, stream, (const double2 *)partial.p,
+                       ncols, cbeg.as<int>(), (int)sys.size(), dst_dev);
+  }
+
+  void zero_ke() { HIPCHK(hipMemsetAsync(th.as<double>() + 9 * (size_t)n, 0, n * sizeof(double), stream)); }
+
+  // ---------------------------------------------------------------- stepping
+  void initial_integrate() {
+    hipLaunchKernelGGL(k_initial_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, x.as<double4>(),
+                       v.as<double4>(), (const double4 *)f.p, sysid.as<int>(), systab.as<SysDev>(),
+                       rmass.as<double>());
+  }
+  void final_integrate(int want_ke) {
+    hipLaunchKernelGGL(k_final_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n,
+                       (const double4 *)x.p, v.as<double4>(), f.as<double4>(), sysid.as<int>(), tag.as<int>(),
+                       systab.as<SysDev>(), rmass.as<double>(), th.as<double>() + 9 * (size_t)n, want_ke);
+    hipLaunchKernelGGL(k_advance_step, dim3(((int)sys.size() + 63) / 64), dim3(64), 0, stream, (int)sys.size(),
+                       systab.as<SysDev>());
+  }
+  void plain_step_body() {
+    size_t mk = 0;
+    mark_begin(5, mk);
+    initial_integrate();
+    mark_end(5, mk);
+    force(0, 0);
+    mark_begin(5, mk);
+    final_integrate(0);
+    mark_end(5, mk);
+  }
+
+  bool rebuild_due() {
+    // Neighbor::decide (neighbor.cpp:1947-1962) with ago already incremented
+    if (!(ago >= delay && ago % every == 0)) return false;
+    if (!check) return true;
+    DBuf &flag = dflag;
+    flag.ensure(sizeof(int));
+    HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), stream));
+    const double trig = 0.5 * skin;
+    hipLaunchKernelGGL(k_displacement, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, (const double4 *)x.p,
+                       (const double4 *)xhold.p, sysid.as<int>(), trig * trig, flag.as<int>());
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return h != 0;
+  }
+
+  void capture_graph(int len) {
+    if (graph) {
+      (void)hipGraphExecDestroy(graph);
+      graph = nullptr;
+    }
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < len; k++) plain_step_body();
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    HIPCHK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    graph_len = len;
+    graph_gen = gen;
+    graph_prec = precision;
+  }
+
+  void evaluate(int eflag, int vflag) {
+    relayout();
+    if (need_rebuild) rebuild();
+    force(eflag, vflag);
+    if (eflag || vflag) {
+      if (eflag || (vflag % 4)) reduce(9, sums.as<double>());
+    }
+    check_errors();
+  }
+
+  void run(int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out, int64_t out_cap) {
+    relayout();
+    if (n == 0) return;
+    if (need_rebuild || !forces_valid) {
+      if (need_rebuild) rebuild();
+      force(0, 0);
+    }
+    const int nsys = (int)sys.size();
+    int64_t nrec = 0;
+    std::vector<int64_t> rec_steps;
+    for (int64_t s = 1; s <= nsteps; s++)
+      if ((thermo_every > 0 && s % thermo_every == 0) || s == nsteps) rec_steps.push_back(s);
+    DBuf recs;
+    recs.ensure(sizeof(double) * 16 * std::max<size_t>(rec_steps.size(), 1) * std::max(nsys, 1));
+    std::vector<long long> base_step(nsys);
+    for (int k = 0; k < nsys; k++) base_step[k] = sys[k].step;
+    if (prof) {
+      ev_used = 0;
+      ev_marks.clear();
+      for (auto &m : prof_ms) m = 0;
+      for (auto &c : prof_cnt) c = 0;
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventRecord(t0, stream));
+    int64_t step = 0;
+    size_t ri = 0;
+    while (step < nsteps) {
+      const int64_t s = step + 1;
+      const bool thermo = ri < rec_steps.size() && rec_steps[ri] == s;
+      // can the next block of steps be replayed from a graph?  (no rebuild, no thermo)
+      if (use_graph && !prof && !check) {
+        // steps s .. s+len-1 are plain when ago+1 .. ago+len are not rebuild steps
+        int len = 0;
+        while (s + len <= nsteps) {
+          const int ago_k = ago + len + 1;
+          const bool rb = (ago_k >= delay && ago_k % every == 0);
+          const bool th_k = ri < rec_steps.size() && rec_steps[ri] == s + len;
+          if (rb || th_k) break;
+          len++;
+        }
+        if (len >= 2) {
+          if (!graph || graph_gen != gen || graph_prec != precision) capture_graph(1);
+          for (int k = 0; k < len; k++) HIPCHK(hipGraphLaunch(graph, stream));
+          ago += len;
+          step += len;
+          global_step += len;
+          continue;
+        }
+      }
+      size_t mk = 0;
+      mark_begin(5, mk);
+      initial_integrate();
+      mark_end(5, mk);
+      ago++;
+      if (rebuild_due()) rebuild();
+      if (thermo) {
+        force(1, 2);
+        mark_begin(5, mk);
+        final_integrate(1);
+        mark_end(5, mk);
+        mark_begin(6, mk);
+        reduce(10, recs.as<double>() + (size_t)ri * nsys * 16);
+        mark_end(6, mk);
+        ri++;
+      } else {
+        force(0, 0);
+        mark_begin(5, mk);
+        final_integrate(0);
+        mark_end(5, mk);
+      }
+      step++;
+      global_step++;
+    }
+    HIPCHK(hipEventRecord(t1, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    prof_total = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (prof) {
+      for (auto &m : ev_marks) {
+        float e = 0;
+        HIPCHK(hipEventElapsedTime(&e, ev_pool[m.second.first], ev_pool[m.second.second]));
+        prof_ms[m.first] += e;
+        prof_cnt[m.first]++;
+      }
+    }
+    check_errors();
+    sync_steps_from_device();
+    // thermo records (reduce() wrote ncols=10 values per system at stride 10; copy out)
+    nrec = (int64_t)rec_steps.size();
+    if (out && nrec) {
+      std::vector<double> h((size_t)nrec * nsys * 16);
+      HIPCHK(hipMemcpy(h.data(), recs.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+      for (int k = 0; k < nsys; k++)
+        for (int64_t r = 0; r < nrec; r++) {
+          if ((int64_t)k * nrec + r >= out_cap) continue;
+          const double *c = &h[(size_t)r * nsys * 16 + (size_t)k * 10];
+          lmff_thermo &o = out[(size_t)k * nrec + r];
+          o.step = base_step[k] + rec_steps[r];
+          o.e_vdw = c[0];
+          o.e_rep = c[1];
+          o.e_tersoff = c[2];
+          o.pe = c[0] + c[1] + c[2];
+          for (int q = 0; q < 6; q++) o.virial[q] = c[3 + q];
+          o.ke = c[9];
+          const double dof = 3.0 * (double)sys[k].n - 3.0;
+          o.temp = dof > 0 ? 2.0 * o.ke / (dof * BOLTZ) : 0.0;
+        }
+    }
+  }
+
+  // ---------------------------------------------------------------- downloads
+  template <typename T>
+  std::vector<T> download(const DBuf &b, size_t off, size_t count) {
+    std::vector<T> h(count);
+    if (count) HIPCHK(hipMemcpy(h.data(), (const T *)b.p + off, count * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+  }
+};
+
+int guard(lmff_ctx *c, const std::function<void()> &fn);
+
+}  // namespace
+
+struct lmff_ctx {
+  Ctx c;
+};
+
+namespace {
+
+int guard(lmff_ctx *ctx, const std::function<void()> &fn) {
+  if (!ctx) return LMFF_ERR_ARG;
+  try {
+    fn();
+    ctx->c.err.clear();
+    return LMFF_OK;
+  } catch (const LmffError &e) {
+    ctx->c.err = e.msg;
+    return e.code;
+  } catch (const HipError &e) {
+    ctx->c.err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return LMFF_ERR_HIP;
+  } catch (const std::bad_alloc &) {
+    ctx->c.err = "host allocation failed";
+    return LMFF_ERR_MEMORY;
+  }
+}
+
+void check_box(const double lo[3], const double hi[3], const int per[3], double cutneigh) {
+  for (int d = 0; d < 3; d++) {
+    if (!(hi[d] > lo[d])) fail(LMFF_ERR_BOX, "box dimension %d is empty", d);
+    if (per[d] && hi[d] - lo[d] <= 2.0 * cutneigh)
+      fail(LMFF_ERR_BOX, "periodic box side %d (%.3f) must exceed twice the neighbor cutoff (%.3f)", d, hi[d] - lo[d],
+           cutneigh);
+  }
+}
+
+void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
+  const double half = 0.5 * cutneigh;
+  s.ncells = 1;
+  for (int d = 0; d < 3; d++) {
+    if (s.per[d]) {
+      const double L = s.hi[d] - s.lo[d];
+      s.nc[d] = std::max(1, (int)(L / half));
+      s.glo[d] = s.lo[d];
+      s.bs[d] = L / s.nc[d];
+    } else {
+      double mn = 1e300, mx = -1e300;
+      for (int64_t i = 0; i < n; i++) {
+        mn = std::min(mn, x[3 * i + d]);
+        mx = std::max(mx, x[3 * i + d]);
+      }
+      if (n == 0) mn = mx = 0.0;
+      const double ext = mx - mn;
+      s.nc[d] = std::max(1, (int)(ext / half));
+      s.glo[d] = mn;
+      s.bs[d] = ext > 0 ? ext / s.nc[d] : half;
+      if (s.bs[d] < half) s.bs[d] = half;
+    }
+    s.ncells *= s.nc[d];
+  }
+}
+
+}  // namespace
+
+// ======================================================================== C ABI
+
+extern "C" {
+
+const char *lmff_version(void) { return "lmff-mi355x 0.1 (gfx950)"; }
+
+int lmff_create(lmff_ctx **out, int device) {
+  if (!out) return LMFF_ERR_ARG;
+  *out = nullptr;
+  lmff_ctx *ctx = new lmff_ctx();
+  int rc = guard(ctx, [&] {
+    ctx->c.device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+  });
+  if (rc) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return LMFF_OK;
+}
+
+void lmff_destroy(lmff_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  delete ctx;
+}
+
+const char *lmff_last_error(const lmff_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int lmff_settings(lmff_ctx *ctx, int ntypes, double ilp_cut_global, int tap_flag) {
+  return guard(ctx, [&] {
+    if (ntypes < 1 || ntypes > MAXT) fail(LMFF_ERR_ARG, "ntypes must be 1..%d", MAXT);
+    if (!(ilp_cut_global > 0)) fail(LMFF_ERR_ARG, "Illegal pair_style command");
+    auto &c = ctx->c;
+    if (!c.sys.empty() && ntypes != c.ntypes) fail(LMFF_ERR_STATE, "cannot change ntypes with resident systems");
+    c.ntypes = ntypes;
+    c.ilp_cut = ilp_cut_global;
+    c.tap_flag = tap_flag ? 1 : 0;
+    c.have_settings = true;
+    c.params_dirty = true;
+  });
+}
+
+int lmff_coeff_tersoff(lmff_ctx *ctx, const char *file, const char *const *elements) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!c.have_settings) fail(LMFF_ERR_ARG, "lmff_settings must precede pair_coeff");
+    if (!file || !elements) fail(LMFF_ERR_ARG, "Incorrect args for pair coefficients");
+    TersoffTable t;
+    std::string err;
+    if (!map_elements(c.ntypes, elements, t.emap, err)) fail(LMFF_ERR_ARG, "%s", err.c_str());
+    int rc = read_tersoff(file, t, err);
+    if (rc) fail(rc, "%s", err.c_str());
+    c.ters = std::move(t);
+    c.have_tersoff = true;
+    c.params_dirty = true;
+  });
+}
+
+int lmff_coeff_ilp(lmff_ctx *ctx, const char *file, const char *const *elements) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!c.have_settings) fail(LMFF_ERR_ARG, "lmff_settings must precede pair_coeff");
+    if (!file || !elements) fail(LMFF_ERR_ARG, "Incorrect args for pair coefficients");
+    IlpTable t;
+    std::string err;
+    if (!map_elements(c.ntypes, elements, t.emap, err)) fail(LMFF_ERR_ARG, "%s", err.c_str());
+    int rc = read_ilp(file, t, err);
+    if (rc) fail(rc, "%s", err.c_str());
+    c.ilp = std::move(t);
+    c.have_ilp = true;
+    c.params_dirty = true;
+  });
+}
+
+int lmff_neighbor(lmff_ctx *ctx, double skin, int every, int delay, int check) {
+  return guard(ctx, [&] {
+    if (skin < 0 || every < 1 || delay < 0) fail(LMFF_ERR_ARG, "Illegal neighbor command");
+    auto &c = ctx->c;
+    if (!c.sys.empty() && skin != c.skin) fail(LMFF_ERR_STATE, "cannot change skin with resident systems");
+    c.skin = skin;
+    c.every = every;
+    c.delay = delay;
+    c.check = check ? 1 : 0;
+    c.params_dirty = true;
+  });
+}
+
+int lmff_precision(lmff_ctx *ctx, int mode) {
+  return guard(ctx, [&] {
+    if (mode != LMFF_FP64 && mode != LMFF_MIXED) fail(LMFF_ERR_ARG, "precision must be LMFF_FP64 or LMFF_MIXED");
+    ctx->c.precision = mode;
+    ctx->c.forces_valid = false;
+  });
+}
+
+int lmff_masses(lmff_ctx *ctx, const double *m) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!c.have_settings || !m) fail(LMFF_ERR_ARG, "masses need lmff_settings first");
+    c.mass.assign(m, m + c.ntypes);
+    for (double q : c.mass)
+      if (!(q > 0)) fail(LMFF_ERR_ARG, "Invalid mass value");
+    c.params_dirty = true;
+  });
+}
+
+int lmff_system_add(lmff_ctx *ctx, int64_t natoms, const double boxlo[3], const double boxhi[3], const int periodic[3],
+                    const int32_t *type, const int64_t *tag, const int64_t *mol, const double *x, const double *v,
+                    int *sys_id) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    c.upload_params();
+    if (natoms < 0 || (natoms > 0 && (!type || !tag || !mol || !x))) fail(LMFF_ERR_ARG, "bad system arrays");
+    if (!boxlo || !boxhi || !periodic) fail(LMFF_ERR_ARG, "bad box");
+    check_box(boxlo, boxhi, periodic, c.cutneigh);
+    SysHost s{};
+    s.id = c.next_id++;
+    s.live = true;
+    s.n = natoms;
+    for (int d = 0; d < 3; d++) {
+      s.lo[d] = boxlo[d];
+      s.hi[d] = boxhi[d];
+      s.per[d] = periodic[d] ? 1 : 0;
+    }
+    Ctx::Staged st;
+    st.id = s.id;
+    st.x.assign(x, x + 3 * natoms);
+    st.v.assign(3 * natoms, 0.0);
+    if (v) st.v.assign(v, v + 3 * natoms);
+    st.type.resize(natoms);
+    st.tag.resize(natoms);
+    st.mol.resize(natoms);
+    for (int64_t i = 0; i < natoms; i++) {
+      if (type[i] < 1 || type[i] > c.ntypes) fail(LMFF_ERR_ARG, "Invalid atom type %d", (int)type[i]);
+      if (tag[i] <= 0 || tag[i] > INT32_MAX) fail(LMFF_ERR_ARG, "atom IDs must be in 1..2^31-1");
+      if (mol[i] < INT32_MIN || mol[i] > INT32_MAX) fail(LMFF_ERR_ARG, "molecule IDs must fit in 32 bits");
+      for (int d = 0; d < 3; d++)
+        if (!std::isfinite(x[3 * i + d])) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords");
+      st.type[i] = type[i];
+      st.tag[i] = (int)tag[i];
+      st.mol[i] = (int)mol[i];
+    }
+    setup_grid(s, x, natoms, c.cutneigh);
+    c.sys.push_back(s);
+    c.staged.push_back(std::move(st));
+    c.layout_dirty = true;
+    if (sys_id) *sys_id = s.id;
+  });
+}
+
+int lmff_system_remove(lmff_ctx *ctx, int sys_id) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    // resident data of the other systems must be current before re-packing
+    c.relayout();
+    s = c.find(sys_id);
+    s->live = false;
+    c.layout_dirty = true;
+    if (c.pair_sys == sys_id) c.pair_sys = -1;
+  });
+}
+
+int lmff_system_integrator(lmff_ctx *ctx, int sys_id, int style, double dt, double t_target, double damp,
+                           uint64_t seed) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    if (style != LMFF_NVE && style != LMFF_LANGEVIN) fail(LMFF_ERR_ARG, "unknown integrator style");
+    if (!(dt > 0)) fail(LMFF_ERR_ARG, "timestep must be positive");
+    if (style == LMFF_LANGEVIN && (!(damp > 0) || t_target < 0)) fail(LMFF_ERR_ARG, "Illegal fix langevin command");
+    s->style = style;
+    s->dt = dt;
+    s->t_target = t_target;
+    s->damp = damp;
+    s->seed = seed;
+    if (!c.layout_dirty) {
+      c.sync_steps_from_device();
+      c.upload_systab();
+    }
+    c.gen++;
+  });
+}
+
+int lmff_system_natoms(lmff_ctx *ctx, int sys_id, int64_t *natoms) {
+  return guard(ctx, [&] {
+    SysHost *s = ctx->c.find(sys_id);
+    if (!s || !natoms) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    *natoms = s->n;
+  });
+}
+
+int lmff_system_get(lmff_ctx *ctx, int sys_id, double *xo, double *vo, double *fo) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    c.relayout();
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    auto put = [&](const DBuf &b, double *o) {
+      if (!o) return;
+      auto h = c.download<double4>(b, s->begin, m);
+      for (size_t q = 0; q < m; q++) {
+        const size_t i = (size_t)org[q];
+        o[3 * i] = h[q].x;
+        o[3 * i + 1] = h[q].y;
+        o[3 * i + 2] = h[q].z;
+      }
+    };
+    put(c.x, xo);
+    put(c.v, vo);
+    put(c.f, fo);
+  });
+}
+
+int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *xi, const double *vi) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    c.relayout();
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    auto upd = [&](DBuf &b, const double *in) {
+      if (!in) return;
+      auto h = c.download<double4>(b, s->begin, m);
+      for (size_t q = 0; q < m; q++) {
+        const size_t i = (size_t)org[q];
+        h[q].x = in[3 * i];
+        h[q].y = in[3 * i + 1];
+        h[q].z = in[3 * i + 2];
+      }
+      HIPCHK(hipMemcpy(b.as<double4>() + s->begin, h.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+    };
+    upd(c.x, xi);
+    upd(c.v, vi);
+    if (xi) {
+      c.need_rebuild = true;
+      c.forces_valid = false;
+    }
+  });
+}
+
+int lmff_system_peratom(lmff_ctx *ctx, int sys_id, double *eatom, double *vatom) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s || c.layout_dirty) fail(LMFF_ERR_STATE, "no evaluated system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    if (eatom) {
+      if (!(c.last_eflag & 2) && !(c.last_eflag & 1)) fail(LMFF_ERR_STATE, "last evaluation had no energy");
+      auto e0 = c.download<double>(c.th, 0 * (size_t)c.n + s->begin, m);
+      auto e1 = c.download<double>(c.th, 1 * (size_t)c.n + s->begin, m);
+      auto e2 = c.download<double>(c.th, 2 * (size_t)c.n + s->begin, m);
+      for (size_t q = 0; q < m; q++) eatom[org[q]] = e0[q] + e1[q] + e2[q];
+    }
+    if (vatom) {
+      if (!(c.last_vflag & 4)) fail(LMFF_ERR_STATE, "last evaluation had no per-atom virial");
+      for (int k = 0; k < 6; k++) {
+        auto col = c.download<double>(c.vatom, (size_t)k * c.n + s->begin, m);
+        for (size_t q = 0; q < m; q++) vatom[6 * (size_t)org[q] + k] = col[q];
+      }
+    }
+  });
+}
+
+int lmff_evaluate(lmff_ctx *ctx, int eflag, int vflag) {
+  return guard(ctx, [&] { ctx->c.evaluate(eflag, vflag); });
+}
+
+int lmff_system_thermo(lmff_ctx *ctx, int sys_id, lmff_thermo *out) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!out) fail(LMFF_ERR_ARG, "null output");
+    int k = -1;
+    for (size_t q = 0; q < c.sys.size(); q++)
+      if (c.sys[q].id == sys_id && c.sys[q].live) k = (int)q;
+    if (k < 0 || c.layout_dirty) fail(LMFF_ERR_STATE, "no evaluated system %d", sys_id);
+    // ke of the current velocities
+    const SysHost &s = c.sys[k];
+    const size_t m = (size_t)(s.end - s.begin);
+    auto hv = c.download<double4>(c.v, s.begin, m);
+    auto hx = c.download<double4>(c.x, s.begin, m);
+    double ke = 0;
+    for (size_t q = 0; q < m; q++) {
+      const double mass = c.mass[(int)hx[q].w - 1];
+      ke += mass * (hv[q].x * hv[q].x + hv[q].y * hv[q].y + hv[q].z * hv[q].z);
+    }
+    ke *= 0.5 * MVV2E;
+    double h[9];
+    HIPCHK(hipMemcpy(h, c.sums.as<double>() + (size_t)k * 9, sizeof h, hipMemcpyDeviceToHost));
+    out->step = s.step;
+    out->e_vdw = h[0];
+    out->e_rep = h[1];
+    out->e_tersoff = h[2];
+    out->pe = h[0] + h[1] + h[2];
+    for (int q = 0; q < 6; q++) out->virial[q] = h[3 + q];
+    out->ke = ke;
+    const double dof = 3.0 * (double)s.n - 3.0;
+    out->temp = dof > 0 ? 2.0 * ke / (dof * BOLTZ) : 0.0;
+  });
+}
+
+int lmff_run(lmff_ctx *ctx, int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out,
+             int64_t out_capacity) {
+  return guard(ctx, [&] {
+    if (nsteps < 0) fail(LMFF_ERR_ARG, "Illegal run command");
+    ctx->c.run(nsteps, thermo_every, use_graph, out, out_capacity);
+  });
+}
+
+int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int32_t *type, const int64_t *tag,
+                      const int64_t *mol, const double boxlo[3], const double boxhi[3], const int periodic[3],
+                      int new_list, int eflag, int vflag, double *f, double *eng_vdwl, double virial[6],
+                      double *eatom, double *vatom, double pvector[2]) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (nlocal < 0 || (nlocal > 0 && !x)) fail(LMFF_ERR_ARG, "bad atom arrays");
+    const bool fresh = new_list || c.pair_sys < 0 || c.pair_n != nlocal || !c.find(c.pair_sys);
+    if (fresh) {
+      if (c.pair_sys >= 0 && c.find(c.pair_sys)) {
+        c.find(c.pair_sys)->live = false;
+        c.layout_dirty = true;
+      }
+      int id = -1;
+      int rc = lmff_system_add(ctx, nlocal, boxlo, boxhi, periodic, type, tag, mol, x, nullptr, &id);
+      if (rc) throw LmffError{rc, c.err};
+      c.pair_sys = id;
+      c.pair_n = nlocal;
+      c.relayout();
+    } else {
+      SysHost *s = c.find(c.pair_sys);
+      const size_t m = (size_t)(s->end - s->begin);
+      auto org = c.download<int>(c.orig, s->begin, m);
+      auto h = c.download<double4>(c.x, s->begin, m);
+      for (size_t q = 0; q < m; q++) {
+        const size_t i = (size_t)org[q];
+        h[q].x = x[3 * i];
+        h[q].y = x[3 * i + 1];
+        h[q].z = x[3 * i + 2];
+      }
+      HIPCHK(hipMemcpy(c.x.as<double4>() + s->begin, h.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+    }
+    c.evaluate(eflag, vflag);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    SysHost *s = c.find(c.pair_sys);
+    int k = 0;
+    for (size_t q = 0; q < c.sys.size(); q++)
+      if (c.sys[q].id == c.pair_sys) k = (int)q;
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    if (f) {
+      auto h = c.download<double4>(c.f, s->begin, m);
+      for (size_t q = 0; q < m; q++) {
+        const size_t i = (size_t)org[q];
+        f[3 * i] += h[q].x;
+        f[3 * i + 1] += h[q].y;
+        f[3 * i + 2] += h[q].z;
+      }
+    }
+    double h9[9] = {0};
+    if (eflag || (vflag % 4)) HIPCHK(hipMemcpy(h9, c.sums.as<double>() + (size_t)k * 9, sizeof h9, hipMemcpyDeviceToHost));
+    if (eng_vdwl) *eng_vdwl = (eflag & 1) ? h9[0] + h9[1] + h9[2] : 0.0;
+    if (pvector) {
+      pvector[0] = (eflag & 1) ? h9[0] : 0.0;
+      pvector[1] = (eflag & 1) ? h9[1] : 0.0;
+    }
+    if (virial && (vflag % 4))
+      for (int q = 0; q < 6; q++) virial[q] = h9[3 + q];
+    if (eatom && (eflag & 2)) {
+      auto e0 = c.download<double>(c.th, 0 * (size_t)c.n + s->begin, m);
+      auto e1 = c.download<double>(c.th, 1 * (size_t)c.n + s->begin, m);
+      auto e2 = c.download<double>(c.th, 2 * (size_t)c.n + s->begin, m);
+      for (size_t q = 0; q < m; q++) eatom[org[q]] += e0[q] + e1[q] + e2[q];
+    }
+    if (vatom && (vflag & 4)) {
+      for (int kk = 0; kk < 6; kk++) {
+        auto col = c.download<double>(c.vatom, (size_t)kk * c.n + s->begin, m);
+        for (size_t q = 0; q < m; q++) vatom[6 * (size_t)org[q] + kk] += col[q];
+      }
+    }
+  });
+}
+
+int lmff_profile(lmff_ctx *ctx, int enable) {
+  return guard(ctx, [&] { ctx->c.prof = enable != 0; });
+}
+
+int lmff_profile_read(lmff_ctx *ctx, double *ms_total, double *ms_per_family, int64_t *launches) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (ms_total) *ms_total = c.prof_total;
+    for (int k = 0; k < 7; k++) {
+      if (ms_per_family) ms_per_family[k] = c.prof_cnt[k] ? c.prof_ms[k] / c.prof_cnt[k] : 0.0;
+      if (launches) launches[k] = c.prof_cnt[k];
+    }
+  });
+}
+
+int lmff_memory(lmff_ctx *ctx, int64_t *bytes, double *per_atom) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (bytes) *bytes = (int64_t)c.bytes_in_use;
+    if (per_atom) {
+      const double fixed = 4 * 32 + 10 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + 3 * MAXSH * 8 + 64 + 48 +
+                           NTH * 8 + 48 + 32;
+      *per_atom = fixed + 4.0 * (c.scap + std::max(c.icap, 448));
+    }
+  });
+}
+
+int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint) {
+  if (!ctx) return -1;
+  auto &c = ctx->c;
+  const double fixed = 4 * 32 + 10 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + 3 * MAXSH * 8 + 64 + 48 +
+                       NTH * 8 + 48 + 32;
+  const int layers = std::max(nlayers_hint, 2);
+  const double list = 4.0 * (c.scap + (c.icap > 0 ? c.icap : 448 * std::min(layers - 1, 5)));
+  return (int64_t)(natoms * (fixed + list));
+}
+
+int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, int capacity, int *nout) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s || c.layout_dirty) fail(LMFF_ERR_STATE, "no evaluated system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    int64_t q = -1;
+    for (size_t p = 0; p < m; p++)
+      if (org[p] == atom) q = (int64_t)p;
+    if (q < 0) fail(LMFF_ERR_ARG, "atom out of range");
+    const size_t gq = (size_t)(s->begin + q);
+    int cnt = 0;
+    HIPCHK(hipMemcpy(&cnt, c.icnt.as<int>() + gq, sizeof(int), hipMemcpyDeviceToHost));
+    auto ent = c.download<uint32_t>(c.ilist, gq * (size_t)c.icap, (size_t)cnt);
+    auto orgall = c.download<int>(c.orig, 0, (size_t)c.n);
+    std::vector<std::array<int32_t, 4>> r;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = ent[k];
+      const int j = (int)(e & IDX_MASK);
+      r.push_back({orgall[j], (int32_t)((e >> IDX_BITS) & 3) - 1, (int32_t)((e >> (IDX_BITS + 2)) & 3) - 1,
+                   (int32_t)((e >> (IDX_BITS + 4)) & 3) - 1});
+    }
+    std::sort(r.begin(), r.end());
+    for (int k = 0; k < cnt && k < capacity; k++)
+      for (int d = 0; d < 4; d++) rows[4 * k + d] = r[k][d];
+    *nout = cnt;
+  });
+}
+
+int lmff_debug_normals(lmff_ctx *ctx, int sys_id, double *normal, int32_t *count) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s || c.layout_dirty) fail(LMFF_ERR_STATE, "no evaluated system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    auto nb = c.download<uint32_t>(c.nnb, (size_t)s->begin * 4, m * 4);
+    std::vector<double> nv(m * 3);
+    if (c.precision == LMFF_MIXED) {
+      auto h = c.download<float4>(c.nrm, s->begin, m);
+      for (size_t q = 0; q < m; q++) { nv[3 * q] = h[q].x; nv[3 * q + 1] = h[q].y; nv[3 * q + 2] = h[q].z; }
+    } else {
+      auto h = c.download<double4>(c.nrm, s->begin, m);
+      for (size_t q = 0; q < m; q++) { nv[3 * q] = h[q].x; nv[3 * q + 1] = h[q].y; nv[3 * q + 2] = h[q].z; }
+    }
+    for (size_t q = 0; q < m; q++) {
+      const size_t i = (size_t)org[q];
+      if (normal) for (int d = 0; d < 3; d++) normal[3 * i + d] = nv[3 * q + d];
+      if (count) count[i] = (int32_t)nb[4 * q];
+    }
+  });
+}
+
+}  // extern "C"

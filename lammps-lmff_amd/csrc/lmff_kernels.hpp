@@ -1,0 +1,1214 @@
+// HIP kernels of the LMFF force path for CDNA4 (gfx950, wave64).
+//
+// Layout in HBM (all per-atom arrays are in cell-sorted order, one packed array for every
+// resident system):
+//   x, v, f        double4 per atom (x.w holds the atom type as an exact double)
+//   mol, tag, sys  int32 per atom; orig = index of the atom in its system's caller order
+//   slist          [n][scap] uint32: every atom within max(Tersoff cutmax, ILP rcut)+skin
+//   ilist          [n][icap] uint32: other-molecule atoms within ILP cut + skin (the
+//                  layer-filtered ILP list)
+//   list entries   bits 0-25 atom index, bits 26-27/28-29/30-31 image shift+1 in x/y/z
+//   nrm, dn, dndri normal (R4), d n / d x_k [3 nbrs][3 comps][3 coords] (27 R),
+//                  d n / d x_i (9 R) -- R = double (FP64 mode) or float (mixed mode)
+//   nnb            [n][4] uint32: count + up to 3 normal neighbours
+//   tsh            [n][MAXSH] uint32: Tersoff neighbours within cutmax
+//   fz, pref, eb   [n][MAXSH] double: per-bond force_zeta scalar, prefactor, energy
+//   fil, S         double4: ILP pair force and S_i = sum_j w_ij del_ij
+//   th             [NTH][n] double columns: e_vdw, e_rep, e_tersoff, virial share (6), ke
+//
+// Forces are gathered ("pulled") by the atom they act on: no kernel writes another atom's
+// force, so results are deterministic and independent of launch order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+namespace lmff {
+
+constexpr int MAXT = 8;        // LAMMPS atom types supported on device
+constexpr int MAXSH = 8;       // Tersoff neighbours within cutmax per atom
+constexpr int IDX_BITS = 26;
+constexpr uint32_t IDX_MASK = (1u << IDX_BITS) - 1u;
+constexpr int NTH = 10;        // thermo columns
+constexpr double FTM2V = 1.0 / 1.0364269e-4;  // LMFF/update.cpp:170
+constexpr double MVV2E = 1.0364269e-4;
+constexpr double BOLTZ = 8.617343e-5;
+constexpr double MY_PI2 = 1.57079632679489661923;
+constexpr double MY_PI4 = 0.78539816339744830962;
+
+enum ErrBits : int {
+  EB_NORMAL_COUNT = 1,
+  EB_NORMAL_ZERO = 2,
+  EB_SHORT = 4,
+  EB_NONFINITE = 8,
+};
+
+struct SysDev {
+  double lo[3], prd[3];
+  int per[3];
+  int nc[3];
+  double glo[3], inv_bs[3];
+  int cell_off;
+  int begin, end;
+  double dtv, dtf;
+  int style;          // 0 = NVE, 1 = Langevin
+  double t_target, damp;
+  unsigned long long seed;
+  long long step;
+};
+
+template <typename R>
+struct IlpPair {
+  R z0, lambda, half_eps, C, d2inv;  // repulsion (normal of the first atom)
+  R C6, d, inv_seff, C6_d_seff;      // van der Waals
+  double rcutsq;                     // normal-neighbour cutoff^2 (cutILPsq)
+  int mapped;                        // both types mapped to ILP elements
+  int vsym;                          // vdW parameters equal for (ti,tj) and (tj,ti)
+  int esym;                          // lambda, z0 equal for (ti,tj) and (tj,ti)
+  int pad;
+};
+
+template <typename R>
+struct TersP {
+  R lam1, biga, lam2, bigb, bigr, bigd, cc, dd, h, gamma, powern, beta, c1, c2, c3, c4, lam3;
+  double cutsq;
+  int powermint, pad;
+};
+
+template <typename R>
+struct DevPtrs {
+  int n;
+  double4 *x, *v, *f;
+  int *mol, *tag, *sys;
+  const SysDev *sys_tab;
+  // lists
+  uint32_t *slist; int *scnt; int scap;
+  uint32_t *ilist; int *icnt; int icap;
+  // per step
+  typename std::conditional<sizeof(R) == 8, double4, float4>::type *nrm;
+  R *dn, *dndri;
+  double *vet;        // [n][9] double, positions of normal neighbours relative to the atom
+  uint32_t *nnb;      // [n][4]
+  uint32_t *tsh; int *tcnt;
+  double *fz, *pref, *eb;
+  double4 *fil, *S;
+  double *T;          // [n][6] (per-atom virial only)
+  double *th;         // [NTH][n]
+  double *vatom;      // [6][n]
+  // parameters
+  const IlpPair<R> *ip;   // [MAXT][MAXT], types 0-based
+  const TersP<R> *tp;     // [nparams]
+  const int *t3;          // [MAXT][MAXT][MAXT] -> tp index or -1
+  int *err;
+  int *ovf;               // [0] max short count, [1] max ILP count
+  double cutneighsq, shortsq, tersoff_cutmaxsq, ilp_cutsq, inv_rcut, rcut;
+  int tap_flag;
+};
+
+// ---------------------------------------------------------------------------------------
+// math helpers
+
+template <typename R> struct MathOps;
+template <> struct MathOps<double> {
+  static __device__ __forceinline__ double ex(double a) { return exp(a); }
+  static __device__ __forceinline__ double sq(double a) { return sqrt(a); }
+  static __device__ __forceinline__ double inv(double a) { return 1.0 / a; }
+  static __device__ __forceinline__ double sn(double a) { return sin(a); }
+  static __device__ __forceinline__ double cs(double a) { return cos(a); }
+  static __device__ __forceinline__ double pw(double a, double b) { return pow(a, b); }
+};
+template <> struct MathOps<float> {
+  static __device__ __forceinline__ float ex(float a) { return __expf(a); }
+  static __device__ __forceinline__ float sq(float a) { return __fsqrt_rn(a); }
+  static __device__ __forceinline__ float inv(float a) { return __builtin_amdgcn_rcpf(a); }
+  static __device__ __forceinline__ float sn(float a) { return sinf(a); }
+  static __device__ __forceinline__ float cs(float a) { return cosf(a); }
+  static __device__ __forceinline__ float pw(float a, float b) { return powf(a, b); }
+};
+
+__device__ __forceinline__ int e_idx(uint32_t e) { return (int)(e & IDX_MASK); }
+__device__ __forceinline__ int e_s(uint32_t e, int d) { return (int)((e >> (IDX_BITS + 2 * d)) & 3u) - 1; }
+__device__ __forceinline__ uint32_t e_make(int j, int sx, int sy, int sz) {
+  return (uint32_t)j | ((uint32_t)(sx + 1) << IDX_BITS) | ((uint32_t)(sy + 1) << (IDX_BITS + 2)) |
+         ((uint32_t)(sz + 1) << (IDX_BITS + 4));
+}
+
+// ghost coordinate of a list entry as LAMMPS forms it: x_j + pbc*prd (atom_vec_full.cpp:402-412)
+__device__ __forceinline__ void ghost(const double4 &xj, uint32_t e, const double *prd, double &gx, double &gy,
+                                      double &gz) {
+  gx = __dadd_rn(xj.x, (double)e_s(e, 0) * prd[0]);
+  gy = __dadd_rn(xj.y, (double)e_s(e, 1) * prd[1]);
+  gz = __dadd_rn(xj.z, (double)e_s(e, 2) * prd[2]);
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// half selection of a full list by tag parity (pair_tersoff.cpp:150-158, ilp.cpp:488-496):
+// true when the pair (i,j) is computed by j in the reference
+__device__ __forceinline__ bool tag_skip(int itag, int jtag, double xi, double yi, double zi, double xj, double yj,
+                                         double zj) {
+  if (itag > jtag) return ((itag + jtag) % 2) == 0;
+  if (itag < jtag) return ((itag + jtag) % 2) == 1;
+  if (zj < zi) return true;
+  if (zj == zi && yj < yi) return true;
+  if (zj == zi && yj == yi && xj < xi) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------------------
+// Rebuild: wrap (Domain::pbc, domain.cpp:501-560), cell key, sort, permute, cell ranges
+
+__global__ void k_wrap_key(int n, double4 *x, const int *sys, const SysDev *st, uint32_t *key, int *val, int *err) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const SysDev &s = st[sys[p]];
+  double4 xp = x[p];
+  double c[3] = {xp.x, xp.y, xp.z};
+  if (!isfinite(c[0]) || !isfinite(c[1]) || !isfinite(c[2])) atomicOr(err, EB_NONFINITE);
+  int cell[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (s.per[d]) {
+      double hi = s.lo[d] + s.prd[d];
+      if (c[d] < s.lo[d]) c[d] += s.prd[d];
+      if (c[d] >= hi) {
+        c[d] -= s.prd[d];
+        c[d] = c[d] > s.lo[d] ? c[d] : s.lo[d];
+      }
+    }
+    double f = floor((c[d] - s.glo[d]) * s.inv_bs[d]);
+    int ci = isfinite(f) ? (int)fmax(fmin(f, (double)(s.nc[d] - 1)), 0.0) : 0;
+    cell[d] = ci;
+  }
+  xp.x = c[0]; xp.y = c[1]; xp.z = c[2];
+  x[p] = xp;
+  key[p] = (uint32_t)(s.cell_off + (cell[2] * s.nc[1] + cell[1]) * s.nc[0] + cell[0]);
+  val[p] = p;
+}
+
+template <typename T>
+__global__ void k_permute(int n, const int *perm, const T *src, T *dst) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[perm[q]];
+}
+
+// cell_start[c] = first sorted position with key >= c, for c in [0, ncells]
+__global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > n) return;
+  int lo = (q == 0) ? 0 : (int)key[q - 1] + 1;
+  int hi = (q == n) ? ncells : (int)key[q];
+  for (int c = lo; c <= hi; c++) start[c] = q;
+}
+
+// ---------------------------------------------------------------------------------------
+// Neighbor build: one wavefront per atom, stencil of +-2 bins (bins of half the list
+// cutoff, LMFF/nbin_standard.cpp:53-112), criterion rsq <= (cut+skin)^2 evaluated in the
+// reference's operation order with no fused multiply-adds (npair_full_bin_ghost.cpp:97-102).
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *cell_start) {
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[i]];
+  const double4 xi = P.x[i];
+  const int mi = P.mol[i];
+  const int ti = (int)xi.w - 1;
+  int c[3];
+  {
+    double cc[3] = {xi.x, xi.y, xi.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double f = floor((cc[d] - s.glo[d]) * s.inv_bs[d]);
+      c[d] = (int)fmax(fmin(f, (double)(s.nc[d] - 1)), 0.0);
+    }
+  }
+  int ns = 0, ni = 0;
+  uint32_t *srow = P.slist + (size_t)i * P.scap;
+  uint32_t *irow = P.ilist + (size_t)i * P.icap;
+  for (int dz = -2; dz <= 2; dz++) {
+    int cz = c[2] + dz, sz = 0;
+    if (cz < 0 || cz >= s.nc[2]) {
+      if (!s.per[2]) continue;
+      int w = ((cz % s.nc[2]) + s.nc[2]) % s.nc[2];
+      sz = (cz - w) / s.nc[2];
+      cz = w;
+    }
+    for (int dy = -2; dy <= 2; dy++) {
+      int cy = c[1] + dy, sy = 0;
+      if (cy < 0 || cy >= s.nc[1]) {
+        if (!s.per[1]) continue;
+        int w = ((cy % s.nc[1]) + s.nc[1]) % s.nc[1];
+        sy = (cy - w) / s.nc[1];
+        cy = w;
+      }
+      const int rowbase = s.cell_off + (cz * s.nc[1] + cy) * s.nc[0];
+      // x runs: consecutive stencil bins with the same image shift form one atom range
+      int dx = -2;
+      while (dx <= 2) {
+        int cx = c[0] + dx, sx = 0;
+        if (cx < 0 || cx >= s.nc[0]) {
+          if (!s.per[0]) { dx++; continue; }
+          int w = ((cx % s.nc[0]) + s.nc[0]) % s.nc[0];
+          sx = (cx - w) / s.nc[0];
+          cx = w;
+        }
+        int cx_end = cx, dx_end = dx;
+        while (dx_end + 1 <= 2) {
+          int nx = c[0] + dx_end + 1;
+          int nsx = 0, nw = nx;
+          if (nx < 0 || nx >= s.nc[0]) {
+            if (!s.per[0]) break;
+            nw = ((nx % s.nc[0]) + s.nc[0]) % s.nc[0];
+            nsx = (nx - nw) / s.nc[0];
+          }
+          if (nsx != sx || nw != cx_end + 1) break;
+          cx_end = nw;
+          dx_end++;
+        }
+        const int a0 = cell_start[rowbase + cx], a1 = cell_start[rowbase + cx_end + 1];
+        const double shx = (double)sx * s.prd[0], shy = (double)sy * s.prd[1], shz = (double)sz * s.prd[2];
+        const bool selfimg = (sx == 0 && sy == 0 && sz == 0);
+        for (int base = a0; base < a1; base += 64) {
+          const int j = base + lane;
+          bool in_i = false, in_s = false;
+          if (j < a1 && !(selfimg && j == i)) {
+            const double4 xj = P.x[j];
+            const double gx = __dadd_rn(xj.x, shx), gy = __dadd_rn(xj.y, shy), gz = __dadd_rn(xj.z, shz);
+            const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
+            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(delx, delx), __dmul_rn(dely, dely)), __dmul_rn(delz, delz));
+            const int tj = (int)xj.w - 1;
+            in_i = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && P.ip[ti * MAXT + tj].mapped;
+            in_s = (rsq <= P.shortsq);
+          }
+          const uint32_t ent = e_make(j, sx, sy, sz);
+          const unsigned long long bi = __ballot(in_i), bs = __ballot(in_s);
+          const int pi = __popcll(bi & ((1ull << lane) - 1ull)), ps = __popcll(bs & ((1ull << lane) - 1ull));
+          if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
+          if (in_s && ns + ps < P.scap) srow[ns + ps] = ent;
+          ni += __popcll(bi);
+          ns += __popcll(bs);
+        }
+        dx = dx_end + 1;
+      }
+    }
+  }
+  if (lane == 0) {
+    P.icnt[i] = ni < P.icap ? ni : P.icap;
+    P.scnt[i] = ns < P.scap ? ns : P.scap;
+    if (ni > P.icap) atomicMax(&P.ovf[1], ni);
+    if (ns > P.scap) atomicMax(&P.ovf[0], ns);
+    atomicMax(&P.ovf[2], ni);
+    atomicMax(&P.ovf[3], ns);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Per step 1: Tersoff neighbours within cutmax, ILP normal neighbours and the normal with its
+// derivatives (ILP_neigh + calc_normal, pair_ilp_graphene_hbn.cpp:681-1075).  One thread/atom.
+
+template <typename R, bool WANT_VET>
+__global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
+  using M = MathOps<R>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[i]];
+  const double4 xi = P.x[i];
+  const int ti = (int)xi.w - 1;
+  const int mi = P.mol[i];
+  const int cnt = P.scnt[i];
+  const uint32_t *row = P.slist + (size_t)i * P.scap;
+  int ns = 0, nn = 0;
+  uint32_t nb[3] = {0, 0, 0};
+  double vetd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  bool too_many = false;
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = row[k];
+    const int j = e_idx(e);
+    const double4 xj = P.x[j];
+    double gx, gy, gz;
+    ghost(xj, e, s.prd, gx, gy, gz);
+    const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+    const double rsq = delx * delx + dely * dely + delz * delz;
+    const int tj = (int)xj.w - 1;
+    if (rsq < P.tersoff_cutmaxsq && P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0) {
+      if (ns < MAXSH) P.tsh[(size_t)i * MAXSH + ns] = e;
+      ns++;
+    }
+    const IlpPair<R> &pp = P.ip[ti * MAXT + tj];
+    if (pp.mapped && P.mol[j] == mi && rsq != 0.0 && rsq < pp.rcutsq) {
+      if (nn < 3) {
+        nb[nn] = e;
+        vetd[nn][0] = gx - xi.x;
+        vetd[nn][1] = gy - xi.y;
+        vetd[nn][2] = gz - xi.z;
+      } else {
+        too_many = true;
+      }
+      nn++;
+    }
+  }
+  if (ns > MAXSH) {
+    atomicOr(P.err, EB_SHORT);
+    ns = MAXSH;
+  }
+  P.tcnt[i] = ns;
+  if (too_many) {
+    atomicOr(P.err, EB_NORMAL_COUNT);
+    nn = 3;
+  }
+  uint32_t *nrow = P.nnb + (size_t)i * 4;
+  nrow[0] = (uint32_t)nn;
+  nrow[1] = nb[0];
+  nrow[2] = nb[1];
+  nrow[3] = nb[2];
+  if (WANT_VET) {
+    double *vo = P.vet + (size_t)i * 9;
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 3; d++) vo[k * 3 + d] = vetd[k][d];
+  }
+  R vet[3][3];
+  for (int k = 0; k < 3; k++)
+    for (int d = 0; d < 3; d++) vet[k][d] = (R)vetd[k][d];
+  R n1[3], normal[3], dn1[3][3][3];
+  R *dno = P.dn + (size_t)i * 27;
+  if (nn <= 1 || !P.ip[ti * MAXT + ti].mapped) {
+    P.nrm[i] = {0, 0, 1, 0};
+    for (int q = 0; q < 27; q++) dno[q] = 0;
+    nrow[0] = (uint32_t)(nn <= 1 ? nn : 0);
+    return;
+  }
+  R nn2, nrm_len;
+  if (nn == 2) {
+    // pv12 = v1 x v2; n depends on x_i (ilp.cpp:835-927)
+    n1[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
+    n1[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
+    n1[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
+    R dpvdri[3][3];
+    dpvdri[0][0] = 0; dpvdri[0][1] = vet[0][2] - vet[1][2]; dpvdri[0][2] = vet[1][1] - vet[0][1];
+    dpvdri[1][0] = vet[1][2] - vet[0][2]; dpvdri[1][1] = 0; dpvdri[1][2] = vet[0][0] - vet[1][0];
+    dpvdri[2][0] = vet[0][1] - vet[1][1]; dpvdri[2][1] = vet[1][0] - vet[0][0]; dpvdri[2][2] = 0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) dn1[a][b][2] = 0;
+    dn1[0][0][0] = 0;          dn1[0][1][0] = vet[1][2];  dn1[0][2][0] = -vet[1][1];
+    dn1[1][0][0] = -vet[1][2]; dn1[1][1][0] = 0;          dn1[1][2][0] = vet[1][0];
+    dn1[2][0][0] = vet[1][1];  dn1[2][1][0] = -vet[1][0]; dn1[2][2][0] = 0;
+    dn1[0][0][1] = 0;          dn1[0][1][1] = -vet[0][2]; dn1[0][2][1] = vet[0][1];
+    dn1[1][0][1] = vet[0][2];  dn1[1][1][1] = 0;          dn1[1][2][1] = -vet[0][0];
+    dn1[2][0][1] = -vet[0][1]; dn1[2][1][1] = vet[0][0];  dn1[2][2][1] = 0;
+    nn2 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+    nrm_len = M::sq(nn2);
+    if (nrm_len == (R)0) {
+      atomicOr(P.err, EB_NORMAL_ZERO);
+      nrm_len = 1;
+      nn2 = 1;
+    }
+    R dni[3];
+    for (int b = 0; b < 3; b++) dni[b] = (n1[0] * dpvdri[0][b] + n1[1] * dpvdri[1][b] + n1[2] * dpvdri[2][b]) / nrm_len;
+    R *dro = P.dndri + (size_t)i * 9;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) dro[a * 3 + b] = dpvdri[a][b] / nrm_len - n1[a] * dni[b] / nn2;
+  } else {
+    // n1 = (v1xv2 + v3xv1 + v2xv3)/3; independent of x_i (ilp.cpp:929-1067)
+    R pv12[3], pv31[3], pv23[3];
+    pv12[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
+    pv12[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
+    pv12[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
+    pv31[0] = vet[2][1] * vet[0][2] - vet[0][1] * vet[2][2];
+    pv31[1] = vet[2][2] * vet[0][0] - vet[0][2] * vet[2][0];
+    pv31[2] = vet[2][0] * vet[0][1] - vet[0][0] * vet[2][1];
+    pv23[0] = vet[1][1] * vet[2][2] - vet[2][1] * vet[1][2];
+    pv23[1] = vet[1][2] * vet[2][0] - vet[2][2] * vet[1][0];
+    pv23[2] = vet[1][0] * vet[2][1] - vet[2][0] * vet[1][1];
+    for (int a = 0; a < 3; a++) n1[a] = (pv12[a] + pv31[a] + pv23[a]) / (R)3;
+    // d n1 / d x_m = (dpv12 + dpv23 + dpv31)/3 for neighbour m; the cross-product derivative
+    // of u x w with respect to u is [[0, w2, -w1], [-w2, 0, w0], [w1, -w0, 0]] (rows: n comps)
+    // m = 0: pv12 (u = v1, w = v2) and pv31 (v3 x v1, derivative wrt v1 = -[v3]x) ...
+    const R third = (R)1 / (R)3;
+    // difference vectors: derivative of n1 wrt x_m equals (1/3) * cross-derivative with
+    // w_m = v_{m+1} - v_{m+2} (cyclic), i.e. d/dx_m [(x2-x1)x(x3-x1)]
+    R wv[3][3];
+    for (int d = 0; d < 3; d++) {
+      wv[0][d] = vet[1][d] - vet[2][d];
+      wv[1][d] = vet[2][d] - vet[0][d];
+      wv[2][d] = vet[0][d] - vet[1][d];
+    }
+    for (int m = 0; m < 3; m++) {
+      dn1[0][0][m] = 0;                  dn1[0][1][m] = wv[m][2] * third;  dn1[0][2][m] = -wv[m][1] * third;
+      dn1[1][0][m] = -wv[m][2] * third;  dn1[1][1][m] = 0;                 dn1[1][2][m] = wv[m][0] * third;
+      dn1[2][0][m] = wv[m][1] * third;   dn1[2][1][m] = -wv[m][0] * third; dn1[2][2][m] = 0;
+    }
+    nn2 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+    nrm_len = M::sq(nn2);
+    if (nrm_len == (R)0) {
+      atomicOr(P.err, EB_NORMAL_ZERO);
+      nrm_len = 1;
+      nn2 = 1;
+    }
+  }
+  const R rl = (R)1 / nrm_len;
+  for (int a = 0; a < 3; a++) normal[a] = n1[a] * rl;
+  P.nrm[i] = {normal[0], normal[1], normal[2], (R)0};
+  // dnormal[id][ip][m] = dn1/nn - n1 * dnn/nn2, dnn[ip][m] = n1 . dn1[:, ip, m] / nn
+  for (int m = 0; m < 3; m++)
+    for (int b = 0; b < 3; b++) {
+      const R dnn = (n1[0] * dn1[0][b][m] + n1[1] * dn1[1][b][m] + n1[2] * dn1[2][b][m]) / nrm_len;
+      for (int a = 0; a < 3; a++) dno[m * 9 + a * 3 + b] = dn1[a][b][m] / nrm_len - n1[a] * dnn / nn2;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Per step 2: ILP pair kernel (calc_FvdW + calc_FRep, ilp.cpp:449-675), one wavefront per
+// atom i.  For each j in the layer-filtered list with r < rc the wave evaluates
+//   vdW(i,j), repulsion (i,j) with n_i and repulsion (j,i) with n_j,
+// and keeps only what acts on i: f_i, S_i = sum_j w_ij del_ij (w = prodnorm*fpair1*Tap,
+// the weight of the normal-derivative forces on i's normal neighbours), per-atom energy halves
+// and the virial.  Lanes own list entries; the wave reduces with butterfly shuffles.
+
+template <typename R>
+struct Tap {
+  R t, dt;
+};
+
+template <typename R>
+__device__ __forceinline__ Tap<R> calc_tap(R r, R inv_rcut, int tap_flag) {
+  Tap<R> o;
+  if (!tap_flag) {
+    o.t = 1;
+    o.dt = 0;
+    return o;
+  }
+  const R x = r * inv_rcut;
+  // Tap(x) = 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1 (pair_ilp_graphene_hbn.h:80-117), Horner
+  R t = (R)20 * x + (R)-70;
+  t = t * x + (R)84;
+  t = t * x + (R)-35;
+  t = t * x;
+  t = t * x;
+  t = t * x;
+  t = t * x + (R)1;
+  R d = (R)140 * x + (R)-420;
+  d = d * x + (R)420;
+  d = d * x + (R)-140;
+  d = d * x;
+  d = d * x;
+  d = d * x;
+  o.t = t;
+  o.dt = d * inv_rcut;
+  return o;
+}
+
+template <typename R, bool EF, bool VF, bool VA>
+__global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
+  using M = MathOps<R>;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[i]];
+  const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
+  const double4 xi = P.x[i];
+  const int ti = (int)xi.w - 1;
+  const auto ni4 = P.nrm[i];
+  const R nix = ni4.x, niy = ni4.y, niz = ni4.z;
+  const int cnt = P.icnt[i];
+  const uint32_t *row = P.ilist + (size_t)i * P.icap;
+  const int itag = P.tag[i];
+  const R inv_rcut = (R)P.inv_rcut;
+  double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
+  double w[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0}, tt[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = lane; k < cnt; k += 64) {
+    const uint32_t e = row[k];
+    const int j = e_idx(e);
+    const double4 xj = P.x[j];
+    double gx, gy, gz;
+    ghost(xj, e, prd, gx, gy, gz);
+    const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
+    const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+    if (rsqd >= P.ilp_cutsq) continue;
+    const int tj = (int)xj.w - 1;
+    const IlpPair<R> &pij = P.ip[ti * MAXT + tj];
+    const IlpPair<R> &pji = P.ip[tj * MAXT + ti];
+    const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd, rsq = (R)rsqd;
+    const R r = M::sq(rsq);
+    const R rinv = M::inv(r);
+    const Tap<R> tp = calc_tap<R>(r, inv_rcut, P.tap_flag);
+    // ---- van der Waals, one half per ordered pair ----
+    const IlpPair<R> *pv = &pij;
+    if (!pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
+    const R r2inv = rinv * rinv;
+    const R r6inv = r2inv * r2inv * r2inv;
+    const R r8inv = r6inv * r2inv;
+    const R TS = (R)1 + M::ex(-pv->d * (r * pv->inv_seff - (R)1));
+    const R TSinv = M::inv(TS);
+    const R Vvdw = -pv->C6 * r6inv * TSinv;
+    const R fpv = (R)-6 * pv->C6 * r8inv * TSinv + pv->C6_d_seff * (TS - (R)1) * TSinv * TSinv * r8inv * r;
+    const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
+    // ---- repulsion (i,j) with n_i ----
+    const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
+    const R pn1 = nix * dx + niy * dy + niz * dz;
+    const R rd1 = (rsq - pn1 * pn1) * pij.d2inv;
+    const R frho1 = M::ex(-rd1) * pij.C;
+    const R Er1 = pij.half_eps + frho1;
+    const R V1 = exp0 * Er1;
+    const R fp1 = pij.lambda * exp0 * rinv * Er1;
+    const R fq1 = (R)2 * exp0 * frho1 * pij.d2inv;
+    const R fs1 = fp1 + fq1;
+    const R a1 = pn1 * fq1;
+    const R b1 = V1 * tp.dt * rinv;
+    const R fkx = (dx * fs1 - a1 * nix) * tp.t - b1 * dx;
+    const R fky = (dy * fs1 - a1 * niy) * tp.t - b1 * dy;
+    const R fkz = (dz * fs1 - a1 * niz) * tp.t - b1 * dz;
+    const R wgt = a1 * tp.t;
+    // ---- repulsion (j,i) with n_j; del_ji = -del ----
+    const auto nj4 = P.nrm[j];
+    const R exp0b = pij.esym ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
+    const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dz);
+    const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
+    const R frho2 = M::ex(-rd2) * pji.C;
+    const R Er2 = pji.half_eps + frho2;
+    const R V2 = exp0b * Er2;
+    const R fp2 = pji.lambda * exp0b * rinv * Er2;
+    const R fq2 = (R)2 * exp0b * frho2 * pji.d2inv;
+    const R fs2 = fp2 + fq2;
+    const R a2 = pn2 * fq2;
+    const R b2 = V2 * tp.dt * rinv;
+    // fkc_ji = (del_ji*fs2 - a2*n_j)*Tap - b2*del_ji, del_ji = -del; force on i is -fkc_ji
+    const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
+    const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
+    const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
+    fx += (double)(fsv * dx) + (double)fkx - (double)gkx;
+    fy += (double)(fsv * dy) + (double)fky - (double)gky;
+    fz += (double)(fsv * dz) + (double)fkz - (double)gkz;
+    sx += (double)(wgt * dx);
+    sy += (double)(wgt * dy);
+    sz += (double)(wgt * dz);
+    if (EF) {
+      ev += 0.5 * (double)(Vvdw * tp.t);
+      er += 0.5 * (double)(tp.t * V1) + 0.5 * (double)(tp.t * V2);
+    }
+    if (VF || VA) {
+      // vdW: half of del (x) del * fsum per ordered pair; repulsion (i,j): del (x) fkc
+      const double hv = 0.5 * (double)fsv;
+      const double dd[3] = {dxd, dyd, dzd};
+      const double fk[3] = {(double)fkx, (double)fky, (double)fkz};
+      const double gk[3] = {(double)gkx, (double)gky, (double)gkz};
+      if (VF) {
+        w[0] += hv * dd[0] * dd[0] + dd[0] * fk[0];
+        w[1] += hv * dd[1] * dd[1] + dd[1] * fk[1];
+        w[2] += hv * dd[2] * dd[2] + dd[2] * fk[2];
+        w[3] += hv * dd[0] * dd[1] + dd[0] * fk[1];
+        w[4] += hv * dd[0] * dd[2] + dd[0] * fk[2];
+        w[5] += hv * dd[1] * dd[2] + dd[1] * fk[2];
+      }
+      if (VA) {
+        // ev_tally (vdW) half, ev_tally_xyz(i,j) half for (i,j), and the j-side half of (j,i):
+        // 0.5 * del_ji (x) fkc_ji = 0.5 * (-del) (x) (-gk)... fkc_ji = gk, del_ji = -del
+        va[0] += hv * dd[0] * dd[0] + 0.5 * (dd[0] * fk[0] - dd[0] * gk[0]);
+        va[1] += hv * dd[1] * dd[1] + 0.5 * (dd[1] * fk[1] - dd[1] * gk[1]);
+        va[2] += hv * dd[2] * dd[2] + 0.5 * (dd[2] * fk[2] - dd[2] * gk[2]);
+        va[3] += hv * dd[0] * dd[1] + 0.5 * (dd[0] * fk[1] - dd[0] * gk[1]);
+        va[4] += hv * dd[0] * dd[2] + 0.5 * (dd[0] * fk[2] - dd[0] * gk[2]);
+        va[5] += hv * dd[1] * dd[2] + 0.5 * (dd[1] * fk[2] - dd[1] * gk[2]);
+        const double wd = (double)wgt;
+        tt[0] += wd * dd[0] * dd[0];
+        tt[1] += wd * dd[1] * dd[1];
+        tt[2] += wd * dd[2] * dd[2];
+        tt[3] += wd * dd[0] * dd[1];
+        tt[4] += wd * dd[0] * dd[2];
+        tt[5] += wd * dd[1] * dd[2];
+        // k-terms of pair (j,i): half of (x_k - x_i) (x) fk lands on i (ev_tally_xyz(k, i)),
+        // fk = -w_ji * D_k(j)^T del_ji, x_k - x_i = vet_k(j) + del_ji
+        const uint32_t *nj = P.nnb + (size_t)j * 4;
+        const int cj = (int)nj[0];
+        const double wji = (double)(a2 * tp.t);
+        for (int kk = 0; kk < cj; kk++) {
+          const R *D = P.dn + (size_t)j * 27 + kk * 9;
+          const double *vt = P.vet + (size_t)j * 9 + kk * 3;
+          double fkk[3];
+          for (int c = 0; c < 3; c++)
+            fkk[c] = -wji * (-(double)D[0 * 3 + c] * dd[0] - (double)D[1 * 3 + c] * dd[1] - (double)D[2 * 3 + c] * dd[2]);
+          const double rk[3] = {vt[0] - dd[0], vt[1] - dd[1], vt[2] - dd[2]};
+          va[0] += 0.5 * rk[0] * fkk[0];
+          va[1] += 0.5 * rk[1] * fkk[1];
+          va[2] += 0.5 * rk[2] * fkk[2];
+          va[3] += 0.5 * rk[0] * fkk[1];
+          va[4] += 0.5 * rk[0] * fkk[2];
+          va[5] += 0.5 * rk[1] * fkk[2];
+        }
+      }
+    }
+  }
+  fx = wsum(fx); fy = wsum(fy); fz = wsum(fz);
+  sx = wsum(sx); sy = wsum(sy); sz = wsum(sz);
+  if (EF) { ev = wsum(ev); er = wsum(er); }
+  if (VF) for (int c = 0; c < 6; c++) w[c] = wsum(w[c]);
+  if (VA) for (int c = 0; c < 6; c++) { va[c] = wsum(va[c]); tt[c] = wsum(tt[c]); }
+  if (lane != 0) return;
+  P.fil[i] = {fx, fy, fz, 0.0};
+  P.S[i] = {sx, sy, sz, 0.0};
+  const size_t n = (size_t)P.n;
+  if (EF) { P.th[0 * n + i] = ev; P.th[1 * n + i] = er; }
+  if (VF) {
+    // virial share of the normal-derivative forces of i's normal: sum_k vet_k (x) F_k,
+    // F_k = -D_k^T S_i  (equals the F.r virial of those forces by translation invariance)
+    const uint32_t *nrow = P.nnb + (size_t)i * 4;
+    const int ci = (int)nrow[0];
+    for (int kk = 0; kk < ci; kk++) {
+      const R *D = P.dn + (size_t)i * 27 + kk * 9;
+      const double *vt = P.vet + (size_t)i * 9 + kk * 3;
+      double Fk[3];
+      for (int c = 0; c < 3; c++) Fk[c] = -((double)D[0 * 3 + c] * sx + (double)D[1 * 3 + c] * sy + (double)D[2 * 3 + c] * sz);
+      w[0] += vt[0] * Fk[0];
+      w[1] += vt[1] * Fk[1];
+      w[2] += vt[2] * Fk[2];
+      w[3] += vt[0] * Fk[1];
+      w[4] += vt[0] * Fk[2];
+      w[5] += vt[1] * Fk[2];
+    }
+    for (int c = 0; c < 6; c++) P.th[(3 + c) * n + i] = w[c];
+  }
+  if (VA) {
+    for (int c = 0; c < 6; c++) {
+      P.vatom[c * n + i] = va[c];
+      P.T[(size_t)i * 6 + c] = tt[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Tersoff functions (pair_tersoff.cpp:584-811, pair_tersoff.h:90-107)
+
+template <typename R>
+__device__ __forceinline__ R t_fc(R r, const TersP<R> &p) {
+  if (r < p.bigr - p.bigd) return (R)1;
+  if (r > p.bigr + p.bigd) return (R)0;
+  return (R)0.5 * ((R)1 - MathOps<R>::sn((R)MY_PI2 * (r - p.bigr) / p.bigd));
+}
+template <typename R>
+__device__ __forceinline__ R t_fc_d(R r, const TersP<R> &p) {
+  if (r < p.bigr - p.bigd) return (R)0;
+  if (r > p.bigr + p.bigd) return (R)0;
+  return -((R)MY_PI4 / p.bigd) * MathOps<R>::cs((R)MY_PI2 * (r - p.bigr) / p.bigd);
+}
+template <typename R>
+__device__ __forceinline__ R t_gijk(R costheta, const TersP<R> &p) {
+  const R hcth = p.h - costheta;
+  return p.gamma * ((R)1 + p.cc / p.dd - p.cc / (p.dd + hcth * hcth));
+}
+template <typename R>
+__device__ __forceinline__ R t_gijk_d(R costheta, const TersP<R> &p) {
+  const R hcth = p.h - costheta;
+  const R numerator = (R)-2 * p.cc * hcth;
+  const R denominator = (R)1 / (p.dd + hcth * hcth);
+  return p.gamma * numerator * denominator * denominator;
+}
+template <typename R>
+__device__ __forceinline__ R t_exdelr(R rij, R rik, const TersP<R> &p, R &ex_delr_d) {
+  R arg;
+  if (p.powermint == 3) {
+    const R t = p.lam3 * (rij - rik);
+    arg = t * t * t;
+  } else {
+    arg = p.lam3 * (rij - rik);
+  }
+  R ex;
+  if (arg > (R)69.0776) ex = (R)1.e30;
+  else if (arg < (R)-69.0776) ex = (R)0;
+  else ex = MathOps<R>::ex(arg);
+  if (p.powermint == 3) ex_delr_d = (R)3 * (p.lam3 * p.lam3 * p.lam3) * (rij - rik) * (rij - rik) * ex;
+  else ex_delr_d = p.lam3 * ex;
+  return ex;
+}
+template <typename R>
+__device__ __forceinline__ R t_bij(R zeta, const TersP<R> &p) {
+  using M = MathOps<R>;
+  const R tmp = p.beta * zeta;
+  if (tmp > p.c1) return (R)1 / M::sq(tmp);
+  if (tmp > p.c2) return ((R)1 - M::pw(tmp, -p.powern) / ((R)2 * p.powern)) / M::sq(tmp);
+  if (tmp < p.c4) return (R)1;
+  if (tmp < p.c3) return (R)1 - M::pw(tmp, p.powern) / ((R)2 * p.powern);
+  return M::pw((R)1 + M::pw(tmp, p.powern), (R)-1 / ((R)2 * p.powern));
+}
+template <typename R>
+__device__ __forceinline__ R t_bij_d(R zeta, const TersP<R> &p) {
+  using M = MathOps<R>;
+  const R tmp = p.beta * zeta;
+  if (tmp > p.c1) return p.beta * (R)-0.5 * M::pw(tmp, (R)-1.5);
+  if (tmp > p.c2)
+    return p.beta * ((R)-0.5 * M::pw(tmp, (R)-1.5) * ((R)1 - ((R)1 + (R)1 / ((R)2 * p.powern)) * M::pw(tmp, -p.powern)));
+  if (tmp < p.c4) return (R)0;
+  if (tmp < p.c3) return (R)-0.5 * p.beta * M::pw(tmp, p.powern - (R)1);
+  const R tmp_n = M::pw(tmp, p.powern);
+  return (R)-0.5 * M::pw((R)1 + tmp_n, (R)-1 - ((R)1 / ((R)2 * p.powern))) * tmp_n / zeta;
+}
+
+// attractive(): forces on i, j, k of the zeta term of bond (i,j) through neighbour k
+// (pair_tersoff.cpp:642-661 -> ters_zetaterm_d 738-795, costheta_d 797-811)
+template <typename R>
+__device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R rsqij, R rsqik, const R *delrij,
+                                             const R *delrik, R *fi, R *fj, R *fk) {
+  using M = MathOps<R>;
+  const R rij = M::sq(rsqij), rijinv = (R)1 / rij;
+  const R rik = M::sq(rsqik), rikinv = (R)1 / rik;
+  R rij_hat[3], rik_hat[3];
+  for (int d = 0; d < 3; d++) {
+    rij_hat[d] = rijinv * delrij[d];
+    rik_hat[d] = rikinv * delrik[d];
+  }
+  const R fc = t_fc(rik, p), dfc = t_fc_d(rik, p);
+  R ex_delr_d;
+  const R ex_delr = t_exdelr(rij, rik, p, ex_delr_d);
+  const R cos_theta = rij_hat[0] * rik_hat[0] + rij_hat[1] * rik_hat[1] + rij_hat[2] * rik_hat[2];
+  const R gijk = t_gijk(cos_theta, p), gijk_d = t_gijk_d(cos_theta, p);
+  R dcosdri[3], dcosdrj[3], dcosdrk[3];
+  for (int d = 0; d < 3; d++) {
+    dcosdrj[d] = (-cos_theta * rij_hat[d] + rik_hat[d]) * ((R)1 / rij);
+    dcosdrk[d] = (-cos_theta * rik_hat[d] + rij_hat[d]) * ((R)1 / rik);
+    dcosdri[d] = -(dcosdrj[d] + dcosdrk[d]);
+  }
+  for (int d = 0; d < 3; d++) {
+    R v = -dfc * gijk * ex_delr * rik_hat[d];
+    v = fc * gijk_d * ex_delr * dcosdri[d] + v;
+    v = fc * gijk * ex_delr_d * rik_hat[d] + v;
+    v = -fc * gijk * ex_delr_d * rij_hat[d] + v;
+    fi[d] = prefactor * v;
+    R u = fc * gijk_d * ex_delr * dcosdrj[d];
+    u = fc * gijk * ex_delr_d * rij_hat[d] + u;
+    fj[d] = prefactor * u;
+    R q = dfc * gijk * ex_delr * rik_hat[d];
+    q = fc * gijk_d * ex_delr * dcosdrk[d] + q;
+    q = -fc * gijk * ex_delr_d * rik_hat[d] + q;
+    fk[d] = prefactor * q;
+  }
+}
+
+// Per step 3: zeta, b_ij and force_zeta for every Tersoff bond of i (pair_tersoff.cpp:177-225)
+template <typename R, bool EF>
+__global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
+  using M = MathOps<R>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[i]];
+  const double4 xi = P.x[i];
+  const int ti = (int)xi.w - 1;
+  const int ns = P.tcnt[i];
+  const uint32_t *row = P.tsh + (size_t)i * MAXSH;
+  R del[MAXSH][3];
+  R rsqs[MAXSH];
+  int tt[MAXSH];
+  for (int a = 0; a < ns; a++) {
+    const uint32_t e = row[a];
+    const double4 xj = P.x[e_idx(e)];
+    double gx, gy, gz;
+    ghost(xj, e, s.prd, gx, gy, gz);
+    const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
+    del[a][0] = (R)d0; del[a][1] = (R)d1; del[a][2] = (R)d2;
+    rsqs[a] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+    tt[a] = (int)xj.w - 1;
+  }
+  for (int jj = 0; jj < ns; jj++) {
+    const size_t slot = (size_t)i * MAXSH + jj;
+    const int pij = P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[jj]];
+    const TersP<R> &p = P.tp[pij];
+    if ((double)rsqs[jj] >= p.cutsq) {
+      P.fz[slot] = 0;
+      P.pref[slot] = 0;
+      if (EF) P.eb[slot] = 0;
+      continue;
+    }
+    R zeta = 0;
+    const R rij = M::sq(rsqs[jj]);
+    for (int kk = 0; kk < ns; kk++) {
+      if (kk == jj) continue;
+      const TersP<R> &q = P.tp[P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[kk]]];
+      if ((double)rsqs[kk] >= q.cutsq) continue;
+      const R rik = M::sq(rsqs[kk]);
+      const R costheta = (del[jj][0] * del[kk][0] + del[jj][1] * del[kk][1] + del[jj][2] * del[kk][2]) / (rij * rik);
+      R dd;
+      const R ex = t_exdelr(rij, rik, q, dd);
+      zeta += t_fc(rik, q) * t_gijk(costheta, q) * ex;
+    }
+    // force_zeta (pair_tersoff.cpp:621-640)
+    const R r = rij;
+    R fa, fa_d;
+    if (r > p.bigr + p.bigd) {
+      fa = 0;
+      fa_d = 0;
+    } else {
+      const R e2 = M::ex(-p.lam2 * r);
+      fa = -p.bigb * e2 * t_fc(r, p);
+      fa_d = p.bigb * e2 * (p.lam2 * t_fc(r, p) - t_fc_d(r, p));
+    }
+    const R bij = t_bij(zeta, p);
+    P.fz[slot] = (double)((R)0.5 * bij * fa_d / r);
+    P.pref[slot] = (double)((R)-0.5 * fa * t_bij_d(zeta, p));
+    if (EF) P.eb[slot] = (double)((R)0.5 * bij * fa);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void tally3(double *va, const R *delr1, const R *fj, const R *delr2, const R *fk) {
+  const double th = 1.0 / 3.0;
+  va[0] += th * ((double)delr1[0] * fj[0] + (double)delr2[0] * fk[0]);
+  va[1] += th * ((double)delr1[1] * fj[1] + (double)delr2[1] * fk[1]);
+  va[2] += th * ((double)delr1[2] * fj[2] + (double)delr2[2] * fk[2]);
+  va[3] += th * ((double)delr1[0] * fj[1] + (double)delr2[0] * fk[1]);
+  va[4] += th * ((double)delr1[0] * fj[2] + (double)delr2[0] * fk[2]);
+  va[5] += th * ((double)delr1[1] * fj[2] + (double)delr2[1] * fk[2]);
+}
+
+// Per step 4: everything that acts on atom a besides the ILP pair sum -- the normal-derivative
+// forces (own d n_a / d x_a term and a as a normal neighbour of m), Tersoff repulsion, and the
+// Tersoff bond-order forces with a as i, as j and as k.  One thread per atom.
+template <typename R, bool EF, bool VF, bool VA>
+__global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
+  using M = MathOps<R>;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[a]];
+  const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
+  const double4 xa = P.x[a];
+  const int ta = (int)xa.w - 1;
+  const int ma = P.mol[a];
+  const size_t n = (size_t)P.n;
+  double4 F = P.fil[a];
+  double et = 0.0;
+  double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff tallies (also the Tersoff global virial share)
+  double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
+  // ---- ILP: d n_a / d x_a term (2-neighbour normals only) ----
+  const uint32_t *na = P.nnb + (size_t)a * 4;
+  if (na[0] == 2) {
+    const double4 Sa = P.S[a];
+    const R *Dri = P.dndri + (size_t)a * 9;
+    for (int c = 0; c < 3; c++) {
+      const double v = (double)Dri[0 * 3 + c] * Sa.x + (double)Dri[1 * 3 + c] * Sa.y + (double)Dri[2 * 3 + c] * Sa.z;
+      (&F.x)[c] -= v;
+    }
+  }
+  // ---- ILP: a as a normal neighbour of m ----
+  const int sc = P.scnt[a];
+  const uint32_t *srow = P.slist + (size_t)a * P.scap;
+  for (int q = 0; q < sc; q++) {
+    const int m = e_idx(srow[q]);
+    if (P.mol[m] != ma) continue;
+    const uint32_t *nm = P.nnb + (size_t)m * 4;
+    const int cm = (int)nm[0];
+    for (int kk = 0; kk < cm; kk++) {
+      if (e_idx(nm[1 + kk]) != a) continue;
+      const double4 Sm = P.S[m];
+      const R *D = P.dn + (size_t)m * 27 + kk * 9;
+      double Fk[3];
+      for (int c = 0; c < 3; c++) Fk[c] = -((double)D[0 * 3 + c] * Sm.x + (double)D[1 * 3 + c] * Sm.y + (double)D[2 * 3 + c] * Sm.z);
+      F.x += Fk[0];
+      F.y += Fk[1];
+      F.z += Fk[2];
+      if (VA) {
+        // 0.5 * (vet_k(m) (x) F_k - T_m D_k)
+        const double *vt = P.vet + (size_t)m * 9 + kk * 3;
+        const double *Tm = P.T + (size_t)m * 6;
+        const double Tf[3][3] = {{Tm[0], Tm[3], Tm[4]}, {Tm[3], Tm[1], Tm[5]}, {Tm[4], Tm[5], Tm[2]}};
+        double TD[3][3];
+        for (int r0 = 0; r0 < 3; r0++)
+          for (int c = 0; c < 3; c++)
+            TD[r0][c] = Tf[r0][0] * (double)D[0 * 3 + c] + Tf[r0][1] * (double)D[1 * 3 + c] + Tf[r0][2] * (double)D[2 * 3 + c];
+        vk[0] += 0.5 * (vt[0] * Fk[0] - TD[0][0]);
+        vk[1] += 0.5 * (vt[1] * Fk[1] - TD[1][1]);
+        vk[2] += 0.5 * (vt[2] * Fk[2] - TD[2][2]);
+        vk[3] += 0.5 * (vt[0] * Fk[1] - TD[0][1]);
+        vk[4] += 0.5 * (vt[0] * Fk[2] - TD[0][2]);
+        vk[5] += 0.5 * (vt[1] * Fk[2] - TD[1][2]);
+      }
+      break;
+    }
+  }
+  // ---- Tersoff: repulsion and bonds with a as i ----
+  const int ns = P.tcnt[a];
+  const uint32_t *trow = P.tsh + (size_t)a * MAXSH;
+  const int atag = P.tag[a];
+  R del[MAXSH][3];
+  R rsqs[MAXSH];
+  int tt[MAXSH];
+  for (int jj = 0; jj < ns; jj++) {
+    const uint32_t e = trow[jj];
+    const int j = e_idx(e);
+    const double4 xj = P.x[j];
+    double gx, gy, gz;
+    ghost(xj, e, prd, gx, gy, gz);
+    const double d0 = gx - xa.x, d1 = gy - xa.y, d2 = gz - xa.z;
+    del[jj][0] = (R)d0; del[jj][1] = (R)d1; del[jj][2] = (R)d2;
+    rsqs[jj] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+    tt[jj] = (int)xj.w - 1;
+    // repulsion: parameter of whichever atom computes the pair in the reference (tag parity)
+    const bool skip = tag_skip(atag, P.tag[j], xa.x, xa.y, xa.z, gx, gy, gz);
+    const int pr = skip ? P.t3[(tt[jj] * MAXT + ta) * MAXT + ta] : P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[jj]];
+    const TersP<R> &p = P.tp[pr];
+    const double rsqd = d0 * d0 + d1 * d1 + d2 * d2;
+    if (rsqd < p.cutsq) {
+      const R r = M::sq(rsqs[jj]);
+      const R tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
+      const R tmp_exp = M::ex(-p.lam1 * r);
+      const R fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
+      // del here is x_j - x_a; the reference's del is x_a - x_j
+      F.x -= (double)(del[jj][0] * fpair);
+      F.y -= (double)(del[jj][1] * fpair);
+      F.z -= (double)(del[jj][2] * fpair);
+      if (EF) et += 0.5 * (double)(tmp_fc * p.biga * tmp_exp);
+      if (VF || VA) {
+        const double hf = 0.5 * (double)fpair;
+        va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
+        va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+      }
+    }
+  }
+  for (int jj = 0; jj < ns; jj++) {
+    const size_t slot = (size_t)a * MAXSH + jj;
+    const R fzv = (R)P.fz[slot];
+    const R pref = (R)P.pref[slot];
+    if (fzv == (R)0 && pref == (R)0) continue;
+    F.x += (double)(del[jj][0] * fzv);
+    F.y += (double)(del[jj][1] * fzv);
+    F.z += (double)(del[jj][2] * fzv);
+    if (EF) et += 0.5 * P.eb[slot];
+    if (VF || VA) {
+      const double hf = -0.5 * (double)fzv;
+      const double d0 = del[jj][0], d1 = del[jj][1], d2 = del[jj][2];
+      va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
+      va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+    }
+    for (int kk = 0; kk < ns; kk++) {
+      if (kk == jj) continue;
+      const TersP<R> &q = P.tp[P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[kk]]];
+      if ((double)rsqs[kk] >= q.cutsq) continue;
+      R fi[3], fj[3], fk[3];
+      t_attractive(q, pref, rsqs[jj], rsqs[kk], del[jj], del[kk], fi, fj, fk);
+      F.x += (double)fi[0];
+      F.y += (double)fi[1];
+      F.z += (double)fi[2];
+      if (VF || VA) tally3(va, del[jj], fj, del[kk], fk);
+    }
+  }
+  // ---- Tersoff: a as j (bond i-a) and as k (bond i-j through a), i in a's short list ----
+  for (int q = 0; q < sc; q++) {
+    const int i = e_idx(srow[q]);
+    const int ni = P.tcnt[i];
+    if (ni == 0) continue;
+    const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
+    int slot_a = -1;
+    for (int b = 0; b < ni; b++)
+      if (e_idx(irow[b]) == a) { slot_a = b; break; }
+    if (slot_a < 0) continue;
+    const double4 xi = P.x[i];
+    const int ti = (int)xi.w - 1;
+    R dl[MAXSH][3];
+    R rq[MAXSH];
+    int ty[MAXSH];
+    for (int b = 0; b < ni; b++) {
+      const uint32_t e = irow[b];
+      const double4 xb = P.x[e_idx(e)];
+      double gx, gy, gz;
+      ghost(xb, e, prd, gx, gy, gz);
+      const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
+      dl[b][0] = (R)d0; dl[b][1] = (R)d1; dl[b][2] = (R)d2;
+      rq[b] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+      ty[b] = (int)xb.w - 1;
+    }
+    // a as j of bond (i, a)
+    {
+      const size_t slot = (size_t)i * MAXSH + slot_a;
+      const R fzv = (R)P.fz[slot];
+      const R pref = (R)P.pref[slot];
+      if (!(fzv == (R)0 && pref == (R)0)) {
+        F.x -= (double)(dl[slot_a][0] * fzv);
+        F.y -= (double)(dl[slot_a][1] * fzv);
+        F.z -= (double)(dl[slot_a][2] * fzv);
+        if (EF) et += 0.5 * P.eb[slot];
+        if (VF || VA) {
+          const double hf = -0.5 * (double)fzv;
+          const double d0 = dl[slot_a][0], d1 = dl[slot_a][1], d2 = dl[slot_a][2];
+          va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
+          va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+        }
+        for (int kk = 0; kk < ni; kk++) {
+          if (kk == slot_a) continue;
+          const TersP<R> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + ty[kk]]];
+          if ((double)rq[kk] >= qp.cutsq) continue;
+          R fi[3], fj[3], fk[3];
+          t_attractive(qp, pref, rq[slot_a], rq[kk], dl[slot_a], dl[kk], fi, fj, fk);
+          F.x += (double)fj[0];
+          F.y += (double)fj[1];
+          F.z += (double)fj[2];
+          if (VF || VA) tally3(va, dl[slot_a], fj, dl[kk], fk);
+        }
+      }
+    }
+    // a as k of bond (i, j)
+    for (int jj = 0; jj < ni; jj++) {
+      if (jj == slot_a) continue;
+      const size_t slot = (size_t)i * MAXSH + jj;
+      const R fzv = (R)P.fz[slot];
+      const R pref = (R)P.pref[slot];
+      if (fzv == (R)0 && pref == (R)0) continue;
+      const TersP<R> &qp = P.tp[P.t3[(ti * MAXT + ty[jj]) * MAXT + ta]];
+      if ((double)rq[slot_a] >= qp.cutsq) continue;
+      R fi[3], fj[3], fk[3];
+      t_attractive(qp, pref, rq[jj], rq[slot_a], dl[jj], dl[slot_a], fi, fj, fk);
+      F.x += (double)fk[0];
+      F.y += (double)fk[1];
+      F.z += (double)fk[2];
+      if (VF || VA) tally3(va, dl[jj], fj, dl[slot_a], fk);
+    }
+  }
+  F.w = 0.0;
+  P.f[a] = F;
+  if (EF) P.th[2 * n + a] = et;
+  if (VF)
+    for (int c = 0; c < 6; c++) P.th[(3 + c) * n + a] += va[c];
+  if (VA)
+    for (int c = 0; c < 6; c++) P.vatom[c * n + a] += va[c] + vk[c];
+}
+
+// ---------------------------------------------------------------------------------------
+// Integration (FixNVE, LMFF/fix_nve.cpp:64-143) and Langevin (FixLangevin::post_force,
+// LMFF/fix_langevin.cpp:771-900 with the default uniform noise)
+
+__device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]) {
+  // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * ctr[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * ctr[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ ctr[1] ^ key[0];
+    const uint32_t n2 = hi0 ^ ctr[3] ^ key[1];
+    ctr[0] = n0; ctr[1] = lo1; ctr[2] = n2; ctr[3] = lo0;
+    key[0] += 0x9E3779B9u;
+    key[1] += 0xBB67AE85u;
+  }
+}
+
+__global__ void k_initial_integrate(int n, double4 *x, double4 *v, const double4 *f, const int *sys,
+                                    const SysDev *st, const double *rmass_type) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const SysDev &s = st[sys[p]];
+  double4 xp = x[p], vp = v[p];
+  const double4 fp = f[p];
+  const double dtfm = s.dtf / rmass_type[(int)xp.w - 1];
+  vp.x += dtfm * fp.x;
+  vp.y += dtfm * fp.y;
+  vp.z += dtfm * fp.z;
+  xp.x += s.dtv * vp.x;
+  xp.y += s.dtv * vp.y;
+  xp.z += s.dtv * vp.z;
+  x[p] = xp;
+  v[p] = vp;
+}
+
+// post_force (Langevin) + final_integrate; ke column on thermo steps
+__global__ void k_final_integrate(int n, const double4 *x, double4 *v, double4 *f, const int *sys, const int *tag,
+                                  SysDev *st, const double *rmass_type, double *ke, int want_ke) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const SysDev &s = st[sys[p]];
+  const double4 xp = x[p];
+  double4 vp = v[p], fp = f[p];
+  const double m = rmass_type[(int)xp.w - 1];
+  if (s.style == 1) {
+    const double gamma1 = -m / s.damp / FTM2V;
+    const double gamma2 = sqrt(m) * sqrt(24.0 * BOLTZ / s.damp / s.dtv / MVV2E) / FTM2V * sqrt(s.t_target);
+    uint32_t ctr[4] = {(uint32_t)s.step, (uint32_t)((unsigned long long)s.step >> 32), (uint32_t)tag[p], 0x4c4d4646u};
+    uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
+    philox4x32(ctr, key);
+    const double u0 = (ctr[0] + 0.5) * 2.3283064365386963e-10 - 0.5;
+    const double u1 = (ctr[1] + 0.5) * 2.3283064365386963e-10 - 0.5;
+    const double u2 = (ctr[2] + 0.5) * 2.3283064365386963e-10 - 0.5;
+    fp.x += gamma1 * vp.x + gamma2 * u0;
+    fp.y += gamma1 * vp.y + gamma2 * u1;
+    fp.z += gamma1 * vp.z + gamma2 * u2;
+    f[p] = fp;
+  }
+  const double dtfm = s.dtf / m;
+  vp.x += dtfm * fp.x;
+  vp.y += dtfm * fp.y;
+  vp.z += dtfm * fp.z;
+  v[p] = vp;
+  if (want_ke) ke[p] = 0.5 * MVV2E * m * (vp.x * vp.x + vp.y * vp.y + vp.z * vp.z);
+}
+
+__global__ void k_advance_step(int nsys, SysDev *st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys) st[s].step++;
+}
+
+// ---------------------------------------------------------------------------------------
+// Deterministic per-system reductions (Neumaier-compensated FP64): stage 1 over fixed chunks
+// that never straddle a system, stage 2 one block per system in chunk order.
+
+struct Chunk {
+  int begin, end, sys, pad;
+};
+
+__device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
+  const double t = sum + v;
+  if (fabs(sum) >= fabs(v)) comp += (sum - t) + v;
+  else comp += (v - t) + sum;
+  sum = t;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double *cols, size_t stride, int ncols,
+                                                       const Chunk *chunks, double2 *partial) {
+  __shared__ double ss[256], sc[256];
+  const Chunk ch = chunks[blockIdx.x];
+  for (int c = 0; c < ncols; c++) {
+    double sum = 0, comp = 0;
+    for (int p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) neumaier(sum, comp, cols[c * stride + p]);
+    ss[threadIdx.x] = sum;
+    sc[threadIdx.x] = comp;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        double s2 = ss[threadIdx.x], c2 = sc[threadIdx.x] + sc[threadIdx.x + o];
+        neumaier(s2, c2, ss[threadIdx.x + o]);
+        ss[threadIdx.x] = s2;
+        sc[threadIdx.x] = c2;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ncols + c] = make_double2(ss[0], sc[0]);
+    __syncthreads();
+  }
+}
+
+// out[s * ncols + c] = sum over the chunks of system s (chunk ranges [cbeg[s], cbeg[s+1]))
+__global__ void k_reduce_systems(const double2 *partial, int ncols, const int *cbeg, int nsys, double *out) {
+  const int s = blockIdx.x;
+  if (s >= nsys) return;
+  for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+    double sum = 0, comp = 0;
+    for (int k = cbeg[s]; k < cbeg[s + 1]; k++) {
+      const double2 v = partial[(size_t)k * ncols + c];
+      neumaier(sum, comp, v.x);
+      comp += v.y;
+    }
+    out[(size_t)s * ncols + c] = sum + comp;
+  }
+}
+
+// max squared displacement since the last build, per system (neigh_modify check yes)
+__global__ void k_displacement(int n, const double4 *x, const double4 *xhold, const int *sys, double trigger_sq,
+                               int *flag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double dx = x[p].x - xhold[p].x, dy = x[p].y - xhold[p].y, dz = x[p].z - xhold[p].z;
+  if (dx * dx + dy * dy + dz * dz > trigger_sq) atomicOr(flag, 1);
+  (void)sys;
+}
+
+}  // namespace lmff

@@ -1,0 +1,304 @@
+"""Pins the CPU oracle (oracle/lmff_oracle.c) before it is trusted as the parity checker.
+
+Every check here is independent of the oracle's own formulas: closed forms, an all-pairs
+numpy evaluation from the published functional forms (tests/refimpl.py), finite
+differences, the strain derivative of the energy, brute-force neighbor sets and
+invariants.  CPU only.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import POT
+import refimpl
+from lmff import structures as S
+
+CT = os.path.join(POT, "C.opt.tersoff")
+BNC = os.path.join(POT, "BNC.tersoff")
+ILP = os.path.join(POT, "BNCH.ILP")
+
+
+def make_oracle(s, tap=1, threads=1):
+    import oracle
+
+    if s.elements == ["C"]:
+        o = oracle.Oracle(1, CT, ["C(O)"], ILP, ["C"], tap_flag=tap, threads=threads)
+    else:
+        o = oracle.Oracle(len(s.elements), BNC, s.elements, ILP, s.elements, tap_flag=tap, threads=threads)
+    o.set_system(s)
+    o.build_list()
+    return o
+
+
+def small(layers=("G", "G"), seed=3, jitter=0.05, nx=15, ny=9):
+    return S.layered_stack(nx, ny, list(layers), seed=seed, jitter=jitter)
+
+
+# ---------------------------------------------------------------- Tap
+def test_tap_closed_form(oracle_lib):
+    xs = np.linspace(0.0, 0.999, 50)
+    vals = np.array([oracle_lib.tap(16.0 * x, 16.0) for x in xs])
+    coef = np.polyfit(xs, vals, 7)
+    np.testing.assert_allclose(coef, [20, -70, 84, -35, 0, 0, 0, 1], atol=1e-7)
+    assert oracle_lib.tap(0.0, 16.0) == 1.0
+    assert oracle_lib.tap(16.0, 16.0) == 0.0 and oracle_lib.tap(17.0, 16.0) == 0.0
+    # Tap' = Tap'' = Tap''' = 0 at x = 1 and dTap is the derivative
+    for r in (1.0, 5.0, 11.3, 15.99):
+        h = 1e-5
+        fd = (oracle_lib.tap(r + h, 16.0) - oracle_lib.tap(r - h, 16.0)) / (2 * h)
+        assert abs(fd - oracle_lib.dtap(r, 16.0)) < 1e-9
+    assert abs(oracle_lib.dtap(16.0 - 1e-9, 16.0)) < 1e-20 * 1e9
+    assert oracle_lib.tap(16.0 * (1 - 1e-3), 16.0) < 1e-8
+
+
+# ---------------------------------------------------------------- parameter files
+def test_ilp_parameters_parsed(oracle_lib):
+    o = oracle_lib.Oracle(3, BNC, ["C", "B", "N"], ILP, ["C", "B", "N"])
+    import ctypes as C
+
+    out = np.zeros(16)
+    n = oracle_lib.lib().orc_ilp_param(o.h, 0, out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert n == 9  # C/B/N pairs only; H lines are skipped
+    # C-C row: z0 alpha delta eps(meV) C(meV) d sR reff C6(meV A^6) S rcut
+    assert out[2] == 3.205843 and out[3] == 7.511126 and out[4] == 1.235334
+    assert math.isclose(out[5], 1.528338e-5 * 1e-3, rel_tol=1e-15)       # epsilon in eV
+    assert math.isclose(out[6], 37.530428e-3, rel_tol=1e-15)             # C in eV
+    assert math.isclose(out[10], 25.714535, rel_tol=1e-15)               # C6 in eV A^6
+    assert math.isclose(out[13], 1.0 / 1.235334**2, rel_tol=1e-15)       # delta^-2
+    assert math.isclose(out[14], 7.511126 / 3.205843, rel_tol=1e-15)     # lambda
+    assert math.isclose(out[15], 0.7954443 * 3.681440, rel_tol=1e-15)    # seff
+
+
+def test_tersoff_parameters_parsed(oracle_lib):
+    o = oracle_lib.Oracle(3, BNC, ["C", "B", "N"], ILP, ["C", "B", "N"])
+    import ctypes as C
+
+    out = np.zeros(22)
+    n = oracle_lib.lib().orc_tersoff_param(o.h, 0, out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert n == 27
+    # first BNC entry is N B B: n = 0.72751, R + D = 2.0
+    assert out[9] == 0.72751 and out[13] == 1.95 and out[14] == 0.05
+    assert math.isclose(out[17], 2.0**2, rel_tol=1e-15)
+    nn = 0.72751
+    assert math.isclose(out[18], (2 * nn * 1e-16) ** (-1 / nn), rel_tol=1e-14)
+    assert math.isclose(out[19], (2 * nn * 1e-8) ** (-1 / nn), rel_tol=1e-14)
+    assert o.cutmax == pytest.approx(2.1)
+
+
+@pytest.mark.parametrize("bad", ["missing", "duplicate", "short"])
+def test_bad_potential_files(oracle_lib, tmp_path, bad):
+    lines = [l for l in open(BNC) if l.strip() and not l.startswith("#")]
+    if bad == "missing":
+        lines = lines[1:]
+    elif bad == "duplicate":
+        lines = lines + lines[:1]
+    else:
+        lines[0] = " ".join(lines[0].split()[:10]) + "\n"
+    p = tmp_path / "x.tersoff"
+    p.write_text("".join(lines))
+    with pytest.raises(oracle_lib.OracleError):
+        oracle_lib.Oracle(3, str(p), ["C", "B", "N"], ILP, ["C", "B", "N"])
+
+
+# ---------------------------------------------------------------- closed forms
+def test_tersoff_perfect_graphene_closed_form(oracle_lib):
+    a = 2.46
+    s = S.layered_stack(15, 9, ["G"], jitter=0.0, a=a, b=a * math.sqrt(3.0), temperature=None)
+    o = make_oracle(s)
+    r = o.compute(1, 2)
+    P = refimpl.tersoff_params(CT, {"C(O)"})[("C(O)", "C(O)", "C(O)")]
+    bond = a / math.sqrt(3.0)
+    g = P["gamma"] * (1 + P["c"] ** 2 / P["d"] ** 2 - P["c"] ** 2 / (P["d"] ** 2 + (-0.5 - P["h"]) ** 2))
+    zeta = 2 * g
+    b = (1 + (P["beta"] * zeta) ** P["n"]) ** (-0.5 / P["n"])
+    e_atom = 1.5 * (P["A"] * math.exp(-P["lam1"] * bond) - b * P["B"] * math.exp(-P["lam2"] * bond))
+    assert r["e_tersoff"] / s.n == pytest.approx(e_atom, rel=1e-12)
+    assert r["e_vdw"] == 0.0 and r["e_rep"] == 0.0  # one layer: no interlayer term
+    assert np.abs(r["f"]).max() < 1e-10
+
+
+def test_ilp_flat_bilayer_normals_are_z(oracle_lib):
+    a = 2.46
+    s = S.layered_stack(15, 9, ["G", "G"], jitter=0.0, a=a, b=a * math.sqrt(3.0), temperature=None)
+    o = make_oracle(s)
+    nrm, cnt, dn, dri = o.normals()
+    assert (cnt == 3).all()
+    np.testing.assert_allclose(np.abs(nrm[:, 2]), 1.0, atol=1e-15)
+    np.testing.assert_allclose(nrm[:, :2], 0.0, atol=1e-15)
+    np.testing.assert_allclose(dri, 0.0, atol=0)  # three neighbours: n does not depend on x_i
+    # AB stacking binds more strongly than AA
+    ab = o.compute(1, 0)
+    s2 = S.layered_stack(15, 9, ["G", "G"], jitter=0.0, a=a, b=a * math.sqrt(3.0), temperature=None)
+    s2.x[s.mol == 2, 1] -= a * math.sqrt(3.0) / 3.0
+    s2.x[:, 1] %= s2.boxhi[1]
+    aa = make_oracle(s2).compute(1, 0)
+    e_ab = ab["e_vdw"] + ab["e_rep"]
+    e_aa = aa["e_vdw"] + aa["e_rep"]
+    assert e_ab < e_aa < 0
+    assert -0.030 < e_ab / s.n < -0.015  # ILP binding of AB graphene ~ -23 meV/atom
+
+
+# ---------------------------------------------------------------- independent brute force
+@pytest.mark.parametrize("layers,seed", [(("G", "G"), 3), (("G", "BN"), 4), (("BN", "G", "BN"), 5)])
+def test_energy_matches_independent_numpy(oracle_lib, layers, seed):
+    s = small(layers, seed=seed, nx=15, ny=9 if len(layers) < 3 else 9)
+    o = make_oracle(s)
+    r = o.compute(1, 2)
+    el = s.elements if s.elements != ["C"] else ["C"]
+    tfile, telem = (CT, ["C(O)"]) if s.elements == ["C"] else (BNC, s.elements)
+    et = refimpl.tersoff_energy(s, tfile, telem)
+    ev, er = refimpl.ilp_energy(s, ILP, el)
+    assert r["e_tersoff"] == pytest.approx(et, rel=1e-12)
+    assert r["e_vdw"] == pytest.approx(ev, rel=1e-12)
+    assert r["e_rep"] == pytest.approx(er, rel=1e-12)
+    nrm, cnt, _, _ = o.normals()
+    nref, cref = refimpl.normals(s, ILP, el)
+    assert (cnt == cref).all()
+    np.testing.assert_allclose(np.abs((nrm * nref).sum(1)), 1.0, atol=1e-12)
+
+
+def _fd_force(o, x0, atom, comp, h=1e-3):
+    e = []
+    for k in (-2, -1, 1, 2):
+        x = x0.copy()
+        x[atom, comp] += k * h
+        o.put_x(x)
+        e.append(o.compute(1, 0)["energy"])
+    o.put_x(x0)
+    return -(e[0] - 8 * e[1] + 8 * e[2] - e[3]) / (12 * h)
+
+
+@pytest.mark.parametrize("layers", [("G", "G"), ("G", "BN")])
+def test_forces_are_minus_energy_gradient(oracle_lib, layers):
+    s = small(layers, seed=11, jitter=0.08)
+    o = make_oracle(s)
+    x0 = o.get_x()
+    f = o.compute(1, 0)["f"]
+    rng = np.random.default_rng(0)
+    for atom in rng.choice(s.n, 6, replace=False):
+        for comp in range(3):
+            fd = _fd_force(o, x0, atom, comp)
+            assert abs(fd - f[atom, comp]) < 2e-7 * max(1.0, abs(f[atom, comp])), (atom, comp, fd, f[atom, comp])
+
+
+def test_forces_with_two_neighbour_normals(oracle_lib):
+    """Armchair-edged ribbons (open in x) give 2-neighbour normals whose d n/d x_i is nonzero."""
+    s = small(("G", "G"), seed=12, jitter=0.06, nx=12, ny=9)
+    s.boxhi[0] += 30.0  # vacuum along x: the layers become ribbons with open edges
+    s.x[:, 0] += 5.0
+    o = make_oracle(s)
+    nrm, cnt, dn, dri = o.normals()
+    assert (cnt == 2).sum() > 0
+    x0 = o.get_x()
+    f = o.compute(1, 0)["f"]
+    edge = np.nonzero(cnt == 2)[0]
+    for atom in list(edge[:4]) + [int(np.nonzero(cnt == 3)[0][0])]:
+        for comp in range(3):
+            fd = _fd_force(o, x0, atom, comp)
+            assert abs(fd - f[atom, comp]) < 2e-7 * max(1.0, abs(f[atom, comp])), (atom, comp, fd, f[atom, comp])
+
+
+def test_virial_is_strain_derivative(oracle_lib):
+    s = small(("G", "BN"), seed=13, jitter=0.05)
+    o = make_oracle(s)
+    r = o.compute(1, 2)
+    x0 = o.get_x()
+    for d in range(3):
+        es = []
+        for eps in (-2e-6, -1e-6, 1e-6, 2e-6):
+            s2 = s.copy()
+            s2.x = x0.copy()
+            s2.x[:, d] *= 1 + eps
+            if s.periodic[d]:
+                s2.boxlo[d] *= 1 + eps
+                s2.boxhi[d] *= 1 + eps
+            es.append(make_oracle(s2).compute(1, 0)["energy"])
+        dEde = (es[0] - 8 * es[1] + 8 * es[2] - es[3]) / (12 * 1e-6)
+        assert r["virial"][d] == pytest.approx(-dEde, rel=2e-6, abs=1e-3)
+
+
+def test_peratom_sums(oracle_lib):
+    s = small(("G", "BN"), seed=14)
+    o = make_oracle(s)
+    r = o.compute(3, 6)
+    assert r["eatom"].sum() == pytest.approx(r["energy"], rel=1e-13)
+    np.testing.assert_allclose(r["vatom"].sum(0), r["virial"], rtol=1e-11, atol=1e-9)
+    np.testing.assert_allclose(r["f"].sum(0), 0.0, atol=1e-10)
+
+
+def test_translation_and_point_symmetry(oracle_lib):
+    s = small(("G", "BN"), seed=15)
+    r0 = make_oracle(s).compute(1, 2)
+    s2 = s.copy()
+    s2.x = s.x + np.array([3.3, -7.1, 1.25])
+    s2.x[:, :2] %= (s.boxhi - s.boxlo)[:2]
+    r1 = make_oracle(s2).compute(1, 2)
+    assert r1["energy"] == pytest.approx(r0["energy"], rel=1e-12)
+    np.testing.assert_allclose(r1["f"], r0["f"], atol=1e-9)
+    # rotation by 180 degrees about z maps the periodic box onto itself
+    s3 = s.copy()
+    L = s.boxhi - s.boxlo
+    s3.x[:, 0] = (L[0] - s.x[:, 0]) % L[0]
+    s3.x[:, 1] = (L[1] - s.x[:, 1]) % L[1]
+    r3 = make_oracle(s3).compute(1, 2)
+    assert r3["energy"] == pytest.approx(r0["energy"], rel=1e-12)
+    np.testing.assert_allclose(r3["f"][:, :2], -r0["f"][:, :2], atol=1e-9)
+    np.testing.assert_allclose(r3["f"][:, 2], r0["f"][:, 2], atol=1e-9)
+
+
+def test_neighbor_list_matches_brute_force(oracle_lib):
+    s = small(("G", "BN"), seed=16, jitter=0.3)
+    o = make_oracle(s)
+    offs, ent, sh = o.neighbor_list()
+    L = s.boxhi - s.boxlo
+    cutsq = 18.0**2
+    x = s.x
+    for i in np.random.default_rng(1).choice(s.n, 40, replace=False):
+        got = sorted((int(j), *map(int, sh[e])) for e, j in zip(range(offs[i], offs[i + 1]), ent[offs[i]:offs[i + 1]]))
+        want = []
+        for sx in (-1, 0, 1):
+            for sy in (-1, 0, 1):
+                # the reference's operation order: x_i - (x_j + s*L), no fused multiply-add
+                gx = x[:, 0] + sx * L[0]
+                gy = x[:, 1] + sy * L[1]
+                dx = x[i, 0] - gx
+                dy = x[i, 1] - gy
+                dz = x[i, 2] - x[:, 2]
+                rsq = dx * dx + dy * dy + dz * dz
+                for j in np.nonzero(rsq <= cutsq)[0]:
+                    if j == i and sx == 0 and sy == 0:
+                        continue
+                    want.append((int(j), sx, sy, 0))
+        assert got == sorted(want)
+
+
+def test_nve_energy_error_is_second_order(oracle_lib):
+    """Velocity Verlet conserves a shadow energy: the total-energy error is bounded and
+    scales as dt^2 (halving dt divides it by ~4).  A force that is not the gradient of the
+    energy would instead show a drift that does not shrink this way."""
+    s = small(("G", "G"), seed=17, jitter=0.02)
+    errs = []
+    for dt in (0.001, 0.0005):
+        o = make_oracle(s, threads=4)
+        nst = int(round(0.1 / dt))
+        x, v, th = o.run_nve(s.x, s.v, s.mass_per_type, dt, nst, rebuild_every=10, thermo_every=nst // 20)
+        etot = th[:, 1] + th[:, 2]
+        errs.append(np.abs(etot - etot[0]).max())
+        assert 100 < 2 * th[-1, 2] / ((3 * s.n - 3) * S.BOLTZ) < 500
+    assert errs[0] / s.n < 5e-4
+    assert 3.0 < errs[0] / errs[1] < 5.0
+
+
+def test_reference_data_file_parses(oracle_lib):
+    path = os.path.join(os.path.dirname(__file__), "golden", "inf0.data")
+    s = S.read_lammps_data(path)
+    assert s.n == 5016 and sorted(set(s.mol.tolist())) == [1, 2, 3, 4, 5, 6]
+    import oracle
+
+    o = oracle.Oracle(6, CT, ["C(O)"] * 6, ILP, ["C"] * 6, tap_flag=0)
+    o.set_system(s)
+    o.build_list()
+    r = o.compute(1, 2)
+    assert -8.1 < r["energy"] / s.n < -7.9  # C(O) graphene at the deck's 1.43876 A bond
