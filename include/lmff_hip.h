@@ -143,6 +143,10 @@ int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint);
 /* ILP list of one atom (caller order index) as sorted (caller-order j, sx, sy, sz) rows, for
  * neighbor-set parity checks; returns the count in *n (rows up to capacity). */
 int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, int capacity, int *n);
+/* Counts over every resident system after the last neighbor build (it builds one if due):
+ * out[0] ILP list entries, out[1] ILP ordered pairs within the ILP cutoff, out[2] Tersoff
+ * neighbours within cutmax (tallied at the last force evaluation), out[3] short-list entries. */
+int lmff_pair_stats(lmff_ctx *ctx, int64_t out[4]);
 /* Normals (caller order): normal[n][3], count[n]. */
 int lmff_debug_normals(lmff_ctx *ctx, int sys_id, double *normal, int32_t *count);
 

@@ -15,8 +15,9 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(HERE, "liblmff_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["lmff_engine.hip", "potentials.cpp"]
-HEADERS = ["lmff_kernels.hpp", "potentials.hpp"]
+SOURCES = ["lmff_engine.hip", "potentials.cpp", "pair_lmff_hip.cpp"]
+HEADERS = ["lmff_kernels.hpp", "potentials.hpp", "pair_lmff_hip.hpp"]
+RUNNER = os.path.join(HERE, "lmff_run")
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any kernel
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -27,7 +28,7 @@ def _stale() -> bool:
     if not os.path.exists(LIB):
         return True
     t = os.path.getmtime(LIB)
-    deps = [os.path.join(CSRC, f) for f in SOURCES + HEADERS] + [os.path.join(INCLUDE, "lmff_hip.h")]
+    deps = [os.path.join(CSRC, f) for f in SOURCES + HEADERS + ["lmff_run.cpp"]] + [os.path.join(INCLUDE, "lmff_hip.h")]
     return any(os.path.getmtime(d) > t for d in deps)
 
 
@@ -53,6 +54,12 @@ def build(force: bool = True, verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # deck-subset runner (host C++ over the C ABI)
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "lmff_run.cpp"),
+           f"-L{HERE}", "-llmff_hip", "-Wl,-rpath,$ORIGIN", "-o", RUNNER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

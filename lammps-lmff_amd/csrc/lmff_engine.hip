@@ -84,6 +84,7 @@ struct DBuf {
 struct SysHost {
   int id;
   bool live;
+  bool staged = false;
   int64_t n;
   double lo[3], hi[3];
   int per[3];
@@ -269,95 +270,91 @@ struct Ctx {
   }
 
   // ---------------------------------------------------------------- layout
-  // Builds the packed device arrays from the resident systems (kept in their current
-  // sorted order) plus staged new systems, in system order.
+  // Re-packs the device arrays: resident systems keep their current (sorted) atom order and
+  // are moved with device-to-device copies; systems added since the last layout are
+  // uploaded from their host staging copies and appended.
   void relayout() {
     if (!layout_dirty) return;
     upload_params();
-    // gather resident systems' device data (contiguous [begin,end) per live system)
-    struct Blob {
-      std::vector<double4> x, v;
-      std::vector<int> mol, tag, orig;
-    };
-    std::vector<SysHost> keep;
-    std::vector<Blob> blobs;
-    for (auto &s : sys) {
-      if (!s.live) continue;
-      Blob b;
-      const size_t m = (size_t)(s.end - s.begin);
-      b.x.resize(m); b.v.resize(m); b.mol.resize(m); b.tag.resize(m); b.orig.resize(m);
-      if (m) {
-        HIPCHK(hipMemcpy(b.x.data(), x.as<double4>() + s.begin, m * sizeof(double4), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.v.data(), v.as<double4>() + s.begin, m * sizeof(double4), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.mol.data(), mol.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.tag.data(), tag.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.orig.data(), orig.as<int>() + s.begin, m * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<SysHost> next;
+    std::vector<int> src_begin;
+    std::vector<const Staged *> src_stage;
+    for (auto &s : sys)
+      if (s.live && !s.staged) {
+        next.push_back(s);
+        src_begin.push_back(s.begin);
+        src_stage.push_back(nullptr);
       }
-      keep.push_back(s);
-      blobs.push_back(std::move(b));
-    }
     for (auto &st : staged) {
-      SysHost *s = nullptr;
-      for (auto &q : sys)
-        if (q.id == st.id) s = &q;
-      if (!s || !s->live) continue;
-      Blob b;
-      const size_t m = (size_t)s->n;
-      b.x.resize(m); b.v.resize(m); b.mol.resize(m); b.tag.resize(m); b.orig.resize(m);
-      for (size_t i = 0; i < m; i++) {
-        b.x[i] = {st.x[3 * i], st.x[3 * i + 1], st.x[3 * i + 2], (double)st.type[i]};
-        b.v[i] = {st.v[3 * i], st.v[3 * i + 1], st.v[3 * i + 2], 0.0};
-        b.mol[i] = st.mol[i];
-        b.tag[i] = st.tag[i];
-        b.orig[i] = (int)i;
-      }
-      SysHost copy = *s;
-      bool found = false;
-      for (size_t k = 0; k < keep.size(); k++)
-        if (keep[k].id == st.id) {
-          blobs[k] = std::move(b);
-          found = true;
+      SysHost *s = find(st.id);
+      if (!s) continue;
+      next.push_back(*s);
+      next.back().staged = false;
+      src_begin.push_back(-1);
+      src_stage.push_back(&st);
+    }
+    int64_t nn = 0;
+    int nc = 0;
+    for (auto &s : next) {
+      s.begin = (int)nn;
+      nn += s.n;
+      s.end = (int)nn;
+      s.cell_off = nc;
+      nc += s.ncells;
+    }
+    if (nn >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)nn);
+    const size_t m1 = std::max<size_t>((size_t)nn, 1);
+    DBuf nx, nv, nmol, ntag, norig, nsys;
+    for (DBuf *b : {&nx, &nv, &nmol, &ntag, &norig, &nsys}) b->tally = &bytes_in_use;
+    nx.ensure(m1 * sizeof(double4));
+    nv.ensure(m1 * sizeof(double4));
+    nmol.ensure(m1 * sizeof(int));
+    ntag.ensure(m1 * sizeof(int));
+    norig.ensure(m1 * sizeof(int));
+    nsys.ensure(m1 * sizeof(int));
+    for (size_t k = 0; k < next.size(); k++) {
+      const auto &s = next[k];
+      const size_t m = (size_t)s.n;
+      if (m == 0) continue;
+      if (src_begin[k] >= 0) {
+        const size_t o = (size_t)src_begin[k];
+        HIPCHK(hipMemcpyAsync(nx.as<double4>() + s.begin, x.as<double4>() + o, m * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nv.as<double4>() + s.begin, v.as<double4>() + o, m * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nmol.as<int>() + s.begin, mol.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(ntag.as<int>() + s.begin, tag.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(norig.as<int>() + s.begin, orig.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
+      } else {
+        const Staged &st = *src_stage[k];
+        std::vector<double4> hx(m), hv(m);
+        std::vector<int> ho(m);
+        for (size_t i = 0; i < m; i++) {
+          hx[i] = {st.x[3 * i], st.x[3 * i + 1], st.x[3 * i + 2], (double)st.type[i]};
+          hv[i] = {st.v[3 * i], st.v[3 * i + 1], st.v[3 * i + 2], 0.0};
+          ho[i] = (int)i;
         }
-      if (!found) {
-        keep.push_back(copy);
-        blobs.push_back(std::move(b));
+        HIPCHK(hipMemcpy(nx.as<double4>() + s.begin, hx.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(nv.as<double4>() + s.begin, hv.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(nmol.as<int>() + s.begin, st.mol.data(), m * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ntag.as<int>() + s.begin, st.tag.data(), m * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(norig.as<int>() + s.begin, ho.data(), m * sizeof(int), hipMemcpyHostToDevice));
       }
+      hipLaunchKernelGGL(k_fill_int, dim3((m + BS - 1) / BS), dim3(BS), 0, stream, (int)m, nsys.as<int>() + s.begin,
+                         (int)k);
     }
+    HIPCHK(hipStreamSynchronize(stream));
+    adopt(x, nx);
+    adopt(v, nv);
+    adopt(mol, nmol);
+    adopt(tag, ntag);
+    adopt(orig, norig);
+    adopt(sysid, nsys);
     staged.clear();
-    sys = keep;
-    // atom ranges, cell grids
-    n = 0;
-    ncells = 0;
-    for (auto &s : sys) {
-      s.begin = (int)n;
-      n += s.n;
-      s.end = (int)n;
-      s.cell_off = ncells;
-      ncells += s.ncells;
-    }
-    if (n >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)n);
+    sys = next;
+    n = nn;
+    ncells = nc;
     alloc_atoms((size_t)n);
-    std::vector<double4> hx((size_t)n), hv((size_t)n);
-    std::vector<int> hm((size_t)n), ht((size_t)n), ho((size_t)n), hs((size_t)n);
-    for (size_t k = 0; k < sys.size(); k++) {
-      auto &s = sys[k];
-      auto &b = blobs[k];
-      std::copy(b.x.begin(), b.x.end(), hx.begin() + s.begin);
-      std::copy(b.v.begin(), b.v.end(), hv.begin() + s.begin);
-      std::copy(b.mol.begin(), b.mol.end(), hm.begin() + s.begin);
-      std::copy(b.tag.begin(), b.tag.end(), ht.begin() + s.begin);
-      std::copy(b.orig.begin(), b.orig.end(), ho.begin() + s.begin);
-      std::fill(hs.begin() + s.begin, hs.begin() + s.end, (int)k);
-    }
-    if (n) {
-      HIPCHK(hipMemcpy(x.p, hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(v.p, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(mol.p, hm.data(), n * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(tag.p, ht.data(), n * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(orig.p, ho.data(), n * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(sysid.p, hs.data(), n * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemset(f.p, 0, n * sizeof(double4)));
-    }
+    if (n) HIPCHK(hipMemsetAsync(f.p, 0, n * sizeof(double4), stream));
     upload_systab();
     // reduction chunks: never straddle a system
     std::vector<Chunk> ch;
@@ -386,6 +383,11 @@ struct Ctx {
     need_rebuild = true;
     forces_valid = false;
     gen++;
+  }
+
+  static void adopt(DBuf &dst, DBuf &src) {
+    std::swap(dst.p, src.p);
+    std::swap(dst.bytes, src.bytes);
   }
 
   void alloc_atoms(size_t m) {
@@ -458,7 +460,7 @@ struct Ctx {
     P.fil = fil.as<double4>(); P.S = S.as<double4>(); P.T = T.as<double>();
     P.th = th.as<double>(); P.vatom = vatom.as<double>();
     P.ip = (const IlpPair<R> *)(sizeof(R) == 8 ? ip_d.p : ip_f.p);
-    P.tp = (const TersP<R> *)(sizeof(R) == 8 ? tp_d.p : tp_f.p);
+    P.tp = (const TersP<double> *)tp_d.p;
     P.t3 = t3.as<int>();
     P.err = errw.as<int>();
     P.ovf = ovf.as<int>();
@@ -617,7 +619,10 @@ struct Ctx {
   // ---------------------------------------------------------------- thermo
   // columns: 0 e_vdw 1 e_rep 2 e_tersoff 3-8 virial 9 ke
   void reduce(int ncols, double *dst_dev) {
-    if (nchunks == 0) return;
+    if (nchunks == 0) {
+      HIPCHK(hipMemsetAsync(dst_dev, 0, sizeof(double) * ncols * std::max<size_t>(sys.size(), 1), stream));
+      return;
+    }
     hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(BS), 0, stream, (const double *)th.p, (size_t)n, ncols,
                        chunks.as<Chunk>(), partial.as<double2>());
     hipLaunchKernelGGL(k_reduce_systems, dim3((unsigned)sys.size()), dim3(64), 0, stream, (const double2 *)partial.p,
@@ -1002,6 +1007,7 @@ int lmff_system_add(lmff_ctx *ctx, int64_t natoms, const double boxlo[3], const 
     SysHost s{};
     s.id = c.next_id++;
     s.live = true;
+    s.staged = true;
     s.n = natoms;
     for (int d = 0; d < 3; d++) {
       s.lo[d] = boxlo[d];
@@ -1340,6 +1346,27 @@ int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, 
     for (int k = 0; k < cnt && k < capacity; k++)
       for (int d = 0; d < 4; d++) rows[4 * k + d] = r[k][d];
     *nout = cnt;
+  });
+}
+
+int lmff_pair_stats(lmff_ctx *ctx, int64_t out[4]) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!out) fail(LMFF_ERR_ARG, "null output");
+    c.relayout();
+    if (c.need_rebuild) c.rebuild();
+    DBuf cnt;
+    cnt.ensure(4 * sizeof(unsigned long long));
+    HIPCHK(hipMemsetAsync(cnt.p, 0, 4 * sizeof(unsigned long long), c.stream));
+    if (c.n) {
+      auto P = c.ptrs<double>();
+      hipLaunchKernelGGL(k_pair_stats<double>, dim3((c.n + BS - 1) / BS), dim3(BS), 0, c.stream, P,
+                         cnt.as<unsigned long long>());
+    }
+    unsigned long long h[4];
+    HIPCHK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 4; k++) out[k] = (int64_t)h[k];
   });
 }
 

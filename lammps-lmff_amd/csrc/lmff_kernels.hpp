@@ -99,7 +99,7 @@ struct DevPtrs {
   double *vatom;      // [6][n]
   // parameters
   const IlpPair<R> *ip;   // [MAXT][MAXT], types 0-based
-  const TersP<R> *tp;     // [nparams]
+  const TersP<double> *tp;  // [nparams]; Tersoff math is FP64 in both precision modes
   const int *t3;          // [MAXT][MAXT][MAXT] -> tp index or -1
   int *err;
   int *ovf;               // [0] max short count, [1] max ILP count
@@ -190,6 +190,11 @@ __global__ void k_wrap_key(int n, double4 *x, const int *sys, const SysDev *st, 
   x[p] = xp;
   key[p] = (uint32_t)(s.cell_off + (cell[2] * s.nc[1] + cell[1]) * s.nc[0] + cell[0]);
   val[p] = p;
+}
+
+__global__ void k_fill_int(int n, int *dst, int value) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = value;
 }
 
 template <typename T>
@@ -792,7 +797,8 @@ __device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R r
 // Per step 3: zeta, b_ij and force_zeta for every Tersoff bond of i (pair_tersoff.cpp:177-225)
 template <typename R, bool EF>
 __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
-  using M = MathOps<R>;
+  using RT = double;
+  using M = MathOps<RT>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
@@ -800,8 +806,8 @@ __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
   const int ti = (int)xi.w - 1;
   const int ns = P.tcnt[i];
   const uint32_t *row = P.tsh + (size_t)i * MAXSH;
-  R del[MAXSH][3];
-  R rsqs[MAXSH];
+  RT del[MAXSH][3];
+  RT rsqs[MAXSH];
   int tt[MAXSH];
   for (int a = 0; a < ns; a++) {
     const uint32_t e = row[a];
@@ -809,47 +815,47 @@ __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
     double gx, gy, gz;
     ghost(xj, e, s.prd, gx, gy, gz);
     const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
-    del[a][0] = (R)d0; del[a][1] = (R)d1; del[a][2] = (R)d2;
-    rsqs[a] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+    del[a][0] = d0; del[a][1] = d1; del[a][2] = d2;
+    rsqs[a] = d0 * d0 + d1 * d1 + d2 * d2;
     tt[a] = (int)xj.w - 1;
   }
   for (int jj = 0; jj < ns; jj++) {
     const size_t slot = (size_t)i * MAXSH + jj;
     const int pij = P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[jj]];
-    const TersP<R> &p = P.tp[pij];
+    const TersP<RT> &p = P.tp[pij];
     if ((double)rsqs[jj] >= p.cutsq) {
       P.fz[slot] = 0;
       P.pref[slot] = 0;
       if (EF) P.eb[slot] = 0;
       continue;
     }
-    R zeta = 0;
-    const R rij = M::sq(rsqs[jj]);
+    RT zeta = 0;
+    const RT rij = M::sq(rsqs[jj]);
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      const TersP<R> &q = P.tp[P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[kk]]];
+      const TersP<RT> &q = P.tp[P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[kk]]];
       if ((double)rsqs[kk] >= q.cutsq) continue;
-      const R rik = M::sq(rsqs[kk]);
-      const R costheta = (del[jj][0] * del[kk][0] + del[jj][1] * del[kk][1] + del[jj][2] * del[kk][2]) / (rij * rik);
-      R dd;
-      const R ex = t_exdelr(rij, rik, q, dd);
+      const RT rik = M::sq(rsqs[kk]);
+      const RT costheta = (del[jj][0] * del[kk][0] + del[jj][1] * del[kk][1] + del[jj][2] * del[kk][2]) / (rij * rik);
+      RT dd;
+      const RT ex = t_exdelr(rij, rik, q, dd);
       zeta += t_fc(rik, q) * t_gijk(costheta, q) * ex;
     }
     // force_zeta (pair_tersoff.cpp:621-640)
-    const R r = rij;
-    R fa, fa_d;
+    const RT r = rij;
+    RT fa, fa_d;
     if (r > p.bigr + p.bigd) {
       fa = 0;
       fa_d = 0;
     } else {
-      const R e2 = M::ex(-p.lam2 * r);
+      const RT e2 = M::ex(-p.lam2 * r);
       fa = -p.bigb * e2 * t_fc(r, p);
       fa_d = p.bigb * e2 * (p.lam2 * t_fc(r, p) - t_fc_d(r, p));
     }
-    const R bij = t_bij(zeta, p);
-    P.fz[slot] = (double)((R)0.5 * bij * fa_d / r);
-    P.pref[slot] = (double)((R)-0.5 * fa * t_bij_d(zeta, p));
-    if (EF) P.eb[slot] = (double)((R)0.5 * bij * fa);
+    const RT bij = t_bij(zeta, p);
+    P.fz[slot] = (double)((RT)0.5 * bij * fa_d / r);
+    P.pref[slot] = (double)((RT)-0.5 * fa * t_bij_d(zeta, p));
+    if (EF) P.eb[slot] = (double)((RT)0.5 * bij * fa);
   }
 }
 
@@ -932,8 +938,8 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   const int ns = P.tcnt[a];
   const uint32_t *trow = P.tsh + (size_t)a * MAXSH;
   const int atag = P.tag[a];
-  R del[MAXSH][3];
-  R rsqs[MAXSH];
+  double del[MAXSH][3];
+  double rsqs[MAXSH];
   int tt[MAXSH];
   for (int jj = 0; jj < ns; jj++) {
     const uint32_t e = trow[jj];
@@ -942,19 +948,19 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     double gx, gy, gz;
     ghost(xj, e, prd, gx, gy, gz);
     const double d0 = gx - xa.x, d1 = gy - xa.y, d2 = gz - xa.z;
-    del[jj][0] = (R)d0; del[jj][1] = (R)d1; del[jj][2] = (R)d2;
-    rsqs[jj] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+    del[jj][0] = d0; del[jj][1] = d1; del[jj][2] = d2;
+    rsqs[jj] = d0 * d0 + d1 * d1 + d2 * d2;
     tt[jj] = (int)xj.w - 1;
     // repulsion: parameter of whichever atom computes the pair in the reference (tag parity)
     const bool skip = tag_skip(atag, P.tag[j], xa.x, xa.y, xa.z, gx, gy, gz);
     const int pr = skip ? P.t3[(tt[jj] * MAXT + ta) * MAXT + ta] : P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[jj]];
-    const TersP<R> &p = P.tp[pr];
+    const TersP<double> &p = P.tp[pr];
     const double rsqd = d0 * d0 + d1 * d1 + d2 * d2;
     if (rsqd < p.cutsq) {
-      const R r = M::sq(rsqs[jj]);
-      const R tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
-      const R tmp_exp = M::ex(-p.lam1 * r);
-      const R fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
+      const double r = sqrt(rsqs[jj]);
+      const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
+      const double tmp_exp = exp(-p.lam1 * r);
+      const double fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
       // del here is x_j - x_a; the reference's del is x_a - x_j
       F.x -= (double)(del[jj][0] * fpair);
       F.y -= (double)(del[jj][1] * fpair);
@@ -969,9 +975,9 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   }
   for (int jj = 0; jj < ns; jj++) {
     const size_t slot = (size_t)a * MAXSH + jj;
-    const R fzv = (R)P.fz[slot];
-    const R pref = (R)P.pref[slot];
-    if (fzv == (R)0 && pref == (R)0) continue;
+    const double fzv = P.fz[slot];
+    const double pref = P.pref[slot];
+    if (fzv == 0.0 && pref == 0.0) continue;
     F.x += (double)(del[jj][0] * fzv);
     F.y += (double)(del[jj][1] * fzv);
     F.z += (double)(del[jj][2] * fzv);
@@ -984,9 +990,9 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     }
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      const TersP<R> &q = P.tp[P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[kk]]];
+      const TersP<double> &q = P.tp[P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[kk]]];
       if ((double)rsqs[kk] >= q.cutsq) continue;
-      R fi[3], fj[3], fk[3];
+      double fi[3], fj[3], fk[3];
       t_attractive(q, pref, rsqs[jj], rsqs[kk], del[jj], del[kk], fi, fj, fk);
       F.x += (double)fi[0];
       F.y += (double)fi[1];
@@ -1006,8 +1012,8 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     if (slot_a < 0) continue;
     const double4 xi = P.x[i];
     const int ti = (int)xi.w - 1;
-    R dl[MAXSH][3];
-    R rq[MAXSH];
+    double dl[MAXSH][3];
+    double rq[MAXSH];
     int ty[MAXSH];
     for (int b = 0; b < ni; b++) {
       const uint32_t e = irow[b];
@@ -1015,16 +1021,16 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       double gx, gy, gz;
       ghost(xb, e, prd, gx, gy, gz);
       const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
-      dl[b][0] = (R)d0; dl[b][1] = (R)d1; dl[b][2] = (R)d2;
-      rq[b] = (R)(d0 * d0 + d1 * d1 + d2 * d2);
+      dl[b][0] = d0; dl[b][1] = d1; dl[b][2] = d2;
+      rq[b] = d0 * d0 + d1 * d1 + d2 * d2;
       ty[b] = (int)xb.w - 1;
     }
     // a as j of bond (i, a)
     {
       const size_t slot = (size_t)i * MAXSH + slot_a;
-      const R fzv = (R)P.fz[slot];
-      const R pref = (R)P.pref[slot];
-      if (!(fzv == (R)0 && pref == (R)0)) {
+      const double fzv = P.fz[slot];
+      const double pref = P.pref[slot];
+      if (!(fzv == 0.0 && pref == 0.0)) {
         F.x -= (double)(dl[slot_a][0] * fzv);
         F.y -= (double)(dl[slot_a][1] * fzv);
         F.z -= (double)(dl[slot_a][2] * fzv);
@@ -1037,9 +1043,9 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
         }
         for (int kk = 0; kk < ni; kk++) {
           if (kk == slot_a) continue;
-          const TersP<R> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + ty[kk]]];
+          const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + ty[kk]]];
           if ((double)rq[kk] >= qp.cutsq) continue;
-          R fi[3], fj[3], fk[3];
+          double fi[3], fj[3], fk[3];
           t_attractive(qp, pref, rq[slot_a], rq[kk], dl[slot_a], dl[kk], fi, fj, fk);
           F.x += (double)fj[0];
           F.y += (double)fj[1];
@@ -1052,12 +1058,12 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     for (int jj = 0; jj < ni; jj++) {
       if (jj == slot_a) continue;
       const size_t slot = (size_t)i * MAXSH + jj;
-      const R fzv = (R)P.fz[slot];
-      const R pref = (R)P.pref[slot];
-      if (fzv == (R)0 && pref == (R)0) continue;
-      const TersP<R> &qp = P.tp[P.t3[(ti * MAXT + ty[jj]) * MAXT + ta]];
+      const double fzv = P.fz[slot];
+      const double pref = P.pref[slot];
+      if (fzv == 0.0 && pref == 0.0) continue;
+      const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ty[jj]) * MAXT + ta]];
       if ((double)rq[slot_a] >= qp.cutsq) continue;
-      R fi[3], fj[3], fk[3];
+      double fi[3], fj[3], fk[3];
       t_attractive(qp, pref, rq[jj], rq[slot_a], dl[jj], dl[slot_a], fi, fj, fk);
       F.x += (double)fk[0];
       F.y += (double)fk[1];
@@ -1209,6 +1215,30 @@ __global__ void k_displacement(int n, const double4 *x, const double4 *xhold, co
   const double dx = x[p].x - xhold[p].x, dy = x[p].y - xhold[p].y, dz = x[p].z - xhold[p].z;
   if (dx * dx + dy * dy + dz * dz > trigger_sq) atomicOr(flag, 1);
   (void)sys;
+}
+
+// counts: [0] ILP list entries, [1] ILP ordered pairs within rc, [2] Tersoff neighbours within
+// cutmax, [3] short-list entries
+template <typename R>
+__global__ void k_pair_stats(DevPtrs<R> P, unsigned long long *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SysDev &s = P.sys_tab[P.sys[i]];
+  const double4 xi = P.x[i];
+  const int cnt = P.icnt[i];
+  const uint32_t *row = P.ilist + (size_t)i * P.icap;
+  unsigned long long np = 0;
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = row[k];
+    double gx, gy, gz;
+    ghost(P.x[e_idx(e)], e, s.prd, gx, gy, gz);
+    const double dx = xi.x - gx, dy = xi.y - gy, dz = xi.z - gz;
+    if (dx * dx + dy * dy + dz * dz < P.ilp_cutsq) np++;
+  }
+  atomicAdd(&out[0], (unsigned long long)cnt);
+  atomicAdd(&out[1], np);
+  atomicAdd(&out[2], (unsigned long long)P.tcnt[i]);
+  atomicAdd(&out[3], (unsigned long long)P.scnt[i]);
 }
 
 }  // namespace lmff

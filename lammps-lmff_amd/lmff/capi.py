@@ -29,6 +29,7 @@ SYMBOLS = [
     "lmff_system_remove", "lmff_system_integrator", "lmff_system_natoms", "lmff_system_get", "lmff_system_set",
     "lmff_system_peratom", "lmff_evaluate", "lmff_system_thermo", "lmff_run", "lmff_profile",
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
+    "lmff_pair_stats",
 ]
 
 
@@ -95,6 +96,7 @@ def load(path: str = LIB_PATH):
     L.lmff_system_bytes.restype = C.c_int64
     L.lmff_debug_ilp_list.argtypes = [vp, C.c_int, C.c_int64, _i32, C.c_int, _int]
     L.lmff_debug_normals.argtypes = [vp, C.c_int, _d, _i32]
+    L.lmff_pair_stats.argtypes = [vp, _i64]
     _lib = L
     return L
 
@@ -280,6 +282,12 @@ class Context:
         n = C.c_int()
         self._chk(self.L.lmff_debug_ilp_list(self.h, int(sid), int(atom), _p(rows, C.c_int32), cap, C.byref(n)))
         return rows[: min(n.value, cap)]
+
+    def pair_stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        self._chk(self.L.lmff_pair_stats(self.h, _p(out, C.c_int64)))
+        return {"ilp_entries": int(out[0]), "ilp_pairs": int(out[1]), "tersoff_nbrs": int(out[2]),
+                "short_entries": int(out[3])}
 
     def normals(self, sid: int):
         n = self.natoms(sid)

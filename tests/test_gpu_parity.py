@@ -1,0 +1,278 @@
+"""GPU parity of the HIP path against the oracle, through the C ABI.
+
+Tolerances (stated here, used below):
+  FP64:  max_i |F_gpu - F_ref| / RMS(|F_ref|) <= 1e-10; |dE|/|E| <= 1e-10 for the total and
+         each of E_vdW, E_rep, E_Tersoff; max |d virial| / max |virial| <= 1e-10;
+         per-atom energy and virial: max |d| <= 1e-10 * max |ref|.
+  mixed: RMS_i |F_mixed - F_ref| <= 1e-5 eV/A, |dE|/|E| <= 1e-6.
+  neighbor sets: bit-exact (sorted (j, image shift) rows).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, POT
+from lmff import capi, structures as S
+
+pytestmark = pytest.mark.gpu
+
+CT = os.path.join(POT, "C.opt.tersoff")
+BNC = os.path.join(POT, "BNC.tersoff")
+ILP = os.path.join(POT, "BNCH.ILP")
+
+
+def oracle_for(s, tap=1, telems=None, ielems=None, threads=8):
+    import oracle
+
+    if telems is None:
+        if s.elements == ["C"]:
+            telems, ielems, tf = ["C(O)"], ["C"], CT
+        else:
+            telems, ielems, tf = list(s.elements), list(s.elements), BNC
+    else:
+        tf = CT if telems[0] == "C(O)" else BNC
+    o = oracle.Oracle(len(telems), tf, telems, ILP, ielems, tap_flag=tap, threads=threads)
+    o.set_system(s)
+    o.build_list()
+    return o
+
+
+def ctx_for(s, precision=capi.LMFF_FP64, tap=1, telems=None, ielems=None, masses=None):
+    ctx = capi.Context(0)
+    if telems is None:
+        capi.setup_for(ctx, s, precision=precision, tap=tap)
+    else:
+        capi.setup_graphene(ctx, len(telems), tuple(telems), tuple(ielems),
+                            "C.opt.tersoff" if telems[0] == "C(O)" else "BNC.tersoff",
+                            tuple(masses), tap=tap, precision=precision)
+    return ctx
+
+
+def ribbons():
+    s = S.layered_stack(12, 9, ["G", "G"], seed=12, jitter=0.06)
+    s.boxhi[0] += 30.0
+    s.x[:, 0] += 5.0
+    return s
+
+
+def reference_stack():
+    s = S.read_lammps_data(os.path.join(GOLDEN, "inf0.data"))
+    s.elements = ["C"] * 6
+    return s
+
+
+CASES = {
+    "c1_bilayer_graphene_4800": lambda: S.config_c1(),
+    "graphene_hbn_bilayer_4800": lambda: S.layered_stack(30, 20, ["G", "BN"], seed=5),
+    "trilayer_g_bn_g_3240": lambda: S.layered_stack(15, 9, ["G", "BN", "G"], seed=7),
+    "ribbons_open_edges": ribbons,
+    "twisted_bilayer_5_3": lambda: S.twisted_bilayer(5, 3, seed=3),
+}
+
+
+def _check_fp64(g, r):
+    rms = math.sqrt((r["f"] ** 2).sum(1).mean())
+    assert np.abs(g["f"] - r["f"]).max() / rms <= 1e-10
+    for k in ("energy", "e_vdw", "e_rep", "e_tersoff"):
+        assert abs(g[k] - r[k]) <= 1e-10 * abs(r[k]), k
+    assert np.abs(g["virial"] - r["virial"]).max() <= 1e-10 * np.abs(r["virial"]).max()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_forces_energy_virial(name):
+    s = CASES[name]()
+    r = oracle_for(s).compute(3, 6)
+    with ctx_for(s) as ctx:
+        g = ctx.pair_compute(s, 3, 6)
+    _check_fp64(g, r)
+    assert np.abs(g["eatom"] - r["eatom"]).max() <= 1e-10 * np.abs(r["eatom"]).max()
+    assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
+
+
+@pytest.mark.parametrize("tap", [0, 1])
+def test_fp64_reference_six_layer_deck(tap):
+    """The reference's own 5,016-atom six-layer graphene data file (periodic in x, y and z)."""
+    s = reference_stack()
+    el_t, el_i = ["C(O)"] * 6, ["C"] * 6
+    r = oracle_for(s, tap=tap, telems=el_t, ielems=el_i).compute(3, 6)
+    with ctx_for(s, tap=tap, telems=el_t, ielems=el_i, masses=[12.0107] * 6) as ctx:
+        g = ctx.pair_compute(s, 3, 6)
+    _check_fp64(g, r)
+    assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_force_error(name):
+    s = CASES[name]()
+    r = oracle_for(s).compute(1, 2)
+    with ctx_for(s, precision=capi.LMFF_MIXED) as ctx:
+        g = ctx.pair_compute(s, 1, 2)
+    rms_err = math.sqrt(((g["f"] - r["f"]) ** 2).sum(1).mean())
+    assert rms_err <= 1e-5, rms_err
+    assert abs(g["energy"] - r["energy"]) <= 1e-6 * abs(r["energy"])
+
+
+def test_neighbor_sets_bit_exact():
+    s = S.layered_stack(15, 9, ["G", "BN", "G"], seed=21, jitter=0.07)
+    o = oracle_for(s)
+    offs, ent, sh = o.neighbor_list()
+    with ctx_for(s) as ctx:
+        ctx.pair_compute(s, 1, 0)
+        for i in np.random.default_rng(2).choice(s.n, 64, replace=False):
+            rows = ctx.ilp_list(0, int(i))
+            want = sorted((int(j), *map(int, sh[e])) for e, j in zip(range(offs[i], offs[i + 1]),
+                                                                       ent[offs[i]:offs[i + 1]])
+                          if s.mol[j] != s.mol[i])
+            got = sorted(map(tuple, rows.tolist()))
+            assert got == want, i
+
+
+def test_normals_match_oracle():
+    s = ribbons()
+    nref, cref, _, _ = oracle_for(s).normals()
+    with ctx_for(s) as ctx:
+        ctx.pair_compute(s, 1, 0)
+        nv, cnt = ctx.normals(0)
+    assert (cnt == cref).all()
+    np.testing.assert_allclose(np.abs((nv * nref).sum(1)), 1.0, atol=1e-13)
+
+
+def test_empty_and_ragged_systems():
+    s = S.layered_stack(15, 9, ["G", "G"], seed=1)
+    with ctx_for(s) as ctx:
+        e = S.System(np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros(0, np.int64),
+                     s.boxlo.copy(), s.boxhi.copy(), s.periodic, ["C"])
+        g = ctx.pair_compute(e, 3, 6)
+        assert g["energy"] == 0.0 and g["f"].shape == (0, 3)
+        # ragged: two systems of different size resident together evaluate like each alone
+        a = S.layered_stack(15, 9, ["G", "G"], seed=2)
+        b = S.layered_stack(17, 10, ["G", "G"], seed=3)
+        ia = ctx.add_system(a)
+        ib = ctx.add_system(b)
+        ctx.evaluate(1, 2)
+        ta, tb = ctx.thermo(ia), ctx.thermo(ib)
+        _, _, fa = ctx.get(ia)
+        _, _, fb = ctx.get(ib)
+    for sysm, t, f in ((a, ta, fa), (b, tb, fb)):
+        r = oracle_for(sysm).compute(1, 2)
+        assert abs(t["pe"] - r["energy"]) <= 1e-10 * abs(r["energy"])
+        assert np.abs(f - r["f"]).max() / math.sqrt((r["f"] ** 2).sum(1).mean()) <= 1e-10
+
+
+def test_resident_step_matches_oracle_nve():
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=31)
+    o = oracle_for(s)
+    x_ref, v_ref, th = o.run_nve(s.x, s.v, s.mass_per_type, 0.001, 40, rebuild_every=10, thermo_every=10)
+    with ctx_for(s) as ctx:
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+        ctx.evaluate(1, 2)
+        rec = ctx.run(40, 10, use_graph=True)[0]
+        x, v, f = ctx.get(sid)
+    L = (s.boxhi - s.boxlo)
+    d = x - x_ref
+    d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
+    assert np.abs(d).max() < 1e-9
+    assert np.abs(v - v_ref).max() < 1e-7
+    pe = np.array([r["pe"] for r in rec])
+    np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
+
+
+def test_graph_replay_equals_eager_and_is_deterministic():
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=41)
+    outs = []
+    for graph in (True, False, True):
+        with ctx_for(s) as ctx:
+            sid = ctx.add_system(s)
+            ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+            ctx.evaluate(0, 0)
+            ctx.run(35, 0, use_graph=graph)
+            outs.append(ctx.get(sid))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("style", [capi.LMFF_NVE, capi.LMFF_LANGEVIN])
+def test_alone_equals_in_mix_bitwise(style):
+    systems = [S.twisted_bilayer(3, 2, seed=5), S.layered_stack(15, 9, ["G", "G"], seed=6),
+               S.twisted_bilayer(4, 1, seed=7)]
+    alone = []
+    for s in systems:
+        with ctx_for(s) as ctx:
+            sid = ctx.add_system(s)
+            ctx.integrator(sid, style, dt=0.001, t_target=300.0, damp=0.1, seed=99)
+            ctx.evaluate(0, 0)
+            ctx.run(30, 10)
+            alone.append(ctx.get(sid))
+    with ctx_for(systems[0]) as ctx:
+        ids = []
+        for s in systems:
+            ids.append(ctx.add_system(s))
+            ctx.integrator(ids[-1], style, dt=0.001, t_target=300.0, damp=0.1, seed=99)
+        ctx.evaluate(0, 0)
+        ctx.run(30, 10)
+        mixed = [ctx.get(i) for i in ids]
+    for a, m in zip(alone, mixed):
+        for p, q in zip(a, m):
+            assert np.array_equal(p, q)
+
+
+def test_langevin_drives_temperature():
+    s = S.layered_stack(15, 9, ["G", "G"], seed=8, temperature=50.0)
+    with ctx_for(s) as ctx:
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_LANGEVIN, dt=0.001, t_target=600.0, damp=0.05, seed=3)
+        ctx.evaluate(0, 0)
+        rec = ctx.run(600, 100)[0]
+    assert rec[0]["temp"] < rec[-1]["temp"]
+    assert 400 < np.mean([r["temp"] for r in rec[-2:]]) < 800
+
+
+def test_error_paths():
+    s = S.layered_stack(15, 9, ["G", "G"], seed=9)
+    with capi.Context(0) as ctx:
+        ctx.settings(1, 16.0, 1)
+        with pytest.raises(capi.LmffError) as e:
+            ctx.coeff_tersoff(os.path.join(POT, "missing.tersoff"), ["C(O)"])
+        assert e.value.code == 2
+        with pytest.raises(capi.LmffError) as e:
+            ctx.coeff_tersoff(BNC, ["C(O)"])  # no C(O) entries in BNC.tersoff
+        assert e.value.code == 3
+    with ctx_for(s) as ctx:
+        small_box = s.copy()
+        small_box.boxhi[0] = 30.0
+        with pytest.raises(capi.LmffError) as e:
+            ctx.pair_compute(small_box)
+        assert e.value.code == 9
+        bad = s.copy()
+        bad.x[5, 0] = np.nan
+        with pytest.raises(capi.LmffError) as e:
+            ctx.pair_compute(bad)
+        assert e.value.code == 7
+        # an atom pushed into its layer gets more than three neighbours within 2 A
+        squeezed = s.copy()
+        squeezed.x[0, :2] = squeezed.x[1, :2] + np.array([0.3, 0.2])
+        squeezed.x[0, 2] = squeezed.x[1, 2]
+        with pytest.raises(capi.LmffError) as e:
+            ctx.pair_compute(squeezed)
+        assert e.value.code in (5, 6)
+        # the context stays usable after an error
+        g = ctx.pair_compute(s, 1, 2)
+        assert np.isfinite(g["energy"])
+
+
+def test_full_size_c2_properties():
+    """228,800-atom graphene/hBN bilayer: size-independent checks at the benchmark size."""
+    s = S.config_c2()
+    with ctx_for(s) as ctx:
+        g = ctx.pair_compute(s, 1, 2)
+    with ctx_for(s, precision=capi.LMFF_MIXED) as ctx:
+        m = ctx.pair_compute(s, 1, 2)
+    assert np.abs(g["f"].sum(0)).max() < 1e-8 * s.n
+    assert -7.9 < g["energy"] / s.n < -7.5  # the 4,800-atom cut of the same stack gives -7.716
+    rms_err = math.sqrt(((m["f"] - g["f"]) ** 2).sum(1).mean())
+    assert rms_err <= 1e-5
+    assert abs(m["energy"] - g["energy"]) <= 1e-6 * abs(g["energy"])
