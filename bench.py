@@ -115,7 +115,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=220)
     ap.add_argument("--ny", type=int, default=130)
+    ap.add_argument("--copy-ref", type=int, default=0,
+                    help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_ref:
+        import torch
+
+        a = torch.empty(args.copy_ref // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_ref_bytes": args.copy_ref}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
               f"note, not a measured path", file=sys.stderr)

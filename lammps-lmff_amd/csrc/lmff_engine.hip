@@ -136,9 +136,11 @@ struct Ctx {
   // device state
   DBuf x, v, f, xhold, mol, tag, sysid, orig, key, key2, val, perm, tmp4, tmpi;
   DBuf slist, scnt, ilist, icnt;
-  DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, fz, pref, eb, fil, S, T, th, vatom;
+  DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, fz, pref, eb, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
-  DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag;
+  DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave;
+  // atom ranges whose last-step force is restored after the next set-up force evaluation
+  std::vector<std::pair<int, int>> restore_f;
   int scap = 64, icap = 0;
   size_t sort_tmp_bytes = 0;
   int end_bit = 1;
@@ -167,9 +169,10 @@ struct Ctx {
 
   Ctx() {
     DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
-                   &slist, &scnt, &ilist, &icnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &fz, &pref, &eb,
+                   &slist, &scnt, &ilist, &icnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
+                   &rnc, &rncnt, &fz, &pref, &eb,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
-                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag};
+                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave};
     for (DBuf *b : all) b->tally = &bytes_in_use;
   }
   ~Ctx() {
@@ -305,8 +308,11 @@ struct Ctx {
     }
     if (nn >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)nn);
     const size_t m1 = std::max<size_t>((size_t)nn, 1);
-    DBuf nx, nv, nmol, ntag, norig, nsys;
-    for (DBuf *b : {&nx, &nv, &nmol, &ntag, &norig, &nsys}) b->tally = &bytes_in_use;
+    DBuf nx, nv, nmol, ntag, norig, nsys, nf;
+    for (DBuf *b : {&nx, &nv, &nmol, &ntag, &norig, &nsys, &nf}) b->tally = &bytes_in_use;
+    nf.ensure(m1 * sizeof(double4));
+    HIPCHK(hipMemsetAsync(nf.p, 0, m1 * sizeof(double4), stream));
+    std::vector<std::pair<int, int>> keep_f;
     nx.ensure(m1 * sizeof(double4));
     nv.ensure(m1 * sizeof(double4));
     nmol.ensure(m1 * sizeof(int));
@@ -324,6 +330,11 @@ struct Ctx {
         HIPCHK(hipMemcpyAsync(nmol.as<int>() + s.begin, mol.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(ntag.as<int>() + s.begin, tag.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(norig.as<int>() + s.begin, orig.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        if (forces_valid) {
+          // the force of the last step (pair + post_force terms such as Langevin) stays with the system
+          HIPCHK(hipMemcpyAsync(nf.as<double4>() + s.begin, f.as<double4>() + o, m * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+          keep_f.push_back({s.begin, s.end});
+        }
       } else {
         const Staged &st = *src_stage[k];
         std::vector<double4> hx(m), hv(m);
@@ -349,12 +360,15 @@ struct Ctx {
     adopt(tag, ntag);
     adopt(orig, norig);
     adopt(sysid, nsys);
+    adopt(fsave, nf);
+    const bool had_new = !staged.empty();
     staged.clear();
     sys = next;
     n = nn;
     ncells = nc;
     alloc_atoms((size_t)n);
-    if (n) HIPCHK(hipMemsetAsync(f.p, 0, n * sizeof(double4), stream));
+    if (n) HIPCHK(hipMemcpyAsync(f.p, fsave.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+    restore_f = had_new ? keep_f : std::vector<std::pair<int, int>>{};
     upload_systab();
     // reduction chunks: never straddle a system
     std::vector<Chunk> ch;
@@ -381,7 +395,7 @@ struct Ctx {
     sort_tmp_bytes = tb;
     layout_dirty = false;
     need_rebuild = true;
-    forces_valid = false;
+    forces_valid = forces_valid && !had_new;
     gen++;
   }
 
@@ -403,6 +417,8 @@ struct Ctx {
     nrm.ensure(m * sizeof(double4));
     dn.ensure(m * 27 * sizeof(double)); dndri.ensure(m * 9 * sizeof(double)); vet.ensure(m * 9 * sizeof(double));
     nnb.ensure(m * 4 * sizeof(uint32_t)); tsh.ensure(m * MAXSH * sizeof(uint32_t)); tcnt.ensure(m * sizeof(int));
+    rtc.ensure(m * MAXSH * sizeof(uint32_t)); rtcnt.ensure(m * sizeof(int));
+    rnc.ensure(m * MAXSH * sizeof(uint32_t)); rncnt.ensure(m * sizeof(int));
     fz.ensure(m * MAXSH * sizeof(double)); pref.ensure(m * MAXSH * sizeof(double)); eb.ensure(m * MAXSH * sizeof(double));
     fil.ensure(m * sizeof(double4)); S.ensure(m * sizeof(double4)); T.ensure(m * 6 * sizeof(double));
     th.ensure(m * NTH * sizeof(double)); vatom.ensure(m * 6 * sizeof(double));
@@ -456,6 +472,7 @@ struct Ctx {
     P.nrm = (typename std::conditional<sizeof(R) == 8, double4, float4>::type *)nrm.p;
     P.dn = dn.as<R>(); P.dndri = dndri.as<R>(); P.vet = vet.as<double>();
     P.nnb = nnb.as<uint32_t>(); P.tsh = tsh.as<uint32_t>(); P.tcnt = tcnt.as<int>();
+    P.rtc = rtc.as<uint32_t>(); P.rtcnt = rtcnt.as<int>(); P.rnc = rnc.as<uint32_t>(); P.rncnt = rncnt.as<int>();
     P.fz = fz.as<double>(); P.pref = pref.as<double>(); P.eb = eb.as<double>();
     P.fil = fil.as<double4>(); P.S = S.as<double4>(); P.T = T.as<double>();
     P.th = th.as<double>(); P.vatom = vatom.as<double>();
@@ -512,6 +529,8 @@ struct Ctx {
                                               perm.as<int>(), (int)n, 0, end_bit, stream));
     permute4(x);
     permute4(v);
+    permute4(f);  // forces kept across a set-up rebuild stay with their atoms
+    if (!restore_f.empty()) permute4(fsave);
     permutei(mol);
     permutei(tag);
     permutei(orig);
@@ -551,10 +570,11 @@ struct Ctx {
         scap = ((int)(o[0] * 1.25) + 63) / 64 * 64;
         grow = true;
       }
-      if (o[1] > 0 || icap == 0) {
-        const int need = std::max(o[1], o[2]);
-        icap = ((int)(need * 1.15) + 64 + 63) / 64 * 64;
+      if (o[1] > 0) {
+        icap = ((int)(o[1] * 1.15) + 64 + 63) / 64 * 64;
         grow = true;
+      } else if (icap == 0) {
+        icap = 64;  // no atom has an ILP neighbour
       }
       if (!grow) return;
       gen++;  // list pointers / capacities change: captured graphs are stale
@@ -584,10 +604,11 @@ struct Ctx {
     else hipLaunchKernelGGL((k_tersoff_zeta<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
     mark_end(3, mk);
     mark_begin(4, mk);
-    if (va) hipLaunchKernelGGL((k_gather<R, true, true, true>), dim3(nb1), dim3(BS), 0, stream, P);
-    else if (vf) hipLaunchKernelGGL((k_gather<R, true, true, false>), dim3(nb1), dim3(BS), 0, stream, P);
-    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, false>), dim3(nb1), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_gather<R, false, false, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
+    if (va) hipLaunchKernelGGL((k_gather<R, true, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_gather<R, true, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_gather<R, false, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
     mark_end(4, mk);
   }
 
@@ -700,10 +721,16 @@ struct Ctx {
   void run(int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out, int64_t out_cap) {
     relayout();
     if (n == 0) return;
-    if (need_rebuild || !forces_valid) {
-      if (need_rebuild) rebuild();
+    // Verlet set-up: lists if due; forces only where no valid force of the last step exists
+    // (systems that kept theirs through a re-pack get it back bit for bit)
+    if (need_rebuild) rebuild();
+    if (!forces_valid) {
       force(0, 0);
+      for (auto &r : restore_f)
+        HIPCHK(hipMemcpyAsync(f.as<double4>() + r.first, fsave.as<double4>() + r.first,
+                              (size_t)(r.second - r.first) * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     }
+    restore_f.clear();
     const int nsys = (int)sys.size();
     int64_t nrec = 0;
     std::vector<int64_t> rec_steps;

@@ -4,7 +4,8 @@
 // resident system):
 //   x, v, f        double4 per atom (x.w holds the atom type as an exact double)
 //   mol, tag, sys  int32 per atom; orig = index of the atom in its system's caller order
-//   slist          [n][scap] uint32: every atom within max(Tersoff cutmax, ILP rcut)+skin
+//   slist          [scap][n] uint32 (transposed): every atom within max(Tersoff cutmax,
+//                  ILP rcut)+skin
 //   ilist          [n][icap] uint32: other-molecule atoms within ILP cut + skin (the
 //                  layer-filtered ILP list)
 //   list entries   bits 0-25 atom index, bits 26-27/28-29/30-31 image shift+1 in x/y/z
@@ -92,6 +93,8 @@ struct DevPtrs {
   double *vet;        // [n][9] double, positions of normal neighbours relative to the atom
   uint32_t *nnb;      // [n][4]
   uint32_t *tsh; int *tcnt;
+  uint32_t *rtc; int *rtcnt;  // [n][MAXSH] atoms that may hold i in their Tersoff list
+  uint32_t *rnc; int *rncnt;  // [n][MAXSH] atoms that may hold i as a normal neighbour
   double *fz, *pref, *eb;
   double4 *fil, *S;
   double *T;          // [n][6] (per-atom virial only)
@@ -115,6 +118,23 @@ template <> struct MathOps<double> {
   static __device__ __forceinline__ double ex(double a) { return exp(a); }
   static __device__ __forceinline__ double sq(double a) { return sqrt(a); }
   static __device__ __forceinline__ double inv(double a) { return 1.0 / a; }
+  // v_rsq_f64 / v_rcp_f64 seeds refined by two Newton steps (relative error at the last
+  // bits of a double, not correctly rounded)
+  static __device__ __forceinline__ double rsqrt(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    y = y * fma(-h, y * y, 1.5);
+    y = y * fma(-h, y * y, 1.5);
+    return y;
+  }
+  static __device__ __forceinline__ double rcp(double a) {
+    double y = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-a, y, 1.0);
+    y = fma(y, e, y);
+    return y;
+  }
   static __device__ __forceinline__ double sn(double a) { return sin(a); }
   static __device__ __forceinline__ double cs(double a) { return cos(a); }
   static __device__ __forceinline__ double pw(double a, double b) { return pow(a, b); }
@@ -123,6 +143,8 @@ template <> struct MathOps<float> {
   static __device__ __forceinline__ float ex(float a) { return __expf(a); }
   static __device__ __forceinline__ float sq(float a) { return __fsqrt_rn(a); }
   static __device__ __forceinline__ float inv(float a) { return __builtin_amdgcn_rcpf(a); }
+  static __device__ __forceinline__ float rsqrt(float a) { return __builtin_amdgcn_rsqf(a); }
+  static __device__ __forceinline__ float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
   static __device__ __forceinline__ float sn(float a) { return sinf(a); }
   static __device__ __forceinline__ float cs(float a) { return cosf(a); }
   static __device__ __forceinline__ float pw(float a, float b) { return powf(a, b); }
@@ -236,7 +258,6 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     }
   }
   int ns = 0, ni = 0;
-  uint32_t *srow = P.slist + (size_t)i * P.scap;
   uint32_t *irow = P.ilist + (size_t)i * P.icap;
   for (int dz = -2; dz <= 2; dz++) {
     int cz = c[2] + dz, sz = 0;
@@ -297,7 +318,9 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
           const unsigned long long bi = __ballot(in_i), bs = __ballot(in_s);
           const int pi = __popcll(bi & ((1ull << lane) - 1ull)), ps = __popcll(bs & ((1ull << lane) - 1ull));
           if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
-          if (in_s && ns + ps < P.scap) srow[ns + ps] = ent;
+          // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
+          // readers coalesce
+          if (in_s && ns + ps < P.scap) P.slist[(size_t)(ns + ps) * P.n + i] = ent;
           ni += __popcll(bi);
           ns += __popcll(bs);
         }
@@ -308,10 +331,10 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   if (lane == 0) {
     P.icnt[i] = ni < P.icap ? ni : P.icap;
     P.scnt[i] = ns < P.scap ? ns : P.scap;
+    // only an overflowing row reports its size (a per-wave atomic on one address would
+    // serialize every wave of the launch)
     if (ni > P.icap) atomicMax(&P.ovf[1], ni);
     if (ns > P.scap) atomicMax(&P.ovf[0], ns);
-    atomicMax(&P.ovf[2], ni);
-    atomicMax(&P.ovf[3], ns);
   }
 }
 
@@ -329,13 +352,20 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   const int ti = (int)xi.w - 1;
   const int mi = P.mol[i];
   const int cnt = P.scnt[i];
-  const uint32_t *row = P.slist + (size_t)i * P.scap;
-  int ns = 0, nn = 0;
+  const uint32_t *row = P.slist + i;  // transposed: entry k at row[k * n]
+  const size_t sstride = (size_t)P.n;
+  int ns = 0, nn = 0, nrt = 0, nrn = 0;
   uint32_t nb[3] = {0, 0, 0};
   double vetd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   bool too_many = false;
+  // reverse candidates: every atom j that may list i as a Tersoff neighbour or as a normal
+  // neighbour (j's own test runs from j's side, so the cutoffs are widened by 1e-12 to form a
+  // superset whatever the last-bit rounding of j's r^2)
+  const double widen = 1.0 + 1e-12;
+  uint32_t *rt_row = P.rtc + (size_t)i * MAXSH;
+  uint32_t *rn_row = P.rnc + (size_t)i * MAXSH;
   for (int k = 0; k < cnt; k++) {
-    const uint32_t e = row[k];
+    const uint32_t e = row[k * sstride];
     const int j = e_idx(e);
     const double4 xj = P.x[j];
     double gx, gy, gz;
@@ -343,11 +373,23 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
     const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
     const double rsq = delx * delx + dely * dely + delz * delz;
     const int tj = (int)xj.w - 1;
-    if (rsq < P.tersoff_cutmaxsq && P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0) {
+    const bool tmapped = P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0;
+    if (rsq < P.tersoff_cutmaxsq && tmapped) {
       if (ns < MAXSH) P.tsh[(size_t)i * MAXSH + ns] = e;
       ns++;
     }
+    if (tmapped && rsq < P.tersoff_cutmaxsq * widen) {
+      if (nrt < MAXSH) rt_row[nrt] = (uint32_t)j;
+      nrt++;
+    }
     const IlpPair<R> &pp = P.ip[ti * MAXT + tj];
+    if (pp.mapped && P.mol[j] == mi && rsq != 0.0) {
+      const double rc2 = fmax(pp.rcutsq, P.ip[tj * MAXT + ti].rcutsq) * widen;
+      if (rsq < rc2) {
+        if (nrn < MAXSH) rn_row[nrn] = (uint32_t)j;
+        nrn++;
+      }
+    }
     if (pp.mapped && P.mol[j] == mi && rsq != 0.0 && rsq < pp.rcutsq) {
       if (nn < 3) {
         nb[nn] = e;
@@ -360,11 +402,15 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
       nn++;
     }
   }
-  if (ns > MAXSH) {
+  if (ns > MAXSH || nrt > MAXSH || nrn > MAXSH) {
     atomicOr(P.err, EB_SHORT);
-    ns = MAXSH;
+    ns = ns < MAXSH ? ns : MAXSH;
+    nrt = nrt < MAXSH ? nrt : MAXSH;
+    nrn = nrn < MAXSH ? nrn : MAXSH;
   }
   P.tcnt[i] = ns;
+  P.rtcnt[i] = nrt;
+  P.rncnt[i] = nrn;
   if (too_many) {
     atomicOr(P.err, EB_NORMAL_COUNT);
     nn = 3;
@@ -513,7 +559,8 @@ __device__ __forceinline__ Tap<R> calc_tap(R r, R inv_rcut, int tap_flag) {
 template <typename R, bool EF, bool VF, bool VA>
 __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // one wavefront per atom: the atom index is wave-uniform (scalar registers)
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
@@ -533,7 +580,14 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
     const int j = e_idx(e);
     const double4 xj = P.x[j];
     double gx, gy, gz;
-    ghost(xj, e, prd, gx, gy, gz);
+    // x_j + 0*prd == x_j exactly, so the image arithmetic is skipped when no lane needs it
+    if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
+      gx = xj.x;
+      gy = xj.y;
+      gz = xj.z;
+    } else {
+      ghost(xj, e, prd, gx, gy, gz);
+    }
     const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
     const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
     if (rsqd >= P.ilp_cutsq) continue;
@@ -541,8 +595,8 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
     const IlpPair<R> &pij = P.ip[ti * MAXT + tj];
     const IlpPair<R> &pji = P.ip[tj * MAXT + ti];
     const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd, rsq = (R)rsqd;
-    const R r = M::sq(rsq);
-    const R rinv = M::inv(r);
+    const R rinv = M::rsqrt(rsq);
+    const R r = rsq * rinv;
     const Tap<R> tp = calc_tap<R>(r, inv_rcut, P.tap_flag);
     // ---- van der Waals, one half per ordered pair ----
     const IlpPair<R> *pv = &pij;
@@ -551,7 +605,7 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
     const R r6inv = r2inv * r2inv * r2inv;
     const R r8inv = r6inv * r2inv;
     const R TS = (R)1 + M::ex(-pv->d * (r * pv->inv_seff - (R)1));
-    const R TSinv = M::inv(TS);
+    const R TSinv = M::rcp(TS);
     const R Vvdw = -pv->C6 * r6inv * TSinv;
     const R fpv = (R)-6 * pv->C6 * r8inv * TSinv + pv->C6_d_seff * (TS - (R)1) * TSinv * TSinv * r8inv * r;
     const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
@@ -588,9 +642,10 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
     const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
     const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
     const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
-    fx += (double)(fsv * dx) + (double)fkx - (double)gkx;
-    fy += (double)(fsv * dy) + (double)fky - (double)gky;
-    fz += (double)(fsv * dz) + (double)fkz - (double)gkz;
+    // the pair's three force terms are combined in R, then accumulated in FP64
+    fx += (double)(fsv * dx + fkx - gkx);
+    fy += (double)(fsv * dy + fky - gky);
+    fz += (double)(fsv * dz + fkz - gkz);
     sx += (double)(wgt * dx);
     sy += (double)(wgt * dy);
     sz += (double)(wgt * dz);
@@ -872,22 +927,27 @@ __device__ __forceinline__ void tally3(double *va, const R *delr1, const R *fj, 
 
 // Per step 4: everything that acts on atom a besides the ILP pair sum -- the normal-derivative
 // forces (own d n_a / d x_a term and a as a normal neighbour of m), Tersoff repulsion, and the
-// Tersoff bond-order forces with a as i, as j and as k.  One thread per atom.
+// Tersoff bond-order forces with a as i, as j and as k.  Four lanes per atom: lane 0 takes
+// the atom's own terms (normal derivatives, repulsion, bonds with a as i), lanes 1-3 take one
+// reverse Tersoff partner i each (a as j of bond i-a and as k of the bonds of i); the quad is
+// summed with a fixed-order shuffle butterfly.
 template <typename R, bool EF, bool VF, bool VA>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
-  using M = MathOps<R>;
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= P.n) return;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt >> 2;
+  const int quad = gt & 3;
+  if (a >= P.n) return;  // the whole quad leaves together
   const SysDev &s = P.sys_tab[P.sys[a]];
   const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
   const double4 xa = P.x[a];
   const int ta = (int)xa.w - 1;
-  const int ma = P.mol[a];
   const size_t n = (size_t)P.n;
-  double4 F = P.fil[a];
+  double4 F = {0.0, 0.0, 0.0, 0.0};
   double et = 0.0;
   double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff tallies (also the Tersoff global virial share)
   double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
+  if (quad == 0) {
+  F = P.fil[a];
   // ---- ILP: d n_a / d x_a term (2-neighbour normals only) ----
   const uint32_t *na = P.nnb + (size_t)a * 4;
   if (na[0] == 2) {
@@ -898,12 +958,11 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       (&F.x)[c] -= v;
     }
   }
-  // ---- ILP: a as a normal neighbour of m ----
-  const int sc = P.scnt[a];
-  const uint32_t *srow = P.slist + (size_t)a * P.scap;
-  for (int q = 0; q < sc; q++) {
-    const int m = e_idx(srow[q]);
-    if (P.mol[m] != ma) continue;
+  // ---- ILP: a as a normal neighbour of m (m among a's reverse normal candidates) ----
+  const int rn = P.rncnt[a];
+  const uint32_t *rnrow = P.rnc + (size_t)a * MAXSH;
+  for (int q = 0; q < rn; q++) {
+    const int m = (int)rnrow[q];
     const uint32_t *nm = P.nnb + (size_t)m * 4;
     const int cm = (int)nm[0];
     for (int kk = 0; kk < cm; kk++) {
@@ -1000,9 +1059,13 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       if (VF || VA) tally3(va, del[jj], fj, del[kk], fk);
     }
   }
-  // ---- Tersoff: a as j (bond i-a) and as k (bond i-j through a), i in a's short list ----
-  for (int q = 0; q < sc; q++) {
-    const int i = e_idx(srow[q]);
+  } else {
+  // ---- Tersoff: a as j (bond i-a) and as k (bond i-j through a), i among a's reverse
+  //      Tersoff candidates (lane q takes candidates q-1, q+2, ...) ----
+  const int rt = P.rtcnt[a];
+  const uint32_t *rtrow = P.rtc + (size_t)a * MAXSH;
+  for (int q = quad - 1; q < rt; q += 3) {
+    const int i = (int)rtrow[q];
     const int ni = P.tcnt[i];
     if (ni == 0) continue;
     const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
@@ -1071,6 +1134,20 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       if (VF || VA) tally3(va, dl[jj], fj, dl[slot_a], fk);
     }
   }
+  }
+  // quad reduction, the same order in every lane
+#pragma unroll
+  for (int o = 1; o <= 2; o <<= 1) {
+    F.x += __shfl_xor(F.x, o, 64);
+    F.y += __shfl_xor(F.y, o, 64);
+    F.z += __shfl_xor(F.z, o, 64);
+    if (EF) et += __shfl_xor(et, o, 64);
+    if (VF || VA)
+      for (int c = 0; c < 6; c++) va[c] += __shfl_xor(va[c], o, 64);
+    if (VA)
+      for (int c = 0; c < 6; c++) vk[c] += __shfl_xor(vk[c], o, 64);
+  }
+  if (quad != 0) return;
   F.w = 0.0;
   P.f[a] = F;
   if (EF) P.th[2 * n + a] = et;
