@@ -600,8 +600,9 @@ struct Ctx {
     else hipLaunchKernelGGL((k_ilp<R, false, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
     mark_end(2, mk);
     mark_begin(3, mk);
-    if (ef || vf || va) hipLaunchKernelGGL((k_tersoff_zeta<R, true>), dim3(nb1), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_tersoff_zeta<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
+    const int nbz = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
+    if (ef || vf || va) hipLaunchKernelGGL((k_tersoff_zeta<R, true>), dim3(nbz), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_tersoff_zeta<R, false>), dim3(nbz), dim3(BS), 0, stream, P);
     mark_end(3, mk);
     mark_begin(4, mk);
     const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom

@@ -165,6 +165,15 @@ __device__ __forceinline__ void ghost(const double4 &xj, uint32_t e, const doubl
   gz = __dadd_rn(xj.z, (double)e_s(e, 2) * prd[2]);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8).  Atoms are
+// cell-sorted, so handing each XCD one contiguous range of workgroups keeps the neighbour data
+// a workgroup gathers in the L2 of the XCD that already holds it.  Bijective on [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -241,7 +250,7 @@ __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start)
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *cell_start) {
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int i = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
 template <typename R, bool WANT_VET>
 __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   using M = MathOps<R>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double4 xi = P.x[i];
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   const uint32_t *row = P.slist + i;  // transposed: entry k at row[k * n]
   const size_t sstride = (size_t)P.n;
   int ns = 0, nn = 0, nrt = 0, nrn = 0;
-  uint32_t nb[3] = {0, 0, 0};
+  uint32_t nb0 = 0, nb1 = 0, nb2 = 0;
   double vetd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   bool too_many = false;
   // reverse candidates: every atom j that may list i as a Tersoff neighbour or as a normal
@@ -364,42 +373,54 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   const double widen = 1.0 + 1e-12;
   uint32_t *rt_row = P.rtc + (size_t)i * MAXSH;
   uint32_t *rn_row = P.rnc + (size_t)i * MAXSH;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t e = row[k * sstride];
-    const int j = e_idx(e);
-    const double4 xj = P.x[j];
-    double gx, gy, gz;
-    ghost(xj, e, s.prd, gx, gy, gz);
-    const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
-    const double rsq = delx * delx + dely * dely + delz * delz;
-    const int tj = (int)xj.w - 1;
-    const bool tmapped = P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0;
-    if (rsq < P.tersoff_cutmaxsq && tmapped) {
-      if (ns < MAXSH) P.tsh[(size_t)i * MAXSH + ns] = e;
-      ns++;
-    }
-    if (tmapped && rsq < P.tersoff_cutmaxsq * widen) {
-      if (nrt < MAXSH) rt_row[nrt] = (uint32_t)j;
-      nrt++;
-    }
-    const IlpPair<R> &pp = P.ip[ti * MAXT + tj];
-    if (pp.mapped && P.mol[j] == mi && rsq != 0.0) {
-      const double rc2 = fmax(pp.rcutsq, P.ip[tj * MAXT + ti].rcutsq) * widen;
-      if (rsq < rc2) {
-        if (nrn < MAXSH) rn_row[nrn] = (uint32_t)j;
-        nrn++;
+  // entries are fetched in batches of NB so that NB independent gathers are in flight; every
+  // index below is a compile-time constant after unrolling (no scratch memory)
+  constexpr int NB = 8;
+  for (int k0 = 0; k0 < cnt; k0 += NB) {
+    uint32_t eb_[NB];
+    double4 xb_[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) eb_[b] = (k0 + b < cnt) ? row[(k0 + b) * sstride] : 0u;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      if (k0 + b < cnt) xb_[b] = P.x[e_idx(eb_[b])];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      if (k0 + b >= cnt) break;
+      const uint32_t e = eb_[b];
+      const int j = e_idx(e);
+      const double4 xj = xb_[b];
+      double gx, gy, gz;
+      ghost(xj, e, s.prd, gx, gy, gz);
+      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      const int tj = (int)xj.w - 1;
+      const bool tmapped = P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0;
+      if (rsq < P.tersoff_cutmaxsq && tmapped) {
+        if (ns < MAXSH) P.tsh[(size_t)i * MAXSH + ns] = e;
+        ns++;
       }
-    }
-    if (pp.mapped && P.mol[j] == mi && rsq != 0.0 && rsq < pp.rcutsq) {
-      if (nn < 3) {
-        nb[nn] = e;
-        vetd[nn][0] = gx - xi.x;
-        vetd[nn][1] = gy - xi.y;
-        vetd[nn][2] = gz - xi.z;
-      } else {
-        too_many = true;
+      if (tmapped && rsq < P.tersoff_cutmaxsq * widen) {
+        if (nrt < MAXSH) rt_row[nrt] = (uint32_t)j;
+        nrt++;
       }
-      nn++;
+      const IlpPair<R> &pp = P.ip[ti * MAXT + tj];
+      if (pp.mapped && P.mol[j] == mi && rsq != 0.0) {
+        const double rc2 = fmax(pp.rcutsq, P.ip[tj * MAXT + ti].rcutsq) * widen;
+        if (rsq < rc2) {
+          if (nrn < MAXSH) rn_row[nrn] = (uint32_t)j;
+          nrn++;
+        }
+        if (rsq < pp.rcutsq) {
+          const double v0 = gx - xi.x, v1 = gy - xi.y, v2 = gz - xi.z;
+          // fixed-register slots (selects, not an indexed array)
+          if (nn == 0) { nb0 = e; vetd[0][0] = v0; vetd[0][1] = v1; vetd[0][2] = v2; }
+          else if (nn == 1) { nb1 = e; vetd[1][0] = v0; vetd[1][1] = v1; vetd[1][2] = v2; }
+          else if (nn == 2) { nb2 = e; vetd[2][0] = v0; vetd[2][1] = v1; vetd[2][2] = v2; }
+          else too_many = true;
+          nn++;
+        }
+      }
     }
   }
   if (ns > MAXSH || nrt > MAXSH || nrn > MAXSH) {
@@ -417,9 +438,9 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   }
   uint32_t *nrow = P.nnb + (size_t)i * 4;
   nrow[0] = (uint32_t)nn;
-  nrow[1] = nb[0];
-  nrow[2] = nb[1];
-  nrow[3] = nb[2];
+  nrow[1] = nb0;
+  nrow[2] = nb1;
+  nrow[3] = nb2;
   if (WANT_VET) {
     double *vo = P.vet + (size_t)i * 9;
     for (int k = 0; k < 3; k++)
@@ -560,7 +581,7 @@ template <typename R, bool EF, bool VF, bool VA>
 __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
-  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
@@ -849,49 +870,60 @@ __device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R r
   }
 }
 
+// displacement x_j(image) - x_i of a list entry, its square and j's type (the Tersoff kernels
+// recompute these where needed instead of keeping per-neighbour arrays, which would live in
+// scratch memory because their indices are not compile-time constants)
+__device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const double *prd, const double4 &xi, double d[3],
+                                       double &rsq, int &t) {
+  const double4 xj = x[e_idx(e)];
+  double gx, gy, gz;
+  ghost(xj, e, prd, gx, gy, gz);
+  d[0] = gx - xi.x;
+  d[1] = gy - xi.y;
+  d[2] = gz - xi.z;
+  rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  t = (int)xj.w - 1;
+}
+
 // Per step 3: zeta, b_ij and force_zeta for every Tersoff bond of i (pair_tersoff.cpp:177-225)
 template <typename R, bool EF>
 __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
   using RT = double;
   using M = MathOps<RT>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // four lanes per atom; lane q handles bonds q, q+4, ...
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int i = gt >> 2;
+  const int quad = gt & 3;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
+  const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
   const double4 xi = P.x[i];
   const int ti = (int)xi.w - 1;
   const int ns = P.tcnt[i];
   const uint32_t *row = P.tsh + (size_t)i * MAXSH;
-  RT del[MAXSH][3];
-  RT rsqs[MAXSH];
-  int tt[MAXSH];
-  for (int a = 0; a < ns; a++) {
-    const uint32_t e = row[a];
-    const double4 xj = P.x[e_idx(e)];
-    double gx, gy, gz;
-    ghost(xj, e, s.prd, gx, gy, gz);
-    const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
-    del[a][0] = d0; del[a][1] = d1; del[a][2] = d2;
-    rsqs[a] = d0 * d0 + d1 * d1 + d2 * d2;
-    tt[a] = (int)xj.w - 1;
-  }
-  for (int jj = 0; jj < ns; jj++) {
+  for (int jj = quad; jj < ns; jj += 4) {
     const size_t slot = (size_t)i * MAXSH + jj;
-    const int pij = P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[jj]];
-    const TersP<RT> &p = P.tp[pij];
-    if ((double)rsqs[jj] >= p.cutsq) {
+    double dj[3], rsqj;
+    int tj;
+    nb_del(P.x, row[jj], prd, xi, dj, rsqj, tj);
+    const TersP<RT> &p = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tj]];
+    if (rsqj >= p.cutsq) {
       P.fz[slot] = 0;
       P.pref[slot] = 0;
       if (EF) P.eb[slot] = 0;
       continue;
     }
     RT zeta = 0;
-    const RT rij = M::sq(rsqs[jj]);
+    const RT rij = M::sq(rsqj);
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      const TersP<RT> &q = P.tp[P.t3[(ti * MAXT + tt[jj]) * MAXT + tt[kk]]];
-      if ((double)rsqs[kk] >= q.cutsq) continue;
-      const RT rik = M::sq(rsqs[kk]);
-      const RT costheta = (del[jj][0] * del[kk][0] + del[jj][1] * del[kk][1] + del[jj][2] * del[kk][2]) / (rij * rik);
+      double dk[3], rsqk;
+      int tk;
+      nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
+      const TersP<RT> &q = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tk]];
+      if (rsqk >= q.cutsq) continue;
+      const RT rik = M::sq(rsqk);
+      const RT costheta = (dj[0] * dk[0] + dj[1] * dk[1] + dj[2] * dk[2]) / (rij * rik);
       RT dd;
       const RT ex = t_exdelr(rij, rik, q, dd);
       zeta += t_fc(rik, q) * t_gijk(costheta, q) * ex;
@@ -933,7 +965,7 @@ __device__ __forceinline__ void tally3(double *va, const R *delr1, const R *fj, 
 // summed with a fixed-order shuffle butterfly.
 template <typename R, bool EF, bool VF, bool VA>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
   const int a = gt >> 2;
   const int quad = gt & 3;
   if (a >= P.n) return;  // the whole quad leaves together
@@ -997,66 +1029,61 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   const int ns = P.tcnt[a];
   const uint32_t *trow = P.tsh + (size_t)a * MAXSH;
   const int atag = P.tag[a];
-  double del[MAXSH][3];
-  double rsqs[MAXSH];
-  int tt[MAXSH];
   for (int jj = 0; jj < ns; jj++) {
     const uint32_t e = trow[jj];
-    const int j = e_idx(e);
-    const double4 xj = P.x[j];
+    double dj[3], rsqj;
+    int tj;
+    nb_del(P.x, e, prd, xa, dj, rsqj, tj);
+    // repulsion: parameter of whichever atom computes the pair in the reference (tag parity)
+    const double4 xj = P.x[e_idx(e)];
     double gx, gy, gz;
     ghost(xj, e, prd, gx, gy, gz);
-    const double d0 = gx - xa.x, d1 = gy - xa.y, d2 = gz - xa.z;
-    del[jj][0] = d0; del[jj][1] = d1; del[jj][2] = d2;
-    rsqs[jj] = d0 * d0 + d1 * d1 + d2 * d2;
-    tt[jj] = (int)xj.w - 1;
-    // repulsion: parameter of whichever atom computes the pair in the reference (tag parity)
-    const bool skip = tag_skip(atag, P.tag[j], xa.x, xa.y, xa.z, gx, gy, gz);
-    const int pr = skip ? P.t3[(tt[jj] * MAXT + ta) * MAXT + ta] : P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[jj]];
+    const bool skip = tag_skip(atag, P.tag[e_idx(e)], xa.x, xa.y, xa.z, gx, gy, gz);
+    const int pr = skip ? P.t3[(tj * MAXT + ta) * MAXT + ta] : P.t3[(ta * MAXT + tj) * MAXT + tj];
     const TersP<double> &p = P.tp[pr];
-    const double rsqd = d0 * d0 + d1 * d1 + d2 * d2;
-    if (rsqd < p.cutsq) {
-      const double r = sqrt(rsqs[jj]);
+    if (rsqj < p.cutsq) {
+      const double r = sqrt(rsqj);
       const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
       const double tmp_exp = exp(-p.lam1 * r);
       const double fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
-      // del here is x_j - x_a; the reference's del is x_a - x_j
-      F.x -= (double)(del[jj][0] * fpair);
-      F.y -= (double)(del[jj][1] * fpair);
-      F.z -= (double)(del[jj][2] * fpair);
-      if (EF) et += 0.5 * (double)(tmp_fc * p.biga * tmp_exp);
+      // dj is x_j - x_a; the reference's del is x_a - x_j
+      F.x -= dj[0] * fpair;
+      F.y -= dj[1] * fpair;
+      F.z -= dj[2] * fpair;
+      if (EF) et += 0.5 * (tmp_fc * p.biga * tmp_exp);
       if (VF || VA) {
-        const double hf = 0.5 * (double)fpair;
-        va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
-        va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+        const double hf = 0.5 * fpair;
+        va[0] += hf * dj[0] * dj[0]; va[1] += hf * dj[1] * dj[1]; va[2] += hf * dj[2] * dj[2];
+        va[3] += hf * dj[0] * dj[1]; va[4] += hf * dj[0] * dj[2]; va[5] += hf * dj[1] * dj[2];
       }
     }
-  }
-  for (int jj = 0; jj < ns; jj++) {
+    // bond (a, j): force_zeta and the three-body terms with a as i
     const size_t slot = (size_t)a * MAXSH + jj;
     const double fzv = P.fz[slot];
     const double pref = P.pref[slot];
     if (fzv == 0.0 && pref == 0.0) continue;
-    F.x += (double)(del[jj][0] * fzv);
-    F.y += (double)(del[jj][1] * fzv);
-    F.z += (double)(del[jj][2] * fzv);
+    F.x += dj[0] * fzv;
+    F.y += dj[1] * fzv;
+    F.z += dj[2] * fzv;
     if (EF) et += 0.5 * P.eb[slot];
     if (VF || VA) {
-      const double hf = -0.5 * (double)fzv;
-      const double d0 = del[jj][0], d1 = del[jj][1], d2 = del[jj][2];
-      va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
-      va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+      const double hf = -0.5 * fzv;
+      va[0] += hf * dj[0] * dj[0]; va[1] += hf * dj[1] * dj[1]; va[2] += hf * dj[2] * dj[2];
+      va[3] += hf * dj[0] * dj[1]; va[4] += hf * dj[0] * dj[2]; va[5] += hf * dj[1] * dj[2];
     }
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      const TersP<double> &q = P.tp[P.t3[(ta * MAXT + tt[jj]) * MAXT + tt[kk]]];
-      if ((double)rsqs[kk] >= q.cutsq) continue;
+      double dk[3], rsqk;
+      int tk;
+      nb_del(P.x, trow[kk], prd, xa, dk, rsqk, tk);
+      const TersP<double> &q = P.tp[P.t3[(ta * MAXT + tj) * MAXT + tk]];
+      if (rsqk >= q.cutsq) continue;
       double fi[3], fj[3], fk[3];
-      t_attractive(q, pref, rsqs[jj], rsqs[kk], del[jj], del[kk], fi, fj, fk);
-      F.x += (double)fi[0];
-      F.y += (double)fi[1];
-      F.z += (double)fi[2];
-      if (VF || VA) tally3(va, del[jj], fj, del[kk], fk);
+      t_attractive(q, pref, rsqj, rsqk, dj, dk, fi, fj, fk);
+      F.x += fi[0];
+      F.y += fi[1];
+      F.z += fi[2];
+      if (VF || VA) tally3(va, dj, fj, dk, fk);
     }
   }
   } else {
@@ -1075,45 +1102,38 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     if (slot_a < 0) continue;
     const double4 xi = P.x[i];
     const int ti = (int)xi.w - 1;
-    double dl[MAXSH][3];
-    double rq[MAXSH];
-    int ty[MAXSH];
-    for (int b = 0; b < ni; b++) {
-      const uint32_t e = irow[b];
-      const double4 xb = P.x[e_idx(e)];
-      double gx, gy, gz;
-      ghost(xb, e, prd, gx, gy, gz);
-      const double d0 = gx - xi.x, d1 = gy - xi.y, d2 = gz - xi.z;
-      dl[b][0] = d0; dl[b][1] = d1; dl[b][2] = d2;
-      rq[b] = d0 * d0 + d1 * d1 + d2 * d2;
-      ty[b] = (int)xb.w - 1;
-    }
+    // displacement of a seen from i, with i's own image shift (as thread i computes it)
+    double da[3], rsqa;
+    int ta_;
+    nb_del(P.x, irow[slot_a], prd, xi, da, rsqa, ta_);
     // a as j of bond (i, a)
     {
       const size_t slot = (size_t)i * MAXSH + slot_a;
       const double fzv = P.fz[slot];
       const double pref = P.pref[slot];
       if (!(fzv == 0.0 && pref == 0.0)) {
-        F.x -= (double)(dl[slot_a][0] * fzv);
-        F.y -= (double)(dl[slot_a][1] * fzv);
-        F.z -= (double)(dl[slot_a][2] * fzv);
+        F.x -= da[0] * fzv;
+        F.y -= da[1] * fzv;
+        F.z -= da[2] * fzv;
         if (EF) et += 0.5 * P.eb[slot];
         if (VF || VA) {
-          const double hf = -0.5 * (double)fzv;
-          const double d0 = dl[slot_a][0], d1 = dl[slot_a][1], d2 = dl[slot_a][2];
-          va[0] += hf * d0 * d0; va[1] += hf * d1 * d1; va[2] += hf * d2 * d2;
-          va[3] += hf * d0 * d1; va[4] += hf * d0 * d2; va[5] += hf * d1 * d2;
+          const double hf = -0.5 * fzv;
+          va[0] += hf * da[0] * da[0]; va[1] += hf * da[1] * da[1]; va[2] += hf * da[2] * da[2];
+          va[3] += hf * da[0] * da[1]; va[4] += hf * da[0] * da[2]; va[5] += hf * da[1] * da[2];
         }
         for (int kk = 0; kk < ni; kk++) {
           if (kk == slot_a) continue;
-          const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + ty[kk]]];
-          if ((double)rq[kk] >= qp.cutsq) continue;
+          double dk[3], rsqk;
+          int tk;
+          nb_del(P.x, irow[kk], prd, xi, dk, rsqk, tk);
+          const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + tk]];
+          if (rsqk >= qp.cutsq) continue;
           double fi[3], fj[3], fk[3];
-          t_attractive(qp, pref, rq[slot_a], rq[kk], dl[slot_a], dl[kk], fi, fj, fk);
-          F.x += (double)fj[0];
-          F.y += (double)fj[1];
-          F.z += (double)fj[2];
-          if (VF || VA) tally3(va, dl[slot_a], fj, dl[kk], fk);
+          t_attractive(qp, pref, rsqa, rsqk, da, dk, fi, fj, fk);
+          F.x += fj[0];
+          F.y += fj[1];
+          F.z += fj[2];
+          if (VF || VA) tally3(va, da, fj, dk, fk);
         }
       }
     }
@@ -1124,14 +1144,17 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       const double fzv = P.fz[slot];
       const double pref = P.pref[slot];
       if (fzv == 0.0 && pref == 0.0) continue;
-      const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ty[jj]) * MAXT + ta]];
-      if ((double)rq[slot_a] >= qp.cutsq) continue;
+      double dj[3], rsqj;
+      int tj;
+      nb_del(P.x, irow[jj], prd, xi, dj, rsqj, tj);
+      const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + tj) * MAXT + ta]];
+      if (rsqa >= qp.cutsq) continue;
       double fi[3], fj[3], fk[3];
-      t_attractive(qp, pref, rq[jj], rq[slot_a], dl[jj], dl[slot_a], fi, fj, fk);
-      F.x += (double)fk[0];
-      F.y += (double)fk[1];
-      F.z += (double)fk[2];
-      if (VF || VA) tally3(va, dl[jj], fj, dl[slot_a], fk);
+      t_attractive(qp, pref, rsqj, rsqa, dj, da, fi, fj, fk);
+      F.x += fk[0];
+      F.y += fk[1];
+      F.z += fk[2];
+      if (VF || VA) tally3(va, dj, fj, da, fk);
     }
   }
   }
