@@ -1,12 +1,18 @@
 """Throughput of the LMFF force path (Tersoff + ILP with normals and Tap) on one MI355X.
 
-Workload: graphene/hBN heterobilayer, 220x130 rectangular cells per layer (228,800 atoms),
-seeded 0.02 A jitter, 300 K Maxwell-Boltzmann velocities, NVE dt = 1 fs, neighbor rebuild
-every 10 steps with a 2 A skin.  A "step" is one Velocity-Verlet step of the whole system
-(force evaluation + integration + the amortised neighbor rebuild), with all state resident
-in HBM.  The FP64 path gives `value`; the mixed-precision path (FP32 pair/bond-order math,
-FP64 displacements and accumulation) is timed on the same input and reported beside it,
-with its force error against FP64.
+Default workload: graphene/hBN heterobilayer, 220x130 rectangular cells per layer (228,800
+atoms), seeded 0.02 A jitter, 300 K Maxwell-Boltzmann velocities, NVE dt = 1 fs, neighbor
+rebuild every 10 steps with a 2 A skin.  A "step" is one Velocity-Verlet step of the whole
+system (force evaluation + integration + the amortised neighbor rebuild), with all state
+resident in HBM.  The FP64 path gives `value`; the mixed-precision path (FP32 ILP pair math,
+FP64 displacements, Tersoff and accumulation) is timed on the same input and reported beside
+it, with its force error against FP64.
+
+Other workloads (not the default line):
+  --workload multilayer   10-layer alternating G/hBN stack, 260x260 cells per layer
+                          (2,704,000 atoms): FP64 vs mixed step time and NVE drift
+  --workload serving      stream of twisted-bilayer Langevin jobs (2k-80k atoms, log-uniform),
+                          packed server vs the same jobs run one at a time: throughput, p50/p99
 
 Prints one JSON line.  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -39,11 +45,21 @@ def build_ctx(s, precision):
     return ctx, sid
 
 
-def one_shot(s, precision):
+def one_shot(s, precision, repeat=1):
+    """Pair::compute through the host-buffer entry point (x in, f/energy/virial out); with
+    repeat > 1 also the median wall time of the later calls (PCIe transfers included)."""
     ctx = capi.Context(0)
     try:
         capi.setup_for(ctx, s, precision=precision)
-        return ctx.pair_compute(s, 1, 2)
+        r = ctx.pair_compute(s, 1, 2)
+        walls = []
+        for _ in range(repeat - 1):
+            t0 = time.perf_counter()
+            ctx.pair_compute(s, 1, 2)
+            walls.append(time.perf_counter() - t0)
+        if walls:
+            r["wall_s"] = float(np.median(walls))
+        return r
     finally:
         ctx.close()
 
@@ -97,6 +113,21 @@ def cpu_baseline(s, seconds_budget=20.0):
             "cpu_model": _cpu_model()}, f_ref
 
 
+def profiled_traffic(kernel, nx, ny):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a plain copy), or None when no summary covers this workload."""
+    path = os.path.join(REPO, "profiles", f"hbm_traffic_bilayer_{nx * ny * 8}.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == f"`{kernel}`":
+            return (float(cells[4]) + float(cells[5])) * 1e6
+    return None
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -107,14 +138,38 @@ def _cpu_model():
     return "unknown"
 
 
+def md_compare(s, steps, warmup):
+    """FP64 and mixed contexts on the same input: single-shot forces, then timed resident runs."""
+    r64 = one_shot(s, capi.LMFF_FP64, repeat=4)
+    rmx = one_shot(s, capi.LMFF_MIXED)
+    ctx, sid = build_ctx(s, capi.LMFF_FP64)
+    stats = ctx.pair_stats()
+    res64 = timed_run(ctx, sid, s, steps, warmup, profile=True)
+    ctx.close()
+    ctxm, sidm = build_ctx(s, capi.LMFF_MIXED)
+    resmx = timed_run(ctxm, sidm, s, steps, warmup, profile=True)
+    ctxm.close()
+    df = rmx["f"] - r64["f"]
+    mixed_err = {"max_abs": float(np.abs(df).max()), "rms": float(np.sqrt((df ** 2).sum(1).mean()))}
+    return r64, rmx, stats, res64, resmx, mixed_err
+
+
+FAMILIES = ["list_build", "normals", "ilp_pair", "tersoff_bonds", "gather", "integrate", "reduce"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--nx", type=int, default=220)
-    ap.add_argument("--ny", type=int, default=130)
+    ap.add_argument("--workload", choices=["bilayer", "multilayer", "serving"], default="bilayer")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--layers", type=int, default=10, help="multilayer: number of layers")
+    ap.add_argument("--jobs", type=int, default=128, help="serving: number of jobs")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: Poisson arrival rate in jobs/s for the latency run (0: derived from capacity)")
     ap.add_argument("--copy-ref", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -132,27 +187,18 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
               f"note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serving":
+        print(json.dumps(serving_bench(args)))
+        return
+    if args.workload == "multilayer":
+        print(json.dumps(multilayer_bench(args)))
+        return
 
-    s = S.layered_stack(args.nx, args.ny, ["G", "BN"], seed=12346)
+    nx = args.nx or 220
+    ny = args.ny or 130
+    s = S.layered_stack(nx, ny, ["G", "BN"], seed=12346)
     n = s.n
-
-    # ---- single force evaluations of the initial state (for the error figures)
-    r64 = one_shot(s, capi.LMFF_FP64)
-    rmx = one_shot(s, capi.LMFF_MIXED)
-
-    # ---- FP64: the headline number
-    ctx, sid = build_ctx(s, capi.LMFF_FP64)
-    stats = ctx.pair_stats()
-    res64 = timed_run(ctx, sid, s, args.steps, args.warmup, profile=True)
-    ctx.close()
-
-    # ---- mixed precision on the same input
-    ctxm, sidm = build_ctx(s, capi.LMFF_MIXED)
-    resmx = timed_run(ctxm, sidm, s, args.steps, args.warmup, profile=True)
-    ctxm.close()
-
-    df = rmx["f"] - r64["f"]
-    mixed_err = {"max_abs": float(np.abs(df).max()), "rms": float(np.sqrt((df ** 2).sum(1).mean()))}
+    r64, rmx, stats, res64, resmx, mixed_err = md_compare(s, args.steps, args.warmup)
 
     ms64 = res64["gpu_ms"] / args.steps
     msmx = resmx["gpu_ms"] / args.steps
@@ -190,25 +236,124 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator: jittered lattice, 300 K Maxwell-Boltzmann velocities)",
-        "config": {"workload": f"graphene/hBN heterobilayer {args.nx}x{args.ny} cells/layer, {n} atoms, "
+        "config": {"workload": f"graphene/hBN heterobilayer {nx}x{ny} cells/layer, {n} atoms, "
                                f"NVE dt=1 fs, rebuild every 10 steps, skin 2 A, ILP 16 A with Tap",
                    "natoms": n, "precision": "fp64"},
         "mixed": {"value": n / (msmx * 1e-3), "ms_per_step": msmx, "speedup_vs_fp64": ms64 / msmx,
                   "force_err_vs_fp64": mixed_err, "ilp_kernel_ms": ilp_ms_mx},
         "drift_eV_per_atom": {"fp64": (res64["e1"] - res64["e0"]) / n, "mixed": (resmx["e1"] - resmx["e0"]) / n,
                               "steps": args.steps},
-        "kernel_ms_per_launch": dict(zip(["list_build", "normals", "ilp_pair", "tersoff_zeta", "gather_tersoff",
-                                          "integrate", "reduce"], res64["fam_ms"])),
+        "kernel_ms_per_launch": dict(zip(FAMILIES, res64["fam_ms"])),
         "roofline": {"bound": "valu", "kernel": "k_ilp<double>", "achieved": achieved, "peak": FP64_VALU_PEAK,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK,
+                     "traffic": profiled_traffic("k_ilp<double, false, false, false>", nx, ny),
                      "algorithmic_bytes_per_launch": traffic_alg,
                      "flop_per_launch": flop, "ilp_pairs": stats["ilp_pairs"],
                      "ilp_list_entries": stats["ilp_entries"]},
         "cpu_baseline": cpu,
         "oracle_force_error": oracle_err,
+        "host_buffer_pair_compute": {
+            "note": "one Pair::compute from host arrays (upload, list build, forces, download), FP64",
+            "wall_ms": r64["wall_s"] * 1e3, "atom_evals_per_s": n / r64["wall_s"]},
         "wall_s": {"fp64": res64["wall_s"], "mixed": resmx["wall_s"]},
     }
     print(json.dumps(out))
+
+
+def multilayer_bench(args):
+    """Alternating G/hBN stack (default 10 layers x 260x260 cells = 2,704,000 atoms): FP64 vs mixed
+    resident NVE runs of the same length, with the mixed force error and both energy drifts."""
+    nx = args.nx or 260
+    ny = args.ny or 260
+    kinds = ["G" if k % 2 == 0 else "BN" for k in range(args.layers)]
+    s = S.layered_stack(nx, ny, kinds, seed=12347)
+    n = s.n
+    r64, rmx, stats, res64, resmx, mixed_err = md_compare(s, args.steps, args.warmup)
+    ms64 = res64["gpu_ms"] / args.steps
+    msmx = resmx["gpu_ms"] / args.steps
+    d64 = (res64["e1"] - res64["e0"]) / n
+    dmx = (resmx["e1"] - resmx["e0"]) / n
+    return {
+        "metric": "atom-steps/s FP64 vs mixed (multilayer)", "value": n / (ms64 * 1e-3), "unit": "atom-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{args.layers}-layer alternating G/hBN stack {nx}x{ny} cells/layer, {n} atoms, "
+                               f"NVE dt=1 fs, rebuild every 10 steps, skin 2 A, ILP 16 A with Tap", "natoms": n},
+        "mixed": {"value": n / (msmx * 1e-3), "ms_per_step": msmx, "speedup_vs_fp64": ms64 / msmx,
+                  "force_err_vs_fp64": mixed_err},
+        "drift_eV_per_atom": {"fp64": d64, "mixed": dmx, "steps": args.steps,
+                              "mixed_over_fp64": (abs(dmx) / abs(d64)) if d64 != 0 else None},
+        "kernel_ms_per_launch": {"fp64": dict(zip(FAMILIES, res64["fam_ms"])),
+                                 "mixed": dict(zip(FAMILIES, resmx["fam_ms"]))},
+        "ilp_pairs": stats["ilp_pairs"], "ilp_list_entries": stats["ilp_entries"],
+        "fp64_ilp_tflops": stats["ilp_pairs"] * ILP_PAIR_FLOP / (res64["fam_ms"][2] * 1e-3) / 1e12,
+        "wall_s": {"fp64": res64["wall_s"], "mixed": resmx["wall_s"]},
+    }
+
+
+def serving_bench(args):
+    """Twisted-bilayer Langevin jobs (mixed precision, 300 K, tau 1 ps).  Capacity: every job
+    queued at t=0, served packed (one launch per kernel per step for all resident jobs) and
+    one at a time through the same server code (max one resident job).  Latency: Poisson
+    arrivals at a stated rate, served packed."""
+    from lmff import serving
+
+    steps = args.steps
+    jobs = serving.tbg_job_stream(args.jobs, rate_hz=1e9, steps=steps, seed=2024)
+    sizes = np.array([j.natoms for j in jobs])
+    small = serving.tbg_job_stream(args.jobs, rate_hz=1e9, steps=steps, seed=2025, sizes="minimal")
+    small_sizes = np.array([j.natoms for j in small])
+
+    def fresh(js, rate=None):
+        out = []
+        t = 0.0
+        rng = np.random.default_rng(77)
+        for j in js:
+            if rate:
+                t += rng.exponential(1.0 / rate)
+            out.append(serving.Job(job_id=j.job_id, system=j.system, steps=j.steps, arrival=t, seed=j.seed))
+        return out
+
+    def capacity(js):
+        srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=10)
+        srv.run(fresh(js[:2]))   # warm the context (module load, first graph capture)
+        packed = srv.run(fresh(js))
+        srv.close()
+        seq_srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=10, max_resident=1)
+        seq_srv.run(fresh(js[:1]))
+        seq = seq_srv.run(fresh(js))
+        seq_srv.close()
+        return packed, seq
+
+    packed, seq = capacity(jobs)
+    packed_s, seq_s = capacity(small)
+    # latency at an offered load of about half the packed capacity
+    rate = args.rate or 0.5 * packed["jobs"] / packed["makespan_s"]
+    srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=10)
+    lat = srv.run(fresh(jobs, rate=rate))
+    srv.close()
+    return {
+        "metric": "job stream atom-steps/s packed vs sequential; job p50/p99 latency",
+        "value": packed["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1, "steps": steps,
+        "higher_is_better": True, "dtype": "mixed (FP32 ILP pair math, FP64 elsewhere)",
+        "data": "synthetic twisted-bilayer stream (seeded)",
+        "config": {"workload": f"{len(jobs)} twisted-bilayer graphene jobs, (m,n) with 1<=n<m<=12, "
+                               f"{sizes.min()}-{sizes.max()} atoms (median {int(np.median(sizes))}), "
+                               f"{steps} Langevin steps each, 300 K, damp 1 ps, dt 1 fs, tick 10"},
+        "capacity": {"packed_atom_steps_per_s": packed["atom_steps_per_s"],
+                     "sequential_atom_steps_per_s": seq["atom_steps_per_s"],
+                     "speedup": packed["atom_steps_per_s"] / seq["atom_steps_per_s"],
+                     "packed_makespan_s": packed["makespan_s"], "sequential_makespan_s": seq["makespan_s"],
+                     "packed_max_resident": packed["max_resident"]},
+        "capacity_minimal_cells": {
+            "workload": f"{len(small)} jobs, each twist's smallest cell with >= 2000 atoms and sides >= 36.5 A "
+                        f"({small_sizes.min()}-{small_sizes.max()} atoms, median {int(np.median(small_sizes))})",
+            "packed_atom_steps_per_s": packed_s["atom_steps_per_s"],
+            "sequential_atom_steps_per_s": seq_s["atom_steps_per_s"],
+            "speedup": packed_s["atom_steps_per_s"] / seq_s["atom_steps_per_s"]},
+        "latency": {"arrival_rate_jobs_per_s": rate, "p50_s": lat["latency_p50_s"], "p99_s": lat["latency_p99_s"],
+                    "atom_steps_per_s": lat["atom_steps_per_s"], "max_resident": lat["max_resident"]},
+    }
 
 
 if __name__ == "__main__":

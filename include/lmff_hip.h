@@ -130,7 +130,7 @@ int lmff_run(lmff_ctx *ctx, int64_t nsteps, int64_t thermo_every, int use_graph,
              int64_t out_capacity);
 
 /* Device-side timing of the last lmff_run: total ms and per-kernel-family averages
- * (index: 0 list build, 1 normals, 2 ILP pair, 3 Tersoff zeta, 4 gather+Tersoff force,
+ * (index: 0 list build, 1 normals, 2 ILP pair, 3 Tersoff bonds, 4 force gather,
  *  5 integrate, 6 reductions).  Requires lmff_profile(ctx, 1) before the run. */
 int lmff_profile(lmff_ctx *ctx, int enable);
 int lmff_profile_read(lmff_ctx *ctx, double *ms_total, double *ms_per_family, int64_t *launches_per_family);

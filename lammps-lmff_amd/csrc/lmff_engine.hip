@@ -135,8 +135,8 @@ struct Ctx {
   std::vector<Staged> staged;
   // device state
   DBuf x, v, f, xhold, mol, tag, sysid, orig, key, key2, val, perm, tmp4, tmpi;
-  DBuf slist, scnt, ilist, icnt;
-  DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, fz, pref, eb, fil, S, T, th, vatom;
+  DBuf slist, scnt, ilist, icnt, ocnt;
+  DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave;
   // atom ranges whose last-step force is restored after the next set-up force evaluation
@@ -169,8 +169,8 @@ struct Ctx {
 
   Ctx() {
     DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
-                   &slist, &scnt, &ilist, &icnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
-                   &rnc, &rncnt, &fz, &pref, &eb,
+                   &slist, &scnt, &ilist, &icnt, &ocnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
+                   &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave};
     for (DBuf *b : all) b->tally = &bytes_in_use;
@@ -412,14 +412,15 @@ struct Ctx {
     orig.ensure(m * sizeof(int)); tmpi.ensure(m * sizeof(int));
     key.ensure(m * sizeof(uint32_t)); key2.ensure(m * sizeof(uint32_t));
     val.ensure(m * sizeof(int)); perm.ensure(m * sizeof(int));
-    scnt.ensure(m * sizeof(int)); icnt.ensure(m * sizeof(int));
+    scnt.ensure(m * sizeof(int)); icnt.ensure(m * sizeof(int)); ocnt.ensure(m * sizeof(int));
     slist.ensure(m * scap * sizeof(uint32_t));
     nrm.ensure(m * sizeof(double4));
     dn.ensure(m * 27 * sizeof(double)); dndri.ensure(m * 9 * sizeof(double)); vet.ensure(m * 9 * sizeof(double));
     nnb.ensure(m * 4 * sizeof(uint32_t)); tsh.ensure(m * MAXSH * sizeof(uint32_t)); tcnt.ensure(m * sizeof(int));
     rtc.ensure(m * MAXSH * sizeof(uint32_t)); rtcnt.ensure(m * sizeof(int));
     rnc.ensure(m * MAXSH * sizeof(uint32_t)); rncnt.ensure(m * sizeof(int));
-    fz.ensure(m * MAXSH * sizeof(double)); pref.ensure(m * MAXSH * sizeof(double)); eb.ensure(m * MAXSH * sizeof(double));
+    eb.ensure(m * MAXSH * sizeof(double)); fb.ensure(m * MAXSH * MAXSH * 3 * sizeof(double));
+    ft.ensure(m * sizeof(double4));
     fil.ensure(m * sizeof(double4)); S.ensure(m * sizeof(double4)); T.ensure(m * 6 * sizeof(double));
     th.ensure(m * NTH * sizeof(double)); vatom.ensure(m * 6 * sizeof(double));
   }
@@ -468,12 +469,12 @@ struct Ctx {
     P.mol = mol.as<int>(); P.tag = tag.as<int>(); P.sys = sysid.as<int>();
     P.sys_tab = systab.as<SysDev>();
     P.slist = slist.as<uint32_t>(); P.scnt = scnt.as<int>(); P.scap = scap;
-    P.ilist = ilist.as<uint32_t>(); P.icnt = icnt.as<int>(); P.icap = icap;
+    P.ilist = ilist.as<uint32_t>(); P.icnt = icnt.as<int>(); P.ocnt = ocnt.as<int>(); P.icap = icap;
     P.nrm = (typename std::conditional<sizeof(R) == 8, double4, float4>::type *)nrm.p;
     P.dn = dn.as<R>(); P.dndri = dndri.as<R>(); P.vet = vet.as<double>();
     P.nnb = nnb.as<uint32_t>(); P.tsh = tsh.as<uint32_t>(); P.tcnt = tcnt.as<int>();
     P.rtc = rtc.as<uint32_t>(); P.rtcnt = rtcnt.as<int>(); P.rnc = rnc.as<uint32_t>(); P.rncnt = rncnt.as<int>();
-    P.fz = fz.as<double>(); P.pref = pref.as<double>(); P.eb = eb.as<double>();
+    P.eb = eb.as<double>(); P.fb = fb.as<double>(); P.wb = wb.as<double>(); P.ft = ft.as<double4>();
     P.fil = fil.as<double4>(); P.S = S.as<double4>(); P.T = T.as<double>();
     P.th = th.as<double>(); P.vatom = vatom.as<double>();
     P.ip = (const IlpPair<R> *)(sizeof(R) == 8 ? ip_d.p : ip_f.p);
@@ -485,6 +486,9 @@ struct Ctx {
     P.shortsq = short_cut * short_cut;
     P.tersoff_cutmaxsq = ters.cutmax * ters.cutmax;
     P.ilp_cutsq = ilp_cut * ilp_cut;
+    // pairs just beyond the cutoff at build time can move into range before the next rebuild;
+    // 0.1 A covers what thermal motion does over a rebuild interval (the order only affects speed)
+    P.ilp_splitsq = (ilp_cut + 0.1) * (ilp_cut + 0.1);
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
@@ -600,22 +604,24 @@ struct Ctx {
     else hipLaunchKernelGGL((k_ilp<R, false, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
     mark_end(2, mk);
     mark_begin(3, mk);
-    const int nbz = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
-    if (ef || vf || va) hipLaunchKernelGGL((k_tersoff_zeta<R, true>), dim3(nbz), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_tersoff_zeta<R, false>), dim3(nbz), dim3(BS), 0, stream, P);
+    const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
+    if (va) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_tersoff_bonds<R, true, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_tersoff_bonds<R, false, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
     mark_end(3, mk);
     mark_begin(4, mk);
-    const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
-    if (va) hipLaunchKernelGGL((k_gather<R, true, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
-    else if (vf) hipLaunchKernelGGL((k_gather<R, true, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
-    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_gather<R, false, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    if (va) hipLaunchKernelGGL((k_gather<R, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_gather<R, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
     mark_end(4, mk);
   }
 
   void force(int eflag, int vflag) {
     if (n == 0) return;
     const int va = (vflag & 4) ? 1 : 0;
+    // per-atom virial shares of the Tersoff triplets: only allocated once asked for
+    if (va) wb.ensure((size_t)n * MAXSH * MAXSH * 6 * sizeof(double));
     const int vf = ((vflag % 4) != 0 || va) ? 1 : 0;
     const int ef = (eflag != 0 || vf) ? 1 : 0;
     if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va);
@@ -636,6 +642,7 @@ struct Ctx {
       fail(LMFF_ERR_NORMAL, "There are too many neighbors for some atoms, please check your configuration");
     if (e[0] & EB_NORMAL_ZERO) fail(LMFF_ERR_NORMAL, "The magnitude of the normal vector is zero");
     if (e[0] & EB_SHORT) fail(LMFF_ERR_SHORT, "more than %d Tersoff neighbours within cutmax", MAXSH);
+    if (e[0] & EB_GRID) fail(LMFF_ERR_STATE, "bin grid too coarse for the neighbor stencil");
   }
 
   // ---------------------------------------------------------------- thermo
@@ -886,10 +893,14 @@ void check_box(const double lo[3], const double hi[3], const int per[3], double 
   }
 }
 
+// Bins: at least a sixth of the list cutoff along x and a third along y and z (so the list
+// build's stencil has at most 7 x 7 rows; the factor keeps ceil(cut / bin) at 3 whatever the
+// rounding).
 void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
-  const double half = 0.5 * cutneigh;
+  const double target[3] = {cutneigh / 6.0, cutneigh / 3.0 * (1.0 + 1e-6), cutneigh / 3.0 * (1.0 + 1e-6)};
   s.ncells = 1;
   for (int d = 0; d < 3; d++) {
+    const double half = target[d];
     if (s.per[d]) {
       const double L = s.hi[d] - s.lo[d];
       s.nc[d] = std::max(1, (int)(L / half));
@@ -908,8 +919,10 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
       s.bs[d] = ext > 0 ? ext / s.nc[d] : half;
       if (s.bs[d] < half) s.bs[d] = half;
     }
-    s.ncells *= s.nc[d];
   }
+  const int64_t cells = (int64_t)s.nc[0] * s.nc[1] * s.nc[2];
+  if (cells > (int64_t)1 << 28) fail(LMFF_ERR_BOX, "box needs %lld neighbor bins (limit 2^28)", (long long)cells);
+  s.ncells = (int)cells;
 }
 
 }  // namespace
@@ -1329,7 +1342,7 @@ int lmff_memory(lmff_ctx *ctx, int64_t *bytes, double *per_atom) {
     auto &c = ctx->c;
     if (bytes) *bytes = (int64_t)c.bytes_in_use;
     if (per_atom) {
-      const double fixed = 4 * 32 + 10 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + 3 * MAXSH * 8 + 64 + 48 +
+      const double fixed = 5 * 32 + 11 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + MAXSH * 8 + MAXSH * MAXSH * 24 + 64 + 48 +
                            NTH * 8 + 48 + 32;
       *per_atom = fixed + 4.0 * (c.scap + std::max(c.icap, 448));
     }
@@ -1339,7 +1352,7 @@ int lmff_memory(lmff_ctx *ctx, int64_t *bytes, double *per_atom) {
 int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint) {
   if (!ctx) return -1;
   auto &c = ctx->c;
-  const double fixed = 4 * 32 + 10 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + 3 * MAXSH * 8 + 64 + 48 +
+  const double fixed = 5 * 32 + 11 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + MAXSH * 8 + MAXSH * MAXSH * 24 + 64 + 48 +
                        NTH * 8 + 48 + 32;
   const int layers = std::max(nlayers_hint, 2);
   const double list = 4.0 * (c.scap + (c.icap > 0 ? c.icap : 448 * std::min(layers - 1, 5)));
@@ -1359,9 +1372,13 @@ int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, 
       if (org[p] == atom) q = (int64_t)p;
     if (q < 0) fail(LMFF_ERR_ARG, "atom out of range");
     const size_t gq = (size_t)(s->begin + q);
-    int cnt = 0;
-    HIPCHK(hipMemcpy(&cnt, c.icnt.as<int>() + gq, sizeof(int), hipMemcpyDeviceToHost));
-    auto ent = c.download<uint32_t>(c.ilist, gq * (size_t)c.icap, (size_t)cnt);
+    int nfront = 0, nback = 0;
+    HIPCHK(hipMemcpy(&nfront, c.icnt.as<int>() + gq, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&nback, c.ocnt.as<int>() + gq, sizeof(int), hipMemcpyDeviceToHost));
+    const int cnt = nfront + nback;
+    auto ent = c.download<uint32_t>(c.ilist, gq * (size_t)c.icap, (size_t)nfront);
+    auto back = c.download<uint32_t>(c.ilist, gq * (size_t)c.icap + (size_t)(c.icap - nback), (size_t)nback);
+    ent.insert(ent.end(), back.begin(), back.end());
     auto orgall = c.download<int>(c.orig, 0, (size_t)c.n);
     std::vector<std::array<int32_t, 4>> r;
     for (int k = 0; k < cnt; k++) {

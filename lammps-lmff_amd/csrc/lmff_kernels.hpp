@@ -44,6 +44,7 @@ enum ErrBits : int {
   EB_NORMAL_ZERO = 2,
   EB_SHORT = 4,
   EB_NONFINITE = 8,
+  EB_GRID = 16,       // bin grid coarser than the list build's stencil allows
 };
 
 struct SysDev {
@@ -86,7 +87,8 @@ struct DevPtrs {
   const SysDev *sys_tab;
   // lists
   uint32_t *slist; int *scnt; int scap;
-  uint32_t *ilist; int *icnt; int icap;
+  // ILP rows: a front segment [0, icnt) and a back segment [icap - ocnt, icap)
+  uint32_t *ilist; int *icnt, *ocnt; int icap;
   // per step
   typename std::conditional<sizeof(R) == 8, double4, float4>::type *nrm;
   R *dn, *dndri;
@@ -95,7 +97,10 @@ struct DevPtrs {
   uint32_t *tsh; int *tcnt;
   uint32_t *rtc; int *rtcnt;  // [n][MAXSH] atoms that may hold i in their Tersoff list
   uint32_t *rnc; int *rncnt;  // [n][MAXSH] atoms that may hold i as a normal neighbour
-  double *fz, *pref, *eb;
+  double *eb;         // [n][MAXSH] bond energy 1/2 b_ij f_A
+  double *fb;         // [n][MAXSH][MAXSH][3] Tersoff force of bond (i, jj) on its kk-th neighbour
+  double *wb;         // [n][MAXSH][MAXSH][6] per-atom virial shares of the same (per-atom virial only)
+  double4 *ft;        // [n] Tersoff force of i's own bonds and repulsive halves
   double4 *fil, *S;
   double *T;          // [n][6] (per-atom virial only)
   double *th;         // [NTH][n]
@@ -107,6 +112,7 @@ struct DevPtrs {
   int *err;
   int *ovf;               // [0] max short count, [1] max ILP count
   double cutneighsq, shortsq, tersoff_cutmaxsq, ilp_cutsq, inv_rcut, rcut;
+  double ilp_splitsq;     // ILP rows hold entries with r^2 < ilp_splitsq (at build time) first
   int tap_flag;
 };
 
@@ -163,6 +169,16 @@ __device__ __forceinline__ void ghost(const double4 &xj, uint32_t e, const doubl
   gx = __dadd_rn(xj.x, (double)e_s(e, 0) * prd[0]);
   gy = __dadd_rn(xj.y, (double)e_s(e, 1) * prd[1]);
   gz = __dadd_rn(xj.z, (double)e_s(e, 2) * prd[2]);
+}
+
+// r^2 = dx*dx + dy*dy + dz*dz as the reference computes it (three rounded products, two
+// rounded sums, left to right; x86-64 code without FMA): every cutoff decision uses this, so
+// a pair within an ulp of a cutoff falls on the same side as in the reference.  The pair
+// arithmetic after the decision may contract freely.
+__device__ __forceinline__ double rsq_ref(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  const double a = dx * dx, b = dy * dy, c = dz * dz;
+  return (a + b) + c;
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8).  Atoms are
@@ -244,101 +260,178 @@ __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start)
 }
 
 // ---------------------------------------------------------------------------------------
-// Neighbor build: one wavefront per atom, stencil of +-2 bins (bins of half the list
-// cutoff, LMFF/nbin_standard.cpp:53-112), criterion rsq <= (cut+skin)^2 evaluated in the
-// reference's operation order with no fused multiply-adds (npair_full_bin_ghost.cpp:97-102).
+// Neighbor build (npair_full_bin_ghost.cpp:35-155 semantics: every image of every atom with
+// rsq <= (cut+skin)^2, evaluated in the reference's operation order with no fused
+// multiply-adds, 97-102).  One wavefront per atom.  Bins are long in y and z (at least a
+// third of the list cutoff, so at most 7 x 7 rows) and short in x.  Lane l takes stencil row
+// l = (dy, dz): it drops the row when the row's nearest point is beyond the cutoff, else
+// keeps only the x bins the cutoff sphere reaches on that row, as at most three runs of
+// consecutive sorted atoms (one per periodic image in x).  The wave lays the runs end to end
+// in LDS and then tests the candidates 64 at a time across run boundaries, so no lane idles
+// at the end of a short row.  Bins in a non-periodic dimension hold every atom beyond the
+// grid in the edge bin, so edge rows and bins reach to infinity.
+
+constexpr int RUNS_PER_LANE = 3;
+constexpr int RUN_SLOTS = 64 * RUNS_PER_LANE;
+
+// nearest distance from coordinate p to the extent of unwrapped bin u of one dimension
+__device__ __forceinline__ double bin_gap(double p, int u, int nc, bool per, double glo, double bs) {
+  const double lo = (!per && u == 0) ? -1e300 : glo + (double)u * bs;
+  const double hi = (!per && u == nc - 1) ? 1e300 : glo + (double)(u + 1) * bs;
+  return fmax(0.0, fmax(lo - p, p - hi));
+}
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *cell_start) {
-  const int i = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  __shared__ int run_beg[4][RUN_SLOTS];
+  __shared__ int run_off[4][RUN_SLOTS + 1];
+  __shared__ int run_sh[4][RUN_SLOTS];
+  const int wv = threadIdx.x >> 6;
+  const int i = xcd_block() * (blockDim.x >> 6) + wv;
   const int lane = threadIdx.x & 63;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double4 xi = P.x[i];
   const int mi = P.mol[i];
   const int ti = (int)xi.w - 1;
+  const double pos[3] = {xi.x, xi.y, xi.z};
   int c[3];
-  {
-    double cc[3] = {xi.x, xi.y, xi.z};
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      double f = floor((cc[d] - s.glo[d]) * s.inv_bs[d]);
-      c[d] = (int)fmax(fmin(f, (double)(s.nc[d] - 1)), 0.0);
+  for (int d = 0; d < 3; d++) {
+    const double f = floor((pos[d] - s.glo[d]) * s.inv_bs[d]);
+    c[d] = (int)fmax(fmin(f, (double)(s.nc[d] - 1)), 0.0);
+  }
+  // geometric slack for the bin arithmetic (coordinates are at most ~1e4 A, so their
+  // rounding is ~1e-12 A); pruning with it never drops an atom the test below accepts
+  const double slack = 1e-6;
+  const double cut = sqrt(P.cutneighsq);
+  const double bsy = 1.0 / s.inv_bs[1], bsz = 1.0 / s.inv_bs[2], bsx = 1.0 / s.inv_bs[0];
+  const int ry = (int)ceil(cut * s.inv_bs[1]), rz = (int)ceil(cut * s.inv_bs[2]);
+  const int wy = 2 * ry + 1;
+  // ---- lane l: stencil row (dy, dz) -> up to three runs ----
+  int nrun = 0, rb[RUNS_PER_LANE], rc_[RUNS_PER_LANE], rs[RUNS_PER_LANE];
+  if (wy * (2 * rz + 1) > 64) {
+    if (lane == 0) atomicOr(P.err, EB_GRID);
+    return;
+  }
+  if (lane < wy * (2 * rz + 1)) {
+    const int dy = lane % wy - ry, dz = lane / wy - rz;
+    int uy = c[1] + dy, uz = c[2] + dz;
+    bool ok = true;
+    if (!s.per[1] && (uy < 0 || uy >= s.nc[1])) ok = false;
+    if (!s.per[2] && (uz < 0 || uz >= s.nc[2])) ok = false;
+    if (ok) {
+      const double gy = fmax(0.0, bin_gap(pos[1], uy, s.nc[1], s.per[1], s.glo[1], bsy) - slack);
+      const double gz = fmax(0.0, bin_gap(pos[2], uz, s.nc[2], s.per[2], s.glo[2], bsz) - slack);
+      const double d2 = gy * gy + gz * gz;
+      if (d2 <= P.cutneighsq) {
+        int sy = 0, sz = 0;
+        if (s.per[1]) { const int w = ((uy % s.nc[1]) + s.nc[1]) % s.nc[1]; sy = (uy - w) / s.nc[1]; uy = w; }
+        if (s.per[2]) { const int w = ((uz % s.nc[2]) + s.nc[2]) % s.nc[2]; sz = (uz - w) / s.nc[2]; uz = w; }
+        const double w = sqrt(P.cutneighsq - d2) + slack;
+        // bins in reach along x; in a periodic dimension at most nc + 2 of them (w < L/2), so
+        // at most three images; in a non-periodic one the edge bins hold what lies beyond
+        const int rx = (int)ceil(cut * s.inv_bs[0]) + 1;
+        int xlo = (int)fmax(floor((pos[0] - w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] - rx));
+        int xhi = (int)fmin(floor((pos[0] + w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] + rx));
+        if (!s.per[0]) {
+          xlo = min(max(xlo, 0), s.nc[0] - 1);
+          xhi = min(max(xhi, 0), s.nc[0] - 1);
+        }
+        const int rowbase = s.cell_off + (uz * s.nc[1] + uy) * s.nc[0];
+        int u = xlo;
+        while (u <= xhi && nrun < RUNS_PER_LANE) {
+          int sx = 0, wx = u, uend = xhi;
+          if (s.per[0]) {
+            wx = ((u % s.nc[0]) + s.nc[0]) % s.nc[0];
+            sx = (u - wx) / s.nc[0];
+            uend = min(xhi, u + (s.nc[0] - 1 - wx));
+          }
+          const int a0 = cell_start[rowbase + wx], a1 = cell_start[rowbase + wx + (uend - u) + 1];
+          rb[nrun] = a0;
+          rc_[nrun] = a1 - a0;
+          rs[nrun] = (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4);
+          nrun++;
+          u = uend + 1;
+        }
+      }
     }
   }
-  int ns = 0, ni = 0;
+  // ---- lay the runs end to end: wave prefix sums of candidate and run counts ----
+  int mycnt = 0, mynr = 0;
+  for (int k = 0; k < nrun; k++)
+    if (rc_[k] > 0) { mycnt += rc_[k]; mynr++; }
+  int scnt_ = mycnt, snr = mynr;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int a = __shfl_up(scnt_, o, 64), b = __shfl_up(snr, o, 64);
+    if (lane >= o) { scnt_ += a; snr += b; }
+  }
+  const int total = __shfl(scnt_, 63, 64), nruns = __shfl(snr, 63, 64);
+  {
+    int off = scnt_ - mycnt, slot = snr - mynr;
+    for (int k = 0; k < nrun; k++) {
+      if (rc_[k] <= 0) continue;
+      run_beg[wv][slot] = rb[k];
+      run_off[wv][slot] = off;
+      run_sh[wv][slot] = rs[k];
+      off += rc_[k];
+      slot++;
+    }
+    if (lane == 0) run_off[wv][nruns] = total;
+  }
+  // the table is read by other lanes of this wave only
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  // ILP rows are partitioned: entries within the split radius at build time are appended from
+  // the front, the rest from the back.  The per-step pair kernel walks the front segment and
+  // then the back one in chunks of 64, so the chunks past the split radius have (almost) no
+  // pair in range and skip the pair arithmetic as a whole.
+  int ns = 0, ni = 0, no = 0;
   uint32_t *irow = P.ilist + (size_t)i * P.icap;
-  for (int dz = -2; dz <= 2; dz++) {
-    int cz = c[2] + dz, sz = 0;
-    if (cz < 0 || cz >= s.nc[2]) {
-      if (!s.per[2]) continue;
-      int w = ((cz % s.nc[2]) + s.nc[2]) % s.nc[2];
-      sz = (cz - w) / s.nc[2];
-      cz = w;
-    }
-    for (int dy = -2; dy <= 2; dy++) {
-      int cy = c[1] + dy, sy = 0;
-      if (cy < 0 || cy >= s.nc[1]) {
-        if (!s.per[1]) continue;
-        int w = ((cy % s.nc[1]) + s.nc[1]) % s.nc[1];
-        sy = (cy - w) / s.nc[1];
-        cy = w;
-      }
-      const int rowbase = s.cell_off + (cz * s.nc[1] + cy) * s.nc[0];
-      // x runs: consecutive stencil bins with the same image shift form one atom range
-      int dx = -2;
-      while (dx <= 2) {
-        int cx = c[0] + dx, sx = 0;
-        if (cx < 0 || cx >= s.nc[0]) {
-          if (!s.per[0]) { dx++; continue; }
-          int w = ((cx % s.nc[0]) + s.nc[0]) % s.nc[0];
-          sx = (cx - w) / s.nc[0];
-          cx = w;
-        }
-        int cx_end = cx, dx_end = dx;
-        while (dx_end + 1 <= 2) {
-          int nx = c[0] + dx_end + 1;
-          int nsx = 0, nw = nx;
-          if (nx < 0 || nx >= s.nc[0]) {
-            if (!s.per[0]) break;
-            nw = ((nx % s.nc[0]) + s.nc[0]) % s.nc[0];
-            nsx = (nx - nw) / s.nc[0];
-          }
-          if (nsx != sx || nw != cx_end + 1) break;
-          cx_end = nw;
-          dx_end++;
-        }
-        const int a0 = cell_start[rowbase + cx], a1 = cell_start[rowbase + cx_end + 1];
-        const double shx = (double)sx * s.prd[0], shy = (double)sy * s.prd[1], shz = (double)sz * s.prd[2];
-        const bool selfimg = (sx == 0 && sy == 0 && sz == 0);
-        for (int base = a0; base < a1; base += 64) {
-          const int j = base + lane;
-          bool in_i = false, in_s = false;
-          if (j < a1 && !(selfimg && j == i)) {
-            const double4 xj = P.x[j];
-            const double gx = __dadd_rn(xj.x, shx), gy = __dadd_rn(xj.y, shy), gz = __dadd_rn(xj.z, shz);
-            const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
-            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(delx, delx), __dmul_rn(dely, dely)), __dmul_rn(delz, delz));
-            const int tj = (int)xj.w - 1;
-            in_i = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && P.ip[ti * MAXT + tj].mapped;
-            in_s = (rsq <= P.shortsq);
-          }
-          const uint32_t ent = e_make(j, sx, sy, sz);
-          const unsigned long long bi = __ballot(in_i), bs = __ballot(in_s);
-          const int pi = __popcll(bi & ((1ull << lane) - 1ull)), ps = __popcll(bs & ((1ull << lane) - 1ull));
-          if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
-          // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
-          // readers coalesce
-          if (in_s && ns + ps < P.scap) P.slist[(size_t)(ns + ps) * P.n + i] = ent;
-          ni += __popcll(bi);
-          ns += __popcll(bs);
-        }
-        dx = dx_end + 1;
+  int r = 0;
+  for (int v0 = 0; v0 < total; v0 += 64) {
+    const int v = v0 + lane;
+    bool in_i = false, in_o = false, in_s = false;
+    uint32_t ent = 0;
+    if (v < total) {
+      while (run_off[wv][r + 1] <= v) r++;
+      const int j = run_beg[wv][r] + (v - run_off[wv][r]);
+      const int sh = run_sh[wv][r];
+      const int sx = (sh & 3) - 1, sy = ((sh >> 2) & 3) - 1, sz = ((sh >> 4) & 3) - 1;
+      ent = e_make(j, sx, sy, sz);
+      if (!(sh == 0x15 && j == i)) {
+        const double4 xj = P.x[j];
+        const double gx = __dadd_rn(xj.x, (double)sx * s.prd[0]);
+        const double gy = __dadd_rn(xj.y, (double)sy * s.prd[1]);
+        const double gz = __dadd_rn(xj.z, (double)sz * s.prd[2]);
+        const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
+        const double rsq = rsq_ref(delx, dely, delz);
+        const int tj = (int)xj.w - 1;
+        const bool ilp = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && P.ip[ti * MAXT + tj].mapped;
+        in_i = ilp && rsq < P.ilp_splitsq;
+        in_o = ilp && !(rsq < P.ilp_splitsq);
+        in_s = (rsq <= P.shortsq);
       }
     }
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long bi = __ballot(in_i), bo = __ballot(in_o), bs = __ballot(in_s);
+    const int pi = __popcll(bi & lt), po = __popcll(bo & lt), ps = __popcll(bs & lt);
+    if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
+    if (in_o && no + po < P.icap) irow[P.icap - 1 - (no + po)] = ent;
+    // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
+    // readers coalesce
+    if (in_s && ns + ps < P.scap) P.slist[(size_t)(ns + ps) * P.n + i] = ent;
+    ni += __popcll(bi);
+    no += __popcll(bo);
+    ns += __popcll(bs);
   }
   if (lane == 0) {
-    P.icnt[i] = ni < P.icap ? ni : P.icap;
+    const int front = ni < P.icap ? ni : P.icap;
+    P.icnt[i] = front;
+    P.ocnt[i] = no < P.icap - front ? no : P.icap - front;
+    ni += no;
     P.scnt[i] = ns < P.scap ? ns : P.scap;
     // only an overflowing row reports its size (a per-wave atomic on one address would
     // serialize every wave of the launch)
@@ -346,6 +439,10 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     if (ns > P.scap) atomicMax(&P.ovf[0], ns);
   }
 }
+
+// slot of the k-th entry of an ILP row: front segment first, then the back segment
+// (back_off = icap - ocnt - icnt)
+__device__ __forceinline__ int ilp_slot(int k, int nfront, int back_off) { return k < nfront ? k : k + back_off; }
 
 // ---------------------------------------------------------------------------------------
 // Per step 1: Tersoff neighbours within cutmax, ILP normal neighbours and the normal with its
@@ -393,7 +490,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
       double gx, gy, gz;
       ghost(xj, e, s.prd, gx, gy, gz);
       const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
-      const double rsq = delx * delx + dely * dely + delz * delz;
+      const double rsq = rsq_ref(delx, dely, delz);
       const int tj = (int)xj.w - 1;
       const bool tmapped = P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0;
       if (rsq < P.tersoff_cutmaxsq && tmapped) {
@@ -590,14 +687,15 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
   const int ti = (int)xi.w - 1;
   const auto ni4 = P.nrm[i];
   const R nix = ni4.x, niy = ni4.y, niz = ni4.z;
-  const int cnt = P.icnt[i];
+  const int nfront = P.icnt[i], nback = P.ocnt[i];
+  const int cnt = nfront + nback, back_off = P.icap - nback - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
   const int itag = P.tag[i];
   const R inv_rcut = (R)P.inv_rcut;
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
   double w[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0}, tt[6] = {0, 0, 0, 0, 0, 0};
   for (int k = lane; k < cnt; k += 64) {
-    const uint32_t e = row[k];
+    const uint32_t e = row[ilp_slot(k, nfront, back_off)];
     const int j = e_idx(e);
     const double4 xj = P.x[j];
     double gx, gy, gz;
@@ -610,7 +708,7 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
       ghost(xj, e, prd, gx, gy, gz);
     }
     const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
-    const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+    const double rsqd = rsq_ref(dxd, dyd, dzd);
     if (rsqd >= P.ilp_cutsq) continue;
     const int tj = (int)xj.w - 1;
     const IlpPair<R> &pij = P.ip[ti * MAXT + tj];
@@ -629,52 +727,53 @@ __global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
     const R TSinv = M::rcp(TS);
     const R Vvdw = -pv->C6 * r6inv * TSinv;
     const R fpv = (R)-6 * pv->C6 * r8inv * TSinv + pv->C6_d_seff * (TS - (R)1) * TSinv * TSinv * r8inv * r;
-    const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
     // ---- repulsion (i,j) with n_i ----
     const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
     const R pn1 = nix * dx + niy * dy + niz * dz;
     const R rd1 = (rsq - pn1 * pn1) * pij.d2inv;
     const R frho1 = M::ex(-rd1) * pij.C;
-    const R Er1 = pij.half_eps + frho1;
-    const R V1 = exp0 * Er1;
-    const R fp1 = pij.lambda * exp0 * rinv * Er1;
+    const R V1 = exp0 * (pij.half_eps + frho1);
     const R fq1 = (R)2 * exp0 * frho1 * pij.d2inv;
-    const R fs1 = fp1 + fq1;
     const R a1 = pn1 * fq1;
-    const R b1 = V1 * tp.dt * rinv;
-    const R fkx = (dx * fs1 - a1 * nix) * tp.t - b1 * dx;
-    const R fky = (dy * fs1 - a1 * niy) * tp.t - b1 * dy;
-    const R fkz = (dz * fs1 - a1 * niz) * tp.t - b1 * dz;
-    const R wgt = a1 * tp.t;
     // ---- repulsion (j,i) with n_j; del_ji = -del ----
     const auto nj4 = P.nrm[j];
     const R exp0b = pij.esym ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
     const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dz);
     const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
     const R frho2 = M::ex(-rd2) * pji.C;
-    const R Er2 = pji.half_eps + frho2;
-    const R V2 = exp0b * Er2;
-    const R fp2 = pji.lambda * exp0b * rinv * Er2;
+    const R V2 = exp0b * (pji.half_eps + frho2);
     const R fq2 = (R)2 * exp0b * frho2 * pji.d2inv;
-    const R fs2 = fp2 + fq2;
     const R a2 = pn2 * fq2;
-    const R b2 = V2 * tp.dt * rinv;
-    // fkc_ji = (del_ji*fs2 - a2*n_j)*Tap - b2*del_ji, del_ji = -del; force on i is -fkc_ji
-    const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
-    const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
-    const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
-    // the pair's three force terms are combined in R, then accumulated in FP64
-    fx += (double)(fsv * dx + fkx - gkx);
-    fy += (double)(fsv * dy + fky - gky);
-    fz += (double)(fsv * dz + fkz - gkz);
-    sx += (double)(wgt * dx);
-    sy += (double)(wgt * dy);
-    sz += (double)(wgt * dz);
+    // Force on i: fsum_vdw del + fkc_ij - fkc_ji (ilp.cpp:525-534, 640-648) with
+    //   fsum_vdw = fpv Tap - V_vdw dTap / r,  fkc_ij = (del fs1 - a1 n_i) Tap - V1 dTap del / r,
+    //   fkc_ji = (-del fs2 - a2 n_j) Tap + V2 dTap del / r,  fs = lambda V / r + fq,
+    // collected as  C del + Tap (a2 n_j - a1 n_i)  (one scalar per pair instead of three
+    // vectors).  The weight of the normal-derivative forces is w_ij = a1 Tap.
+    const R ta1 = a1 * tp.t, ta2 = a2 * tp.t;
+    const R Cd = tp.t * (fpv + rinv * (pij.lambda * V1 + pji.lambda * V2) + fq1 + fq2) -
+                 tp.dt * rinv * (Vvdw + V1 + V2);
+    fx += (double)(Cd * dx + (ta2 * nj4.x - ta1 * nix));
+    fy += (double)(Cd * dy + (ta2 * nj4.y - ta1 * niy));
+    fz += (double)(Cd * dz + (ta2 * nj4.z - ta1 * niz));
+    sx += (double)(ta1 * dx);
+    sy += (double)(ta1 * dy);
+    sz += (double)(ta1 * dz);
     if (EF) {
       ev += 0.5 * (double)(Vvdw * tp.t);
       er += 0.5 * (double)(tp.t * V1) + 0.5 * (double)(tp.t * V2);
     }
     if (VF || VA) {
+      // the separate force terms, for the tallies
+      const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
+      const R fs1 = pij.lambda * rinv * V1 + fq1, b1 = V1 * tp.dt * rinv;
+      const R fs2 = pji.lambda * rinv * V2 + fq2, b2 = V2 * tp.dt * rinv;
+      const R fkx = (dx * fs1 - a1 * nix) * tp.t - b1 * dx;
+      const R fky = (dy * fs1 - a1 * niy) * tp.t - b1 * dy;
+      const R fkz = (dz * fs1 - a1 * niz) * tp.t - b1 * dz;
+      const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
+      const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
+      const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
+      const R wgt = ta1;
       // vdW: half of del (x) del * fsum per ordered pair; repulsion (i,j): del (x) fkc
       const double hv = 0.5 * (double)fsv;
       const double dd[3] = {dxd, dyd, dzd};
@@ -881,36 +980,81 @@ __device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const doubl
   d[0] = gx - xi.x;
   d[1] = gy - xi.y;
   d[2] = gz - xi.z;
-  rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  rsq = rsq_ref(d[0], d[1], d[2]);
   t = (int)xj.w - 1;
 }
 
-// Per step 3: zeta, b_ij and force_zeta for every Tersoff bond of i (pair_tersoff.cpp:177-225)
-template <typename R, bool EF>
-__global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
+// Per step 3: the Tersoff bonds of i, one lane per bond (four lanes per atom, lane q takes
+// bonds q, q+4, ...).  For bond (i, j): zeta_ij and b_ij (zeta 599, force_zeta 621-640), the
+// three-body forces of every triplet (i, j, k) (attractive 642, ters_zetaterm_d 738-795), and
+// the repulsive pair with i's half (584-597; pair_tersoff.cpp:141-262 for the loop structure).
+// Every triplet is evaluated once.  What acts on i is summed over the quad into ft[i]; the
+// force on j and on every k is stored per bond in fb[i][jj][kk] (kk == jj: the force on j),
+// where the gather kernel picks it up.  The global virial of the bond is kept at i; per-atom
+// virial shares (v_tally3 thirds, ev_tally halves) are stored in wb for the gather.
+template <typename R, bool EF, bool VF, bool VA>
+__global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
   using RT = double;
   using M = MathOps<RT>;
-  // four lanes per atom; lane q handles bonds q, q+4, ...
   const int gt = xcd_block() * blockDim.x + threadIdx.x;
   const int i = gt >> 2;
   const int quad = gt & 3;
-  if (i >= P.n) return;
+  if (i >= P.n) return;  // the whole quad leaves together
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
   const double4 xi = P.x[i];
   const int ti = (int)xi.w - 1;
   const int ns = P.tcnt[i];
+  const int itag = P.tag[i];
   const uint32_t *row = P.tsh + (size_t)i * MAXSH;
+  const size_t n = (size_t)P.n;
+  double4 F = {0.0, 0.0, 0.0, 0.0};
+  double et = 0.0;
+  double vg[6] = {0, 0, 0, 0, 0, 0};  // global virial of i's bonds (VF)
+  double vo[6] = {0, 0, 0, 0, 0, 0};  // i's own per-atom share (VA)
+  const double third = 1.0 / 3.0;
   for (int jj = quad; jj < ns; jj += 4) {
+    const uint32_t ej = row[jj];
     const size_t slot = (size_t)i * MAXSH + jj;
+    double *fbj = P.fb + slot * MAXSH * 3;
     double dj[3], rsqj;
     int tj;
-    nb_del(P.x, row[jj], prd, xi, dj, rsqj, tj);
+    nb_del(P.x, ej, prd, xi, dj, rsqj, tj);
+    // ---- repulsion, i's half; the parameter is the one of whichever atom computes the pair
+    //      in the reference (tag parity), so both halves agree ----
+    {
+      double gx, gy, gz;
+      ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
+      const bool skip = tag_skip(itag, P.tag[e_idx(ej)], xi.x, xi.y, xi.z, gx, gy, gz);
+      const int pr = skip ? P.t3[(tj * MAXT + ti) * MAXT + ti] : P.t3[(ti * MAXT + tj) * MAXT + tj];
+      const TersP<double> &p = P.tp[pr];
+      if (rsqj < p.cutsq) {
+        const double r = sqrt(rsqj);
+        const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
+        const double tmp_exp = exp(-p.lam1 * r);
+        const double fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
+        // dj is x_j - x_i; the reference's del is x_i - x_j
+        F.x -= dj[0] * fpair;
+        F.y -= dj[1] * fpair;
+        F.z -= dj[2] * fpair;
+        if (EF) et += 0.5 * (tmp_fc * p.biga * tmp_exp);
+        if (VF || VA) {
+          const double hf = 0.5 * fpair;
+          const double w[6] = {hf * dj[0] * dj[0], hf * dj[1] * dj[1], hf * dj[2] * dj[2],
+                               hf * dj[0] * dj[1], hf * dj[0] * dj[2], hf * dj[1] * dj[2]};
+          for (int c = 0; c < 6; c++) { vg[c] += w[c]; vo[c] += w[c]; }
+        }
+      }
+    }
+    // ---- bond (i, j) ----
     const TersP<RT> &p = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tj]];
     if (rsqj >= p.cutsq) {
-      P.fz[slot] = 0;
-      P.pref[slot] = 0;
-      if (EF) P.eb[slot] = 0;
+      for (int kk = 0; kk < ns; kk++) {
+        fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
+        if (VA)
+          for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + kk) * 6 + c] = 0.0;
+      }
+      if (EF) P.eb[slot] = 0.0;
       continue;
     }
     RT zeta = 0;
@@ -928,7 +1072,6 @@ __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
       const RT ex = t_exdelr(rij, rik, q, dd);
       zeta += t_fc(rik, q) * t_gijk(costheta, q) * ex;
     }
-    // force_zeta (pair_tersoff.cpp:621-640)
     const RT r = rij;
     RT fa, fa_d;
     if (r > p.bigr + p.bigd) {
@@ -940,223 +1083,63 @@ __global__ __launch_bounds__(256) void k_tersoff_zeta(DevPtrs<R> P) {
       fa_d = p.bigb * e2 * (p.lam2 * t_fc(r, p) - t_fc_d(r, p));
     }
     const RT bij = t_bij(zeta, p);
-    P.fz[slot] = (double)((RT)0.5 * bij * fa_d / r);
-    P.pref[slot] = (double)((RT)-0.5 * fa * t_bij_d(zeta, p));
-    if (EF) P.eb[slot] = (double)((RT)0.5 * bij * fa);
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ void tally3(double *va, const R *delr1, const R *fj, const R *delr2, const R *fk) {
-  const double th = 1.0 / 3.0;
-  va[0] += th * ((double)delr1[0] * fj[0] + (double)delr2[0] * fk[0]);
-  va[1] += th * ((double)delr1[1] * fj[1] + (double)delr2[1] * fk[1]);
-  va[2] += th * ((double)delr1[2] * fj[2] + (double)delr2[2] * fk[2]);
-  va[3] += th * ((double)delr1[0] * fj[1] + (double)delr2[0] * fk[1]);
-  va[4] += th * ((double)delr1[0] * fj[2] + (double)delr2[0] * fk[2]);
-  va[5] += th * ((double)delr1[1] * fj[2] + (double)delr2[1] * fk[2]);
-}
-
-// Per step 4: everything that acts on atom a besides the ILP pair sum -- the normal-derivative
-// forces (own d n_a / d x_a term and a as a normal neighbour of m), Tersoff repulsion, and the
-// Tersoff bond-order forces with a as i, as j and as k.  Four lanes per atom: lane 0 takes
-// the atom's own terms (normal derivatives, repulsion, bonds with a as i), lanes 1-3 take one
-// reverse Tersoff partner i each (a as j of bond i-a and as k of the bonds of i); the quad is
-// summed with a fixed-order shuffle butterfly.
-template <typename R, bool EF, bool VF, bool VA>
-__global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
-  const int gt = xcd_block() * blockDim.x + threadIdx.x;
-  const int a = gt >> 2;
-  const int quad = gt & 3;
-  if (a >= P.n) return;  // the whole quad leaves together
-  const SysDev &s = P.sys_tab[P.sys[a]];
-  const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
-  const double4 xa = P.x[a];
-  const int ta = (int)xa.w - 1;
-  const size_t n = (size_t)P.n;
-  double4 F = {0.0, 0.0, 0.0, 0.0};
-  double et = 0.0;
-  double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff tallies (also the Tersoff global virial share)
-  double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
-  if (quad == 0) {
-  F = P.fil[a];
-  // ---- ILP: d n_a / d x_a term (2-neighbour normals only) ----
-  const uint32_t *na = P.nnb + (size_t)a * 4;
-  if (na[0] == 2) {
-    const double4 Sa = P.S[a];
-    const R *Dri = P.dndri + (size_t)a * 9;
-    for (int c = 0; c < 3; c++) {
-      const double v = (double)Dri[0 * 3 + c] * Sa.x + (double)Dri[1 * 3 + c] * Sa.y + (double)Dri[2 * 3 + c] * Sa.z;
-      (&F.x)[c] -= v;
+    const double fzv = (double)((RT)0.5 * bij * fa_d / r);
+    const double pref = (double)((RT)-0.5 * fa * t_bij_d(zeta, p));
+    if (EF) {
+      const double eb = (double)((RT)0.5 * bij * fa);
+      P.eb[slot] = eb;
+      et += 0.5 * eb;
     }
-  }
-  // ---- ILP: a as a normal neighbour of m (m among a's reverse normal candidates) ----
-  const int rn = P.rncnt[a];
-  const uint32_t *rnrow = P.rnc + (size_t)a * MAXSH;
-  for (int q = 0; q < rn; q++) {
-    const int m = (int)rnrow[q];
-    const uint32_t *nm = P.nnb + (size_t)m * 4;
-    const int cm = (int)nm[0];
-    for (int kk = 0; kk < cm; kk++) {
-      if (e_idx(nm[1 + kk]) != a) continue;
-      const double4 Sm = P.S[m];
-      const R *D = P.dn + (size_t)m * 27 + kk * 9;
-      double Fk[3];
-      for (int c = 0; c < 3; c++) Fk[c] = -((double)D[0 * 3 + c] * Sm.x + (double)D[1 * 3 + c] * Sm.y + (double)D[2 * 3 + c] * Sm.z);
-      F.x += Fk[0];
-      F.y += Fk[1];
-      F.z += Fk[2];
-      if (VA) {
-        // 0.5 * (vet_k(m) (x) F_k - T_m D_k)
-        const double *vt = P.vet + (size_t)m * 9 + kk * 3;
-        const double *Tm = P.T + (size_t)m * 6;
-        const double Tf[3][3] = {{Tm[0], Tm[3], Tm[4]}, {Tm[3], Tm[1], Tm[5]}, {Tm[4], Tm[5], Tm[2]}};
-        double TD[3][3];
-        for (int r0 = 0; r0 < 3; r0++)
-          for (int c = 0; c < 3; c++)
-            TD[r0][c] = Tf[r0][0] * (double)D[0 * 3 + c] + Tf[r0][1] * (double)D[1 * 3 + c] + Tf[r0][2] * (double)D[2 * 3 + c];
-        vk[0] += 0.5 * (vt[0] * Fk[0] - TD[0][0]);
-        vk[1] += 0.5 * (vt[1] * Fk[1] - TD[1][1]);
-        vk[2] += 0.5 * (vt[2] * Fk[2] - TD[2][2]);
-        vk[3] += 0.5 * (vt[0] * Fk[1] - TD[0][1]);
-        vk[4] += 0.5 * (vt[0] * Fk[2] - TD[0][2]);
-        vk[5] += 0.5 * (vt[1] * Fk[2] - TD[1][2]);
-      }
-      break;
-    }
-  }
-  // ---- Tersoff: repulsion and bonds with a as i ----
-  const int ns = P.tcnt[a];
-  const uint32_t *trow = P.tsh + (size_t)a * MAXSH;
-  const int atag = P.tag[a];
-  for (int jj = 0; jj < ns; jj++) {
-    const uint32_t e = trow[jj];
-    double dj[3], rsqj;
-    int tj;
-    nb_del(P.x, e, prd, xa, dj, rsqj, tj);
-    // repulsion: parameter of whichever atom computes the pair in the reference (tag parity)
-    const double4 xj = P.x[e_idx(e)];
-    double gx, gy, gz;
-    ghost(xj, e, prd, gx, gy, gz);
-    const bool skip = tag_skip(atag, P.tag[e_idx(e)], xa.x, xa.y, xa.z, gx, gy, gz);
-    const int pr = skip ? P.t3[(tj * MAXT + ta) * MAXT + ta] : P.t3[(ta * MAXT + tj) * MAXT + tj];
-    const TersP<double> &p = P.tp[pr];
-    if (rsqj < p.cutsq) {
-      const double r = sqrt(rsqj);
-      const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
-      const double tmp_exp = exp(-p.lam1 * r);
-      const double fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
-      // dj is x_j - x_a; the reference's del is x_a - x_j
-      F.x -= dj[0] * fpair;
-      F.y -= dj[1] * fpair;
-      F.z -= dj[2] * fpair;
-      if (EF) et += 0.5 * (tmp_fc * p.biga * tmp_exp);
-      if (VF || VA) {
-        const double hf = 0.5 * fpair;
-        va[0] += hf * dj[0] * dj[0]; va[1] += hf * dj[1] * dj[1]; va[2] += hf * dj[2] * dj[2];
-        va[3] += hf * dj[0] * dj[1]; va[4] += hf * dj[0] * dj[2]; va[5] += hf * dj[1] * dj[2];
-      }
-    }
-    // bond (a, j): force_zeta and the three-body terms with a as i
-    const size_t slot = (size_t)a * MAXSH + jj;
-    const double fzv = P.fz[slot];
-    const double pref = P.pref[slot];
-    if (fzv == 0.0 && pref == 0.0) continue;
+    // force_zeta: +del*fpair on i, -del*fpair on j (ev_tally halves of the virial)
     F.x += dj[0] * fzv;
     F.y += dj[1] * fzv;
     F.z += dj[2] * fzv;
-    if (EF) et += 0.5 * P.eb[slot];
+    double fjx = -dj[0] * fzv, fjy = -dj[1] * fzv, fjz = -dj[2] * fzv;
+    double vj[6] = {0, 0, 0, 0, 0, 0};  // j's per-atom share (VA)
     if (VF || VA) {
       const double hf = -0.5 * fzv;
-      va[0] += hf * dj[0] * dj[0]; va[1] += hf * dj[1] * dj[1]; va[2] += hf * dj[2] * dj[2];
-      va[3] += hf * dj[0] * dj[1]; va[4] += hf * dj[0] * dj[2]; va[5] += hf * dj[1] * dj[2];
+      const double w[6] = {hf * dj[0] * dj[0], hf * dj[1] * dj[1], hf * dj[2] * dj[2],
+                           hf * dj[0] * dj[1], hf * dj[0] * dj[2], hf * dj[1] * dj[2]};
+      for (int c = 0; c < 6; c++) { vg[c] += 2.0 * w[c]; vo[c] += w[c]; vj[c] = w[c]; }
     }
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
       double dk[3], rsqk;
       int tk;
-      nb_del(P.x, trow[kk], prd, xa, dk, rsqk, tk);
-      const TersP<double> &q = P.tp[P.t3[(ta * MAXT + tj) * MAXT + tk]];
-      if (rsqk >= q.cutsq) continue;
+      nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
+      const TersP<double> &q = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tk]];
+      if (rsqk >= q.cutsq) {
+        fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
+        if (VA)
+          for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + kk) * 6 + c] = 0.0;
+        continue;
+      }
       double fi[3], fj[3], fk[3];
       t_attractive(q, pref, rsqj, rsqk, dj, dk, fi, fj, fk);
       F.x += fi[0];
       F.y += fi[1];
       F.z += fi[2];
-      if (VF || VA) tally3(va, dj, fj, dk, fk);
-    }
-  }
-  } else {
-  // ---- Tersoff: a as j (bond i-a) and as k (bond i-j through a), i among a's reverse
-  //      Tersoff candidates (lane q takes candidates q-1, q+2, ...) ----
-  const int rt = P.rtcnt[a];
-  const uint32_t *rtrow = P.rtc + (size_t)a * MAXSH;
-  for (int q = quad - 1; q < rt; q += 3) {
-    const int i = (int)rtrow[q];
-    const int ni = P.tcnt[i];
-    if (ni == 0) continue;
-    const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
-    int slot_a = -1;
-    for (int b = 0; b < ni; b++)
-      if (e_idx(irow[b]) == a) { slot_a = b; break; }
-    if (slot_a < 0) continue;
-    const double4 xi = P.x[i];
-    const int ti = (int)xi.w - 1;
-    // displacement of a seen from i, with i's own image shift (as thread i computes it)
-    double da[3], rsqa;
-    int ta_;
-    nb_del(P.x, irow[slot_a], prd, xi, da, rsqa, ta_);
-    // a as j of bond (i, a)
-    {
-      const size_t slot = (size_t)i * MAXSH + slot_a;
-      const double fzv = P.fz[slot];
-      const double pref = P.pref[slot];
-      if (!(fzv == 0.0 && pref == 0.0)) {
-        F.x -= da[0] * fzv;
-        F.y -= da[1] * fzv;
-        F.z -= da[2] * fzv;
-        if (EF) et += 0.5 * P.eb[slot];
-        if (VF || VA) {
-          const double hf = -0.5 * fzv;
-          va[0] += hf * da[0] * da[0]; va[1] += hf * da[1] * da[1]; va[2] += hf * da[2] * da[2];
-          va[3] += hf * da[0] * da[1]; va[4] += hf * da[0] * da[2]; va[5] += hf * da[1] * da[2];
-        }
-        for (int kk = 0; kk < ni; kk++) {
-          if (kk == slot_a) continue;
-          double dk[3], rsqk;
-          int tk;
-          nb_del(P.x, irow[kk], prd, xi, dk, rsqk, tk);
-          const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + ta) * MAXT + tk]];
-          if (rsqk >= qp.cutsq) continue;
-          double fi[3], fj[3], fk[3];
-          t_attractive(qp, pref, rsqa, rsqk, da, dk, fi, fj, fk);
-          F.x += fj[0];
-          F.y += fj[1];
-          F.z += fj[2];
-          if (VF || VA) tally3(va, da, fj, dk, fk);
+      fjx += fj[0];
+      fjy += fj[1];
+      fjz += fj[2];
+      fbj[kk * 3 + 0] = fk[0]; fbj[kk * 3 + 1] = fk[1]; fbj[kk * 3 + 2] = fk[2];
+      if (VF || VA) {
+        // v_tally3: delr1 (x) fj + delr2 (x) fk, a third to each of i, j, k
+        const double w[6] = {dj[0] * fj[0] + dk[0] * fk[0], dj[1] * fj[1] + dk[1] * fk[1],
+                             dj[2] * fj[2] + dk[2] * fk[2], dj[0] * fj[1] + dk[0] * fk[1],
+                             dj[0] * fj[2] + dk[0] * fk[2], dj[1] * fj[2] + dk[1] * fk[2]};
+        for (int c = 0; c < 6; c++) {
+          vg[c] += w[c];
+          const double t = third * w[c];
+          vo[c] += t;
+          vj[c] += t;
+          if (VA) P.wb[(slot * MAXSH + kk) * 6 + c] = t;
         }
       }
     }
-    // a as k of bond (i, j)
-    for (int jj = 0; jj < ni; jj++) {
-      if (jj == slot_a) continue;
-      const size_t slot = (size_t)i * MAXSH + jj;
-      const double fzv = P.fz[slot];
-      const double pref = P.pref[slot];
-      if (fzv == 0.0 && pref == 0.0) continue;
-      double dj[3], rsqj;
-      int tj;
-      nb_del(P.x, irow[jj], prd, xi, dj, rsqj, tj);
-      const TersP<double> &qp = P.tp[P.t3[(ti * MAXT + tj) * MAXT + ta]];
-      if (rsqa >= qp.cutsq) continue;
-      double fi[3], fj[3], fk[3];
-      t_attractive(qp, pref, rsqj, rsqa, dj, da, fi, fj, fk);
-      F.x += fk[0];
-      F.y += fk[1];
-      F.z += fk[2];
-      if (VF || VA) tally3(va, dj, fj, da, fk);
-    }
-  }
+    fbj[jj * 3 + 0] = fjx; fbj[jj * 3 + 1] = fjy; fbj[jj * 3 + 2] = fjz;
+    if (VA)
+      for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + jj) * 6 + c] = vj[c];
   }
   // quad reduction, the same order in every lane
 #pragma unroll
@@ -1165,17 +1148,127 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
     F.y += __shfl_xor(F.y, o, 64);
     F.z += __shfl_xor(F.z, o, 64);
     if (EF) et += __shfl_xor(et, o, 64);
-    if (VF || VA)
-      for (int c = 0; c < 6; c++) va[c] += __shfl_xor(va[c], o, 64);
+    if (VF)
+      for (int c = 0; c < 6; c++) vg[c] += __shfl_xor(vg[c], o, 64);
     if (VA)
-      for (int c = 0; c < 6; c++) vk[c] += __shfl_xor(vk[c], o, 64);
+      for (int c = 0; c < 6; c++) vo[c] += __shfl_xor(vo[c], o, 64);
+  }
+  if (quad != 0) return;
+  F.w = 0.0;
+  P.ft[i] = F;
+  if (EF) P.th[2 * n + i] = et;
+  if (VF)
+    for (int c = 0; c < 6; c++) P.th[(3 + c) * n + i] += vg[c];
+  if (VA)
+    for (int c = 0; c < 6; c++) P.vatom[c * n + i] += vo[c];
+}
+
+// Per step 4: the force on atom a.  Lane 0 of a quad: the ILP pair sum, a's own Tersoff bonds
+// (both from the kernels before), and the normal-derivative forces (own d n_a / d x_a term and
+// a as a normal neighbour of m).  Lanes 1-3 take one reverse Tersoff partner i each (lane q
+// takes candidates q-1, q+2, ...) and add what i's bonds left for a: fb[i][jj][slot of a] over
+// jj (a as j when jj is a's own slot, as k otherwise).  The quad is summed with a fixed-order
+// shuffle butterfly.
+template <typename R, bool EF, bool VA>
+__global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int a = gt >> 2;
+  const int quad = gt & 3;
+  if (a >= P.n) return;  // the whole quad leaves together
+  const size_t n = (size_t)P.n;
+  double4 F = {0.0, 0.0, 0.0, 0.0};
+  double et = 0.0;
+  double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff shares of i's bonds (per-atom only)
+  double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
+  if (quad == 0) {
+    const double4 f0 = P.fil[a], f1 = P.ft[a];
+    F = {f0.x + f1.x, f0.y + f1.y, f0.z + f1.z, 0.0};
+    // ---- ILP: d n_a / d x_a term (2-neighbour normals only) ----
+    const uint32_t *na = P.nnb + (size_t)a * 4;
+    if (na[0] == 2) {
+      const double4 Sa = P.S[a];
+      const R *Dri = P.dndri + (size_t)a * 9;
+      for (int c = 0; c < 3; c++) {
+        const double v = (double)Dri[0 * 3 + c] * Sa.x + (double)Dri[1 * 3 + c] * Sa.y + (double)Dri[2 * 3 + c] * Sa.z;
+        (&F.x)[c] -= v;
+      }
+    }
+    // ---- ILP: a as a normal neighbour of m (m among a's reverse normal candidates) ----
+    const int rn = P.rncnt[a];
+    const uint32_t *rnrow = P.rnc + (size_t)a * MAXSH;
+    for (int q = 0; q < rn; q++) {
+      const int m = (int)rnrow[q];
+      const uint32_t *nm = P.nnb + (size_t)m * 4;
+      const int cm = (int)nm[0];
+      for (int kk = 0; kk < cm; kk++) {
+        if (e_idx(nm[1 + kk]) != a) continue;
+        const double4 Sm = P.S[m];
+        const R *D = P.dn + (size_t)m * 27 + kk * 9;
+        double Fk[3];
+        for (int c = 0; c < 3; c++) Fk[c] = -((double)D[0 * 3 + c] * Sm.x + (double)D[1 * 3 + c] * Sm.y + (double)D[2 * 3 + c] * Sm.z);
+        F.x += Fk[0];
+        F.y += Fk[1];
+        F.z += Fk[2];
+        if (VA) {
+          // 0.5 * (vet_k(m) (x) F_k - T_m D_k)
+          const double *vt = P.vet + (size_t)m * 9 + kk * 3;
+          const double *Tm = P.T + (size_t)m * 6;
+          const double Tf[3][3] = {{Tm[0], Tm[3], Tm[4]}, {Tm[3], Tm[1], Tm[5]}, {Tm[4], Tm[5], Tm[2]}};
+          double TD[3][3];
+          for (int r0 = 0; r0 < 3; r0++)
+            for (int c = 0; c < 3; c++)
+              TD[r0][c] = Tf[r0][0] * (double)D[0 * 3 + c] + Tf[r0][1] * (double)D[1 * 3 + c] + Tf[r0][2] * (double)D[2 * 3 + c];
+          vk[0] += 0.5 * (vt[0] * Fk[0] - TD[0][0]);
+          vk[1] += 0.5 * (vt[1] * Fk[1] - TD[1][1]);
+          vk[2] += 0.5 * (vt[2] * Fk[2] - TD[2][2]);
+          vk[3] += 0.5 * (vt[0] * Fk[1] - TD[0][1]);
+          vk[4] += 0.5 * (vt[0] * Fk[2] - TD[0][2]);
+          vk[5] += 0.5 * (vt[1] * Fk[2] - TD[1][2]);
+        }
+        break;
+      }
+    }
+  } else {
+    // ---- Tersoff: what the bonds of i left for a ----
+    const int rt = P.rtcnt[a];
+    const uint32_t *rtrow = P.rtc + (size_t)a * MAXSH;
+    for (int q = quad - 1; q < rt; q += 3) {
+      const int i = (int)rtrow[q];
+      const int ni = P.tcnt[i];
+      const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
+      int slot_a = -1;
+      for (int b = 0; b < ni; b++)
+        if (e_idx(irow[b]) == a) { slot_a = b; break; }
+      if (slot_a < 0) continue;
+      const size_t base = (size_t)i * MAXSH;
+      for (int jj = 0; jj < ni; jj++) {
+        const double *fv = P.fb + ((base + jj) * MAXSH + slot_a) * 3;
+        F.x += fv[0];
+        F.y += fv[1];
+        F.z += fv[2];
+        if (VA)
+          for (int c = 0; c < 6; c++) va[c] += P.wb[((base + jj) * MAXSH + slot_a) * 6 + c];
+      }
+      if (EF) et += 0.5 * P.eb[base + slot_a];
+    }
+  }
+  // quad reduction, the same order in every lane
+#pragma unroll
+  for (int o = 1; o <= 2; o <<= 1) {
+    F.x += __shfl_xor(F.x, o, 64);
+    F.y += __shfl_xor(F.y, o, 64);
+    F.z += __shfl_xor(F.z, o, 64);
+    if (EF) et += __shfl_xor(et, o, 64);
+    if (VA)
+      for (int c = 0; c < 6; c++) {
+        va[c] += __shfl_xor(va[c], o, 64);
+        vk[c] += __shfl_xor(vk[c], o, 64);
+      }
   }
   if (quad != 0) return;
   F.w = 0.0;
   P.f[a] = F;
-  if (EF) P.th[2 * n + a] = et;
-  if (VF)
-    for (int c = 0; c < 6; c++) P.th[(3 + c) * n + a] += va[c];
+  if (EF) P.th[2 * n + a] += et;
   if (VA)
     for (int c = 0; c < 6; c++) P.vatom[c * n + a] += va[c] + vk[c];
 }
@@ -1325,11 +1418,12 @@ __global__ void k_pair_stats(DevPtrs<R> P, unsigned long long *out) {
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double4 xi = P.x[i];
-  const int cnt = P.icnt[i];
+  const int nfront = P.icnt[i], nback = P.ocnt[i];
+  const int cnt = nfront + nback, back_off = P.icap - nback - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
   unsigned long long np = 0;
   for (int k = 0; k < cnt; k++) {
-    const uint32_t e = row[k];
+    const uint32_t e = row[ilp_slot(k, nfront, back_off)];
     double gx, gy, gz;
     ghost(P.x[e_idx(e)], e, s.prd, gx, gy, gz);
     const double dx = xi.x - gx, dy = xi.y - gy, dz = xi.z - gz;

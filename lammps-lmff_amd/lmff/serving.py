@@ -48,11 +48,15 @@ class Job:
 
 
 def tbg_job_stream(njobs: int = 512, rate_hz: float = 50.0, steps: int = 2000, seed: int = 2024,
-                   min_atoms: int = 2000, max_atoms: int = 80000):
+                   min_atoms: int = 2000, max_atoms: int = 80000, sizes: str = "log-uniform"):
     """Seeded stream of twisted-bilayer jobs with Poisson arrivals.
 
-    Each job draws a commensurate twist (m, n), 1 <= n < m <= 12, and a target size in
-    [min_atoms, max_atoms]; the moire cell is replicated to reach it."""
+    Each job draws a commensurate twist (m, n), 1 <= n < m <= 12.  sizes="log-uniform": a
+    target size log-uniform in [min_atoms, max_atoms] (small cells are the common case), the
+    moire cell replicated to reach it; sizes="minimal": the smallest replication with at least
+    min_atoms atoms and every side >= 36.5 A (the size set by the twist alone)."""
+    if sizes not in ("log-uniform", "minimal"):
+        raise ValueError(f"unknown size distribution {sizes!r}")
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
@@ -61,7 +65,9 @@ def tbg_job_stream(njobs: int = 512, rate_hz: float = 50.0, steps: int = 2000, s
         while True:
             m = int(rng.integers(2, 13))
             n = int(rng.integers(1, m))
-            target = int(rng.integers(min_atoms, max_atoms + 1))
+            target = int(round(math.exp(rng.uniform(math.log(min_atoms), math.log(max_atoms)))))
+            if sizes == "minimal":
+                target = min_atoms
             try:
                 s = S.twisted_bilayer(m, n, min_atoms=min_atoms, max_atoms=max_atoms, target_atoms=target,
                                       seed=int(rng.integers(1 << 30)))

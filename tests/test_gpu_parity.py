@@ -180,6 +180,37 @@ def test_resident_step_matches_oracle_nve():
     np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
 
 
+def test_fp64_nve_drift_10k_matches_oracle():
+    """10,000 FP64 NVE steps of the 4,800-atom AB bilayer (dt 1 fs, rebuild every 10 steps)
+    against the oracle's record of the same run (tests/golden/nve_drift_c1.json).  While
+    roundoff differences are small the two runs are one trajectory; later they separate (the
+    dynamics is chaotic), so the total-energy error is compared by its statistics."""
+    import json
+
+    with open(os.path.join(GOLDEN, "nve_drift_c1.json")) as f:
+        fx = json.load(f)
+    s = S.config_c1()
+    with ctx_for(s) as ctx:
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=fx["dt_ps"])
+        ctx.evaluate(1, 2)
+        th0 = ctx.thermo(sid)
+        rec = ctx.run(fx["nsteps"], fx["thermo_every"], use_graph=True)[0]
+    e_gpu = np.array([th0["pe"] + th0["ke"]] + [r["pe"] + r["ke"] for r in rec]) / s.n
+    e_orc = (np.array(fx["pe"]) + np.array(fx["ke"])) / s.n
+    assert len(e_gpu) == len(e_orc)
+    early = np.abs(e_gpu[:5] - e_orc[:5])
+    print("first 200 steps |dE| per atom:", early.tolist())
+    assert early.max() < 1e-8
+    d_gpu, d_orc = e_gpu - e_gpu[0], e_orc - e_orc[0]
+    half = len(e_orc) // 2
+    sd = d_orc[half:].std()
+    print("late mean gpu/oracle:", d_gpu[half:].mean(), d_orc[half:].mean(), "std:", d_gpu[half:].std(), sd)
+    assert abs(d_gpu[half:].mean() - d_orc[half:].mean()) < 3 * sd + 1e-6
+    assert 0.5 < d_gpu[half:].std() / sd < 2.0
+    assert abs(np.abs(d_gpu).max() / np.abs(d_orc).max() - 1.0) < 0.5
+
+
 def test_graph_replay_equals_eager_and_is_deterministic():
     s = S.layered_stack(15, 9, ["G", "BN"], seed=41)
     outs = []

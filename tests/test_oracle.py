@@ -302,3 +302,26 @@ def test_reference_data_file_parses(oracle_lib):
     o.build_list()
     r = o.compute(1, 2)
     assert -8.1 < r["energy"] / s.n < -7.9  # C(O) graphene at the deck's 1.43876 A bond
+
+
+def test_nve_drift_fixture_is_the_current_oracle(oracle_lib):
+    """The committed 10,000-step NVE record (tests/golden/make_nve_drift.py) starts as the
+    current oracle does: the first 100 steps are recomputed and must agree."""
+    import json
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "nve_drift_c1.json")) as f:
+        fx = json.load(f)
+    s = S.config_c1()
+    assert s.n == fx["natoms"]
+    o = make_oracle(s, threads=8)
+    _, _, th = o.run_nve(s.x, s.v, s.mass_per_type, fx["dt_ps"], 100, rebuild_every=10, thermo_every=fx["thermo_every"])
+    k = len(th)
+    assert k == 3 and fx["step"][:k] == [int(v) for v in th[:, 0]]
+    np.testing.assert_allclose(th[:, 1], fx["pe"][:k], rtol=1e-11)
+    np.testing.assert_allclose(th[:, 2], fx["ke"][:k], rtol=1e-11)
+    # the record itself: bounded total-energy error, no trend after the initial transient
+    e = (np.array(fx["pe"]) + np.array(fx["ke"])) / s.n
+    assert np.abs(e - e[0]).max() < 5e-4
+    late = e[len(e) // 2:]
+    slope = np.polyfit(np.array(fx["step"][len(e) // 2:], dtype=float), late, 1)[0]
+    assert abs(slope) * 5000 < 3 * late.std() + 1e-6

@@ -1,4 +1,4 @@
-"""The job server on the GPU: a small Poisson stream of twisted-bilayer jobs of uneven size,
+"""The job server on the GPU: a small stream of twisted-bilayer jobs of uneven size and length,
 packed into one launch per step, gives every job exactly (bitwise) the result it gets alone."""
 import numpy as np
 import pytest
@@ -10,6 +10,10 @@ pytestmark = pytest.mark.gpu
 
 def test_stream_results_equal_alone_bitwise():
     jobs = serving.tbg_job_stream(5, rate_hz=200.0, steps=40, seed=11, max_atoms=6000)
+    # three jobs queued at the start (packed from the first tick), two arriving later, uneven
+    # lengths so that jobs retire while others keep running
+    for j, t, n in zip(jobs, [0.0, 0.0, 0.0, 0.003, 0.006], [40, 20, 60, 30, 40]):
+        j.arrival, j.steps = t, n
     alone = {j.job_id: serving.run_alone(j, precision=capi.LMFF_MIXED, tick=10) for j in jobs}
     srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=4e9, tick=10)
     try:
