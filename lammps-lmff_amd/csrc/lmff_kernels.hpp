@@ -674,8 +674,13 @@ __device__ __forceinline__ Tap<R> calc_tap(R r, R inv_rcut, int tap_flag) {
   return o;
 }
 
+// minimum resident waves per SIMD requested for the ILP pair kernel (caps its registers)
+#ifndef LMFF_ILP_MIN_WAVES
+#define LMFF_ILP_MIN_WAVES 1
+#endif
+
 template <typename R, bool EF, bool VF, bool VA>
-__global__ __launch_bounds__(256) void k_ilp(DevPtrs<R> P) {
+__global__ __launch_bounds__(256, LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
