@@ -138,7 +138,7 @@ struct Ctx {
   DBuf slist, scnt, ilist, icnt, ocnt;
   DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
-  DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave;
+  DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
   // atom ranges whose last-step force is restored after the next set-up force evaluation
   std::vector<std::pair<int, int>> restore_f;
   int scap = 64, icap = 0;
@@ -172,7 +172,7 @@ struct Ctx {
                    &slist, &scnt, &ilist, &icnt, &ocnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
-                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave};
+                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag};
     for (DBuf *b : all) b->tally = &bytes_in_use;
   }
   ~Ctx() {
@@ -413,6 +413,10 @@ struct Ctx {
     key.ensure(m * sizeof(uint32_t)); key2.ensure(m * sizeof(uint32_t));
     val.ensure(m * sizeof(int)); perm.ensure(m * sizeof(int));
     scnt.ensure(m * sizeof(int)); icnt.ensure(m * sizeof(int)); ocnt.ensure(m * sizeof(int));
+    if (!farflag.p) {
+      farflag.ensure(sizeof(int));
+      HIPCHK(hipMemset(farflag.p, 0, sizeof(int)));
+    }
     slist.ensure(m * scap * sizeof(uint32_t));
     nrm.ensure(m * sizeof(double4));
     dn.ensure(m * 27 * sizeof(double)); dndri.ensure(m * 9 * sizeof(double)); vet.ensure(m * 9 * sizeof(double));
@@ -486,13 +490,23 @@ struct Ctx {
     P.shortsq = short_cut * short_cut;
     P.tersoff_cutmaxsq = ters.cutmax * ters.cutmax;
     P.ilp_cutsq = ilp_cut * ilp_cut;
-    // pairs just beyond the cutoff at build time can move into range before the next rebuild;
-    // 0.1 A covers what thermal motion does over a rebuild interval (the order only affects speed)
-    P.ilp_splitsq = (ilp_cut + 0.1) * (ilp_cut + 0.1);
+    P.ilp_splitsq = (ilp_cut + ilp_margin()) * (ilp_cut + ilp_margin());
+    P.farflag = farflag.as<int>();
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
     return P;
+  }
+
+  // ILP rows keep pairs with r < rc + margin at the build in front; the rest is only walked once
+  // some atom has moved more than margin/2 (the far flag).  The margin trades the length of the
+  // front segment against how soon thermal motion raises the flag (0.3 A of displacement is
+  // beyond what a 300 K atom travels in a 10-step rebuild interval); results do not depend on it.
+  double ilp_margin() const { return std::min(0.6, skin); }
+  void far_check() {
+    const double h = 0.5 * ilp_margin();
+    hipLaunchKernelGGL(k_displacement, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, (const double4 *)x.p,
+                       (const double4 *)xhold.p, sysid.as<int>(), h * h, farflag.as<int>());
   }
 
   // ---------------------------------------------------------------- profiling helpers
@@ -541,6 +555,7 @@ struct Ctx {
     hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
                        key2.as<uint32_t>(), cell_start.as<int>());
     HIPCHK(hipMemcpyAsync(xhold.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 1, farflag.as<int>(), 0);
     build_lists();
     mark_end(0, mk);
     need_rebuild = false;
@@ -662,9 +677,10 @@ struct Ctx {
 
   // ---------------------------------------------------------------- stepping
   void initial_integrate() {
+    const double h = 0.5 * ilp_margin();
     hipLaunchKernelGGL(k_initial_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, x.as<double4>(),
                        v.as<double4>(), (const double4 *)f.p, sysid.as<int>(), systab.as<SysDev>(),
-                       rmass.as<double>());
+                       rmass.as<double>(), (const double4 *)xhold.p, h * h, farflag.as<int>());
   }
   void final_integrate(int want_ke) {
     hipLaunchKernelGGL(k_final_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n,
@@ -1280,6 +1296,7 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
         h[q].z = x[3 * i + 2];
       }
       HIPCHK(hipMemcpy(c.x.as<double4>() + s->begin, h.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+      c.far_check();
     }
     c.evaluate(eflag, vflag);
     HIPCHK(hipStreamSynchronize(c.stream));

@@ -113,6 +113,7 @@ struct DevPtrs {
   int *ovf;               // [0] max short count, [1] max ILP count
   double cutneighsq, shortsq, tersoff_cutmaxsq, ilp_cutsq, inv_rcut, rcut;
   double ilp_splitsq;     // ILP rows hold entries with r^2 < ilp_splitsq (at build time) first
+  const int *farflag;     // set once an atom has moved more than half the split margin since the build
   int tap_flag;
 };
 
@@ -692,8 +693,10 @@ __global__ __launch_bounds__(256, LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   const int ti = (int)xi.w - 1;
   const auto ni4 = P.nrm[i];
   const R nix = ni4.x, niy = ni4.y, niz = ni4.z;
-  const int nfront = P.icnt[i], nback = P.ocnt[i];
-  const int cnt = nfront + nback, back_off = P.icap - nback - nfront;
+  // the back segment (r > rc + margin at the build) can hold a pair within rc only after some
+  // atom has moved more than margin/2 since the build
+  const int nfront = P.icnt[i], nback = *P.farflag ? P.ocnt[i] : 0;
+  const int cnt = nfront + nback, back_off = P.icap - P.ocnt[i] - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
   const int itag = P.tag[i];
   const R inv_rcut = (R)P.inv_rcut;
@@ -1297,8 +1300,12 @@ __device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]) {
   }
 }
 
+// initial_integrate; also raises the far flag (sticky until the next build) when an atom is now
+// more than half the ILP split margin away from its position at the last build: only then can a
+// pair from the back segment of an ILP row be within the cutoff
 __global__ void k_initial_integrate(int n, double4 *x, double4 *v, const double4 *f, const int *sys,
-                                    const SysDev *st, const double *rmass_type) {
+                                    const SysDev *st, const double *rmass_type, const double4 *xhold,
+                                    double far_sq, int *farflag) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const SysDev &s = st[sys[p]];
@@ -1313,6 +1320,11 @@ __global__ void k_initial_integrate(int n, double4 *x, double4 *v, const double4
   xp.z += s.dtv * vp.z;
   x[p] = xp;
   v[p] = vp;
+  const double4 h = xhold[p];
+  const double dx = xp.x - h.x, dy = xp.y - h.y, dz = xp.z - h.z;
+  const unsigned long long far = __ballot(dx * dx + dy * dy + dz * dz > far_sq);
+  // one atomic per wave, none once the flag is up
+  if (far && (threadIdx.x & 63) == __ffsll((long long)far) - 1 && *(volatile int *)farflag == 0) atomicOr(farflag, 1);
 }
 
 // post_force (Langevin) + final_integrate; ke column on thermo steps

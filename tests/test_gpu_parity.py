@@ -180,6 +180,35 @@ def test_resident_step_matches_oracle_nve():
     np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
 
 
+def test_pairs_entering_from_beyond_the_split_between_rebuilds():
+    """3000 K and one list build for 60 steps: atoms move well over 0.3 A, so pairs that were
+    beyond rc + 0.6 A at the build come within rc before the next build (the ILP rows' back
+    segment must then be walked).  Positions and energies follow the oracle run with the same
+    schedule."""
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=51, temperature=3000.0)
+    o = oracle_for(s)
+    x_ref, v_ref, th = o.run_nve(s.x, s.v, s.mass_per_type, 0.001, 60, rebuild_every=60, thermo_every=20)
+    ctx = capi.Context(0)
+    try:
+        capi.setup_for(ctx, s, every=60, delay=0, check=0)
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+        ctx.evaluate(1, 2)
+        rec = ctx.run(60, 20, use_graph=True)[0]
+        x, v, f = ctx.get(sid)
+    finally:
+        ctx.close()
+    L = (s.boxhi - s.boxlo)
+    d = x - x_ref
+    d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
+    disp = s.x - x_ref
+    disp[:, :2] -= L[:2] * np.round(disp[:, :2] / L[:2])
+    assert np.sqrt((disp ** 2).sum(1)).max() > 0.5
+    assert np.abs(d).max() < 1e-8
+    pe = np.array([r["pe"] for r in rec])
+    np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
+
+
 def test_fp64_nve_drift_10k_matches_oracle():
     """10,000 FP64 NVE steps of the 4,800-atom AB bilayer (dt 1 fs, rebuild every 10 steps)
     against the oracle's record of the same run (tests/golden/nve_drift_c1.json).  While
