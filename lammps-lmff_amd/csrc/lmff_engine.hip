@@ -139,6 +139,7 @@ struct Ctx {
   DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
+  int ntp = 0;  // Tersoff parameter records
   // atom ranges whose last-step force is restored after the next set-up force evaluation
   std::vector<std::pair<int, int>> restore_f;
   int scap = 64, icap = 0;
@@ -249,6 +250,7 @@ struct Ctx {
     ip_d.ensure(sizeof(IlpPair<double>) * ipd.size());
     ip_f.ensure(sizeof(IlpPair<float>) * ipf.size());
     tp_d.ensure(sizeof(TersP<double>) * tpd.size());
+    ntp = (int)tpd.size();
     tp_f.ensure(sizeof(TersP<float>) * tpf.size());
     t3.ensure(sizeof(int) * t3v.size());
     rmass.ensure(sizeof(double) * MAXT);
@@ -483,6 +485,7 @@ struct Ctx {
     P.th = th.as<double>(); P.vatom = vatom.as<double>();
     P.ip = (const IlpPair<R> *)(sizeof(R) == 8 ? ip_d.p : ip_f.p);
     P.tp = (const TersP<double> *)tp_d.p;
+    P.ntp = ntp;
     P.t3 = t3.as<int>();
     P.err = errw.as<int>();
     P.ovf = ovf.as<int>();
@@ -602,6 +605,14 @@ struct Ctx {
   }
 
   // ---------------------------------------------------------------- force evaluation
+  template <typename R, bool LDSP>
+  void launch_bonds(const DevPtrs<R> &P, int ef, int vf, int va, int nb4) {
+    if (va) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, true, LDSP>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, false, LDSP>), dim3(nb4), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_tersoff_bonds<R, true, false, false, LDSP>), dim3(nb4), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_tersoff_bonds<R, false, false, false, LDSP>), dim3(nb4), dim3(BS), 0, stream, P);
+  }
+
   template <typename R>
   void force_impl(int ef, int vf, int va) {
     auto P = ptrs<R>();
@@ -620,10 +631,8 @@ struct Ctx {
     mark_end(2, mk);
     mark_begin(3, mk);
     const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
-    if (va) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
-    else if (vf) hipLaunchKernelGGL((k_tersoff_bonds<R, true, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
-    else if (ef) hipLaunchKernelGGL((k_tersoff_bonds<R, true, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_tersoff_bonds<R, false, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    if (ntp <= TP_LDS) launch_bonds<R, true>(P, ef, vf, va, nb4);
+    else launch_bonds<R, false>(P, ef, vf, va, nb4);
     mark_end(3, mk);
     mark_begin(4, mk);
     if (va) hipLaunchKernelGGL((k_gather<R, true, true>), dim3(nb4), dim3(BS), 0, stream, P);

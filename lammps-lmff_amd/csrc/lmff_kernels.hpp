@@ -107,7 +107,8 @@ struct DevPtrs {
   double *vatom;      // [6][n]
   // parameters
   const IlpPair<R> *ip;   // [MAXT][MAXT], types 0-based
-  const TersP<double> *tp;  // [nparams]; Tersoff math is FP64 in both precision modes
+  const TersP<double> *tp;  // [ntp]; Tersoff math is FP64 in both precision modes
+  int ntp;
   const int *t3;          // [MAXT][MAXT][MAXT] -> tp index or -1
   int *err;
   int *ovf;               // [0] max short count, [1] max ILP count
@@ -180,6 +181,14 @@ __device__ __forceinline__ double rsq_ref(double dx, double dy, double dz) {
 #pragma clang fp contract(off)
   const double a = dx * dx, b = dy * dy, c = dz * dz;
   return (a + b) + c;
+}
+
+// Copies a small read-only table into LDS with every thread of the block (call before any
+// thread leaves; a __syncthreads must follow).  sizeof(T) is a multiple of 4.
+template <typename T>
+__device__ __forceinline__ void stage_table(T *dst, const T *src, int count) {
+  const int words = count * (int)(sizeof(T) / 4);
+  for (int k = threadIdx.x; k < words; k += blockDim.x) ((int *)dst)[k] = ((const int *)src)[k];
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8).  Atoms are
@@ -686,6 +695,10 @@ __global__ __launch_bounds__(256, LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
+  // the pair-parameter table (MAXT x MAXT entries) is read per lane and per entry: from LDS
+  __shared__ IlpPair<R> s_ip[MAXT * MAXT];
+  stage_table(s_ip, P.ip, MAXT * MAXT);
+  __syncthreads();
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
@@ -719,8 +732,8 @@ __global__ __launch_bounds__(256, LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
     const double rsqd = rsq_ref(dxd, dyd, dzd);
     if (rsqd >= P.ilp_cutsq) continue;
     const int tj = (int)xj.w - 1;
-    const IlpPair<R> &pij = P.ip[ti * MAXT + tj];
-    const IlpPair<R> &pji = P.ip[tj * MAXT + ti];
+    const IlpPair<R> &pij = s_ip[ti * MAXT + tj];
+    const IlpPair<R> &pji = s_ip[tj * MAXT + ti];
     const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd, rsq = (R)rsqd;
     const R rinv = M::rsqrt(rsq);
     const R r = rsq * rinv;
@@ -1000,10 +1013,23 @@ __device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const doubl
 // force on j and on every k is stored per bond in fb[i][jj][kk] (kk == jj: the force on j),
 // where the gather kernel picks it up.  The global virial of the bond is kept at i; per-atom
 // virial shares (v_tally3 thirds, ev_tally halves) are stored in wb for the gather.
-template <typename R, bool EF, bool VF, bool VA>
+// With LDSP the type-triple index and the parameter records (at most TP_LDS of them) are
+// staged in LDS, which shortens the chains of dependent loads that bound this kernel.
+constexpr int TP_LDS = 64;
+
+template <typename R, bool EF, bool VF, bool VA, bool LDSP>
 __global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
   using RT = double;
   using M = MathOps<RT>;
+  __shared__ int s_t3[LDSP ? MAXT * MAXT * MAXT : 1];
+  __shared__ TersP<double> s_tp[LDSP ? TP_LDS : 1];
+  if (LDSP) {
+    stage_table(s_t3, P.t3, MAXT * MAXT * MAXT);
+    stage_table(s_tp, P.tp, P.ntp);
+    __syncthreads();
+  }
+  const int *t3 = LDSP ? s_t3 : P.t3;
+  const TersP<double> *tp = LDSP ? s_tp : P.tp;
   const int gt = xcd_block() * blockDim.x + threadIdx.x;
   const int i = gt >> 2;
   const int quad = gt & 3;
@@ -1034,8 +1060,8 @@ __global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
       double gx, gy, gz;
       ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
       const bool skip = tag_skip(itag, P.tag[e_idx(ej)], xi.x, xi.y, xi.z, gx, gy, gz);
-      const int pr = skip ? P.t3[(tj * MAXT + ti) * MAXT + ti] : P.t3[(ti * MAXT + tj) * MAXT + tj];
-      const TersP<double> &p = P.tp[pr];
+      const int pr = skip ? t3[(tj * MAXT + ti) * MAXT + ti] : t3[(ti * MAXT + tj) * MAXT + tj];
+      const TersP<double> &p = tp[pr];
       if (rsqj < p.cutsq) {
         const double r = sqrt(rsqj);
         const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
@@ -1055,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
       }
     }
     // ---- bond (i, j) ----
-    const TersP<RT> &p = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tj]];
+    const TersP<RT> &p = tp[t3[(ti * MAXT + tj) * MAXT + tj]];
     if (rsqj >= p.cutsq) {
       for (int kk = 0; kk < ns; kk++) {
         fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
@@ -1072,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
       double dk[3], rsqk;
       int tk;
       nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
-      const TersP<RT> &q = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tk]];
+      const TersP<RT> &q = tp[t3[(ti * MAXT + tj) * MAXT + tk]];
       if (rsqk >= q.cutsq) continue;
       const RT rik = M::sq(rsqk);
       const RT costheta = (dj[0] * dk[0] + dj[1] * dk[1] + dj[2] * dk[2]) / (rij * rik);
@@ -1115,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
       double dk[3], rsqk;
       int tk;
       nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
-      const TersP<double> &q = P.tp[P.t3[(ti * MAXT + tj) * MAXT + tk]];
+      const TersP<double> &q = tp[t3[(ti * MAXT + tj) * MAXT + tk]];
       if (rsqk >= q.cutsq) {
         fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
         if (VA)
