@@ -296,14 +296,17 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   __shared__ int run_beg[4][RUN_SLOTS];
   __shared__ int run_off[4][RUN_SLOTS + 1];
   __shared__ int run_sh[4][RUN_SLOTS];
-  const int wv = threadIdx.x >> 6;
-  const int i = xcd_block() * (blockDim.x >> 6) + wv;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + wv);
   const int lane = threadIdx.x & 63;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double4 xi = P.x[i];
   const int mi = P.mol[i];
   const int ti = (int)xi.w - 1;
+  // bit tj: the ILP acts between types ti and tj (wave-uniform)
+  unsigned ilp_mask = 0;
+  for (int t = 0; t < MAXT; t++) ilp_mask |= (P.ip[ti * MAXT + t].mapped ? 1u : 0u) << t;
   const double pos[3] = {xi.x, xi.y, xi.z};
   int c[3];
 #pragma unroll
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
         const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
         const double rsq = rsq_ref(delx, dely, delz);
         const int tj = (int)xj.w - 1;
-        const bool ilp = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && P.ip[ti * MAXT + tj].mapped;
+        const bool ilp = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && ((ilp_mask >> tj) & 1u);
         in_i = ilp && rsq < P.ilp_splitsq;
         in_o = ilp && !(rsq < P.ilp_splitsq);
         in_s = (rsq <= P.shortsq);
@@ -461,6 +464,21 @@ __device__ __forceinline__ int ilp_slot(int k, int nfront, int back_off) { retur
 template <typename R, bool WANT_VET>
 __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   using M = MathOps<R>;
+  // per type pair (ti, tj): Tersoff-mapped, ILP-mapped, cutILPsq and the widened reverse cutoff,
+  // built once per block in LDS (read per short-list entry)
+  struct NPair {
+    double rcutsq, rc2w;
+    int tmapped, imapped;
+  };
+  __shared__ NPair s_np[MAXT * MAXT];
+  const double widen = 1.0 + 1e-12;
+  if (threadIdx.x < MAXT * MAXT) {
+    const int a = threadIdx.x / MAXT, b = threadIdx.x % MAXT;
+    const IlpPair<R> &q = P.ip[threadIdx.x];
+    s_np[threadIdx.x] = {q.rcutsq, fmax(q.rcutsq, P.ip[b * MAXT + a].rcutsq) * widen,
+                         P.t3[(a * MAXT + b) * MAXT + b] >= 0 ? 1 : 0, q.mapped};
+  }
+  __syncthreads();
   const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
@@ -477,7 +495,6 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   // reverse candidates: every atom j that may list i as a Tersoff neighbour or as a normal
   // neighbour (j's own test runs from j's side, so the cutoffs are widened by 1e-12 to form a
   // superset whatever the last-bit rounding of j's r^2)
-  const double widen = 1.0 + 1e-12;
   uint32_t *rt_row = P.rtc + (size_t)i * MAXSH;
   uint32_t *rn_row = P.rnc + (size_t)i * MAXSH;
   // entries are fetched in batches of NB so that NB independent gathers are in flight; every
@@ -502,7 +519,8 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
       const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
       const double rsq = rsq_ref(delx, dely, delz);
       const int tj = (int)xj.w - 1;
-      const bool tmapped = P.t3[(ti * MAXT + tj) * MAXT + tj] >= 0;
+      const NPair &pp = s_np[ti * MAXT + tj];
+      const bool tmapped = pp.tmapped != 0;
       if (rsq < P.tersoff_cutmaxsq && tmapped) {
         if (ns < MAXSH) P.tsh[(size_t)i * MAXSH + ns] = e;
         ns++;
@@ -511,10 +529,8 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
         if (nrt < MAXSH) rt_row[nrt] = (uint32_t)j;
         nrt++;
       }
-      const IlpPair<R> &pp = P.ip[ti * MAXT + tj];
-      if (pp.mapped && P.mol[j] == mi && rsq != 0.0) {
-        const double rc2 = fmax(pp.rcutsq, P.ip[tj * MAXT + ti].rcutsq) * widen;
-        if (rsq < rc2) {
+      if (pp.imapped && P.mol[j] == mi && rsq != 0.0) {
+        if (rsq < pp.rc2w) {
           if (nrn < MAXSH) rn_row[nrn] = (uint32_t)j;
           nrn++;
         }
@@ -558,7 +574,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
     for (int d = 0; d < 3; d++) vet[k][d] = (R)vetd[k][d];
   R n1[3], normal[3], dn1[3][3][3];
   R *dno = P.dn + (size_t)i * 27;
-  if (nn <= 1 || !P.ip[ti * MAXT + ti].mapped) {
+  if (nn <= 1 || !s_np[ti * MAXT + ti].imapped) {
     P.nrm[i] = {0, 0, 1, 0};
     for (int q = 0; q < 27; q++) dno[q] = 0;
     nrow[0] = (uint32_t)(nn <= 1 ? nn : 0);
