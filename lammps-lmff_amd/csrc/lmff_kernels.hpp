@@ -461,6 +461,11 @@ __device__ __forceinline__ int ilp_slot(int k, int nfront, int back_off) { retur
 // Per step 1: Tersoff neighbours within cutmax, ILP normal neighbours and the normal with its
 // derivatives (ILP_neigh + calc_normal, pair_ilp_graphene_hbn.cpp:681-1075).  One thread/atom.
 
+// short-list entries fetched together per thread in the normals kernel
+#ifndef LMFF_NORMALS_BATCH
+#define LMFF_NORMALS_BATCH 8
+#endif
+
 template <typename R, bool WANT_VET>
 __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   using M = MathOps<R>;
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   uint32_t *rn_row = P.rnc + (size_t)i * MAXSH;
   // entries are fetched in batches of NB so that NB independent gathers are in flight; every
   // index below is a compile-time constant after unrolling (no scratch memory)
-  constexpr int NB = 8;
+  constexpr int NB = LMFF_NORMALS_BATCH;
   for (int k0 = 0; k0 < cnt; k0 += NB) {
     uint32_t eb_[NB];
     double4 xb_[NB];
@@ -700,13 +705,15 @@ __device__ __forceinline__ Tap<R> calc_tap(R r, R inv_rcut, int tap_flag) {
   return o;
 }
 
-// minimum resident waves per SIMD requested for the ILP pair kernel (caps its registers)
+// minimum resident waves per SIMD requested for the plain (force-only) ILP pair kernel: five
+// waves fit its registers without spilling and hide more of the gather latency (measured 10 %
+// faster than the four the compiler picks by itself); the tally variants keep their registers
 #ifndef LMFF_ILP_MIN_WAVES
-#define LMFF_ILP_MIN_WAVES 1
+#define LMFF_ILP_MIN_WAVES 5
 #endif
 
 template <typename R, bool EF, bool VF, bool VA>
-__global__ __launch_bounds__(256, LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
+__global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -1030,11 +1037,12 @@ __device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const doubl
 // where the gather kernel picks it up.  The global virial of the bond is kept at i; per-atom
 // virial shares (v_tally3 thirds, ev_tally halves) are stored in wb for the gather.
 // With LDSP the type-triple index and the parameter records (at most TP_LDS of them) are
-// staged in LDS, which shortens the chains of dependent loads that bound this kernel.
+// staged in LDS, which shortens the chains of dependent loads that bound this kernel.  Three
+// resident waves per SIMD (a few bytes of spill) measured 10 % faster than two.
 constexpr int TP_LDS = 64;
 
 template <typename R, bool EF, bool VF, bool VA, bool LDSP>
-__global__ __launch_bounds__(256) void k_tersoff_bonds(DevPtrs<R> P) {
+__global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   using RT = double;
   using M = MathOps<RT>;
   __shared__ int s_t3[LDSP ? MAXT * MAXT * MAXT : 1];
