@@ -102,6 +102,51 @@ int lmff_system_get(lmff_ctx *ctx, int sys_id, double *x, double *v, double *f);
 int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *x, const double *v);
 /* per-atom energy [n] and virial [n][6] of the last evaluation with those flags */
 int lmff_system_peratom(lmff_ctx *ctx, int sys_id, double *eatom, double *vatom);
+/* Step counter of a system (the Langevin noise is keyed by it); the reset_timestep command
+ * (LMFF/input.cpp:1865) so that a run restarted from write_data continues the stream. */
+int lmff_system_set_step(lmff_ctx *ctx, int sys_id, int64_t step);
+/* When enabled, the last step of every lmff_run evaluates with eflag_atom and vflag_atom set,
+ * as LAMMPS does on the steps a compute pe/atom or stress/atom is invoked
+ * (Integrate::ev_set, LMFF/integrate.cpp:131); lmff_system_peratom then returns that step's tallies
+ * and the forces stay the integrator's (pair + thermostat). */
+int lmff_peratom_on_last_step(lmff_ctx *ctx, int enable);
+
+/* ---- Energy minimisation (min_style fire, LMFF/min_fire.cpp; minimize etol ftol maxiter
+ * maxeval, LMFF/minimize.cpp) --------------------------------------------------------------
+ * Every resident system is relaxed at once, each with its own FIRE state (semi-implicit Euler
+ * integrator, half step back), starting from its integrator timestep; lists follow Min::init
+ * (every 1, delay 0, check yes).  Velocities are zeroed at the start as MinFire::setup_style
+ * does; each system's step counter advances by its iterations.  FIRE stops on etol, ftol,
+ * maxiter or too many consecutive v.f <= 0 steps (it does not count force evaluations, so the
+ * maxeval of the minimize command has no effect, as in the reference). */
+typedef struct {
+  double dmax;              /* max displacement per iteration (A), default 0.1 */
+  int delaystep;            /* default 20 */
+  double dtgrow, dtshrink;  /* 1.1, 0.5 */
+  double alpha0, alphashrink; /* 0.25, 0.99 */
+  double tmax, tmin;        /* dtmax = tmax dt, dtmin = tmin dt: 10, 0.02 */
+  int halfstepback;         /* 1 */
+  int delaystep_start;      /* 1 */
+  int max_vdotf_negatif;    /* 2000 */
+  int norm;                 /* force norm for ftol: 0 two, 1 max, 2 inf (min_modify norm) */
+} lmff_fire_params;
+
+enum { LMFF_STOP_MAXITER = 0, LMFF_STOP_ETOL = 2, LMFF_STOP_FTOL = 3, LMFF_STOP_MAXVDOTF = 11 };
+
+typedef struct {
+  int stop;                 /* LMFF_STOP_* (the reference's stop_condition codes) */
+  int64_t iterations, evaluations;
+  double e_initial, e_next_to_last, e_final;   /* pair energy (eV) */
+  double fnorm2_initial, fnorm2_final;         /* Euclidean force norm (eV/A) */
+  double fnorminf_initial, fnorminf_final;     /* largest force component */
+  double dt_final;
+} lmff_min_stats;
+
+/* LAMMPS' FIRE defaults (Min::Min, LMFF/min.cpp:54-71) */
+void lmff_fire_defaults(lmff_fire_params *p);
+/* params NULL: defaults; out[k] for the k-th resident system (in order of addition) */
+int lmff_minimize(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, const lmff_fire_params *params,
+                  lmff_min_stats *out, int out_cap);
 
 /* Thermo record of one system (metal units). */
 typedef struct {

@@ -126,6 +126,9 @@ struct Ctx {
   bool forces_valid = false;
   int ago = 0;
   long long global_step = 0;
+  // the last step of every run also tallies per-atom energy and virial (for dumps of
+  // compute pe/atom and stress/atom, which LAMMPS requests on the steps they are output)
+  bool peratom_last = false;
   // staging for systems added since the last layout (host copies, caller order)
   struct Staged {
     int id;
@@ -671,15 +674,162 @@ struct Ctx {
 
   // ---------------------------------------------------------------- thermo
   // columns: 0 e_vdw 1 e_rep 2 e_tersoff 3-8 virial 9 ke
-  void reduce(int ncols, double *dst_dev) {
+  // per-system sums of the first ncols thermo columns (the last nmax of them: maxima)
+  void reduce(int ncols, double *dst_dev, int nmax = 0) {
     if (nchunks == 0) {
       HIPCHK(hipMemsetAsync(dst_dev, 0, sizeof(double) * ncols * std::max<size_t>(sys.size(), 1), stream));
       return;
     }
     hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(BS), 0, stream, (const double *)th.p, (size_t)n, ncols,
-                       chunks.as<Chunk>(), partial.as<double2>());
+                       chunks.as<Chunk>(), partial.as<double2>(), nmax);
     hipLaunchKernelGGL(k_reduce_systems, dim3((unsigned)sys.size()), dim3(64), 0, stream, (const double2 *)partial.p,
-                       ncols, cbeg.as<int>(), (int)sys.size(), dst_dev);
+                       ncols, cbeg.as<int>(), (int)sys.size(), dst_dev, nmax);
+  }
+
+  // ---------------------------------------------------------------- FIRE minimisation
+  // MinFire::iterate (LMFF/min_fire.cpp:104-460) with the semi-implicit Euler integrator, run
+  // for every resident system at once, each with its own FIRE state; neighbor lists follow
+  // Min::init (every 1, delay 0, check yes: LMFF/min.cpp:190-196).
+  void minimize(double etol, double ftol, int64_t maxiter, const lmff_fire_params &fp, std::vector<lmff_min_stats> &st) {
+    relayout();
+    const int nsys = (int)sys.size();
+    st.assign(nsys, lmff_min_stats{});
+    if (n == 0 || nsys == 0) return;
+    const double ftm2v = 1.0 / MVV2E, eps_energy = 1.0e-8;
+    struct S {
+      double dt, dtmax, dtmin, alpha, dtv_prev = 0, ecur = 0, eprev = 0;
+      long long ntimestep, last_negative, start;
+      int negatif = 0, done = 0;
+    };
+    std::vector<S> fs(nsys);
+    for (int k = 0; k < nsys; k++) {
+      fs[k].dt = sys[k].dt;
+      fs[k].dtmax = fp.tmax * sys[k].dt;
+      fs[k].dtmin = fp.tmin * sys[k].dt;
+      fs[k].alpha = fp.alpha0;
+      fs[k].ntimestep = fs[k].last_negative = fs[k].start = sys[k].step;
+    }
+    const int sv_every = every, sv_delay = delay, sv_check = check;
+    every = 1;
+    delay = 0;
+    check = 1;
+    DBuf out, steps;
+    out.ensure(sizeof(double) * 16 * nsys);
+    steps.ensure(sizeof(FireStep) * nsys);
+    std::vector<double> h(16 * (size_t)nsys);
+    // energies (columns 0-2 of the thermo block) and the FIRE statistics of the current state
+    auto measure = [&]() {
+      force(1, 0);
+      reduce(3, out.as<double>());
+      hipLaunchKernelGGL(k_fire_stats, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, (const double4 *)v.p,
+                         (const double4 *)f.p, th.as<double>());
+      reduce(6, out.as<double>() + 3 * (size_t)nsys, 3);
+      HIPCHK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 9 * nsys, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      check_errors();
+    };
+    auto energy = [&](int k) { return h[3 * k] + h[3 * k + 1] + h[3 * k + 2]; };
+    auto stat = [&](int k, int c) { return h[3 * (size_t)nsys + 6 * k + c]; };
+    auto fnorm = [&](int k) {  // the quantity compared with ftol^2 (Min::fnorm_sqr / _max / _inf)
+      return fp.norm == 1 ? stat(k, 4) : fp.norm == 2 ? stat(k, 5) : stat(k, 2);
+    };
+    // Min setup: velocities zeroed (MinFire::setup_style), lists and forces of the start
+    hipLaunchKernelGGL(k_zero_v, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, v.as<double4>());
+    if (need_rebuild) rebuild();
+    measure();
+    for (int k = 0; k < nsys; k++) {
+      fs[k].ecur = energy(k);
+      st[k].e_initial = st[k].e_next_to_last = st[k].e_final = fs[k].ecur;
+      st[k].fnorm2_initial = std::sqrt(stat(k, 2));
+      st[k].fnorminf_initial = stat(k, 5);
+      st[k].stop = LMFF_STOP_MAXITER;
+    }
+    std::vector<FireStep> q(nsys);
+    for (int64_t iter = 0; iter < maxiter; iter++) {
+      bool any = false;
+      for (int k = 0; k < nsys; k++) {
+        S &a = fs[k];
+        FireStep &c = q[k];
+        c = FireStep{};
+        if (a.done) continue;
+        const long long ts = ++a.ntimestep;
+        st[k].iterations++;
+        const double vdotf = stat(k, 0);
+        if (vdotf > 0.0) {
+          a.negatif = 0;
+          const double vdotv = stat(k, 1), fdotf = stat(k, 2);
+          c.scale1 = 1.0 - a.alpha;
+          c.scale2 = fdotf <= 1e-20 ? 0.0 : a.alpha * std::sqrt(vdotv / fdotf);
+          c.mix = 1;
+          if (ts - a.last_negative > fp.delaystep) {
+            a.dt = std::min(a.dt * fp.dtgrow, a.dtmax);
+            a.alpha *= fp.alphashrink;
+          }
+        } else {
+          a.last_negative = ts;
+          const bool delayflag = !(ts - a.start < fp.delaystep && fp.delaystep_start);
+          if (delayflag) {
+            a.alpha = fp.alpha0;
+            if (a.dt * fp.dtshrink >= a.dtmin) a.dt *= fp.dtshrink;
+          }
+          a.negatif++;
+          if (fp.max_vdotf_negatif > 0 && a.negatif > fp.max_vdotf_negatif) {
+            a.done = 1;
+            st[k].stop = LMFF_STOP_MAXVDOTF;
+            continue;
+          }
+          c.zero = 1;
+          c.back = fp.halfstepback ? 0.5 * a.dtv_prev : 0.0;
+        }
+        // no atom moves further than dmax (velocities after the zeroing above)
+        const double vmax = c.zero ? 0.0 : stat(k, 3);
+        double dtv = a.dt;
+        if (dtv * vmax > fp.dmax) dtv = fp.dmax / vmax;
+        a.dtv_prev = dtv;
+        c.dtv = dtv;
+        c.dtf = dtv * ftm2v;
+        c.active = 1;
+        any = true;
+      }
+      if (!any) break;
+      HIPCHK(hipMemcpyAsync(steps.p, q.data(), sizeof(FireStep) * nsys, hipMemcpyHostToDevice, stream));
+      const double hm = 0.5 * ilp_margin();
+      hipLaunchKernelGGL(k_fire_step, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, x.as<double4>(),
+                         v.as<double4>(), (const double4 *)f.p, sysid.as<int>(), steps.as<FireStep>(),
+                         rmass.as<double>(), (const double4 *)xhold.p, hm * hm, farflag.as<int>());
+      ago++;
+      if (rebuild_due()) rebuild();
+      measure();
+      for (int k = 0; k < nsys; k++) {
+        S &a = fs[k];
+        if (a.done || !q[k].active) continue;
+        st[k].evaluations++;
+        a.eprev = a.ecur;
+        a.ecur = energy(k);
+        st[k].e_next_to_last = a.eprev;
+        st[k].e_final = a.ecur;
+        if (etol > 0.0 && a.ntimestep - a.last_negative > fp.delaystep &&
+            std::fabs(a.ecur - a.eprev) < etol * 0.5 * (std::fabs(a.ecur) + std::fabs(a.eprev) + eps_energy)) {
+          a.done = 1;
+          st[k].stop = LMFF_STOP_ETOL;
+        } else if (ftol > 0.0 && fnorm(k) < ftol * ftol) {
+          a.done = 1;
+          st[k].stop = LMFF_STOP_FTOL;
+        }
+      }
+    }
+    for (int k = 0; k < nsys; k++) {
+      st[k].fnorm2_final = std::sqrt(stat(k, 2));
+      st[k].fnorminf_final = stat(k, 5);
+      st[k].dt_final = fs[k].dt;
+      sys[k].step = fs[k].ntimestep;
+    }
+    every = sv_every;
+    delay = sv_delay;
+    check = sv_check;
+    upload_systab();
+    gen++;
+    forces_valid = true;
   }
 
   void zero_ke() { HIPCHK(hipMemsetAsync(th.as<double>() + 9 * (size_t)n, 0, n * sizeof(double), stream)); }
@@ -815,7 +965,8 @@ struct Ctx {
       ago++;
       if (rebuild_due()) rebuild();
       if (thermo) {
-        force(1, 2);
+        if (peratom_last && s == nsteps) force(3, 6);
+        else force(1, 2);
         mark_begin(5, mk);
         final_integrate(1);
         mark_end(5, mk);
@@ -1138,6 +1289,60 @@ int lmff_system_integrator(lmff_ctx *ctx, int sys_id, int style, double dt, doub
       c.sync_steps_from_device();
       c.upload_systab();
     }
+    c.gen++;
+  });
+}
+
+void lmff_fire_defaults(lmff_fire_params *p) {
+  if (!p) return;
+  // Min::Min (LMFF/min.cpp:54-71)
+  p->dmax = 0.1;
+  p->delaystep = 20;
+  p->dtgrow = 1.1;
+  p->dtshrink = 0.5;
+  p->alpha0 = 0.25;
+  p->alphashrink = 0.99;
+  p->tmax = 10.0;
+  p->tmin = 0.02;
+  p->halfstepback = 1;
+  p->delaystep_start = 1;
+  p->max_vdotf_negatif = 2000;
+  p->norm = 0;
+}
+
+int lmff_minimize(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, const lmff_fire_params *params,
+                  lmff_min_stats *out, int out_cap) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (etol < 0.0 || ftol < 0.0) fail(LMFF_ERR_ARG, "Illegal minimize command");
+    if (maxiter < 0) fail(LMFF_ERR_ARG, "Illegal minimize command");
+    lmff_fire_params p;
+    lmff_fire_defaults(&p);
+    if (params) p = *params;
+    if (p.tmax < p.tmin) fail(LMFF_ERR_ARG, "tmax has to be larger than tmin");
+    if (p.dtgrow < 1.0) fail(LMFF_ERR_ARG, "dtgrow has to be larger than 1.0");
+    if (p.dtshrink > 1.0) fail(LMFF_ERR_ARG, "dtshrink has to be smaller than 1.0");
+    if (p.norm < 0 || p.norm > 2) fail(LMFF_ERR_ARG, "Illegal min_modify command");
+    std::vector<lmff_min_stats> st;
+    c.minimize(etol, ftol, maxiter, p, st);
+    if (out)
+      for (size_t k = 0; k < st.size() && (int)k < out_cap; k++) out[k] = st[k];
+  });
+}
+
+int lmff_peratom_on_last_step(lmff_ctx *ctx, int enable) {
+  return guard(ctx, [&] { ctx->c.peratom_last = enable != 0; });
+}
+
+int lmff_system_set_step(lmff_ctx *ctx, int sys_id, int64_t step) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    if (step < 0) fail(LMFF_ERR_ARG, "Timestep must be >= 0");
+    if (!c.layout_dirty) c.sync_steps_from_device();
+    s->step = step;
+    if (!c.layout_dirty) c.upload_systab();
     c.gen++;
   });
 }

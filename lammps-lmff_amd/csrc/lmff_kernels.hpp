@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // rounding is ~1e-12 A); pruning with it never drops an atom the test below accepts
   const double slack = 1e-6;
   const double cut = sqrt(P.cutneighsq);
-  const double bsy = 1.0 / s.inv_bs[1], bsz = 1.0 / s.inv_bs[2], bsx = 1.0 / s.inv_bs[0];
+  const double bsy = 1.0 / s.inv_bs[1], bsz = 1.0 / s.inv_bs[2];
   const int ry = (int)ceil(cut * s.inv_bs[1]), rz = (int)ceil(cut * s.inv_bs[2]);
   const int wy = 2 * ry + 1;
   // ---- lane l: stencil row (dy, dz) -> up to three runs ----
@@ -1428,11 +1428,25 @@ __device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
   sum = t;
 }
 
+// columns c >= ncols - nmax are reduced by max (non-negative values) instead of summed
 __global__ __launch_bounds__(256) void k_reduce_chunks(const double *cols, size_t stride, int ncols,
-                                                       const Chunk *chunks, double2 *partial) {
+                                                       const Chunk *chunks, double2 *partial, int nmax) {
   __shared__ double ss[256], sc[256];
   const Chunk ch = chunks[blockIdx.x];
   for (int c = 0; c < ncols; c++) {
+    if (c >= ncols - nmax) {
+      double m = 0;
+      for (int p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) m = fmax(m, cols[c * stride + p]);
+      ss[threadIdx.x] = m;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) ss[threadIdx.x] = fmax(ss[threadIdx.x], ss[threadIdx.x + o]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ncols + c] = make_double2(ss[0], 0.0);
+      __syncthreads();
+      continue;
+    }
     double sum = 0, comp = 0;
     for (int p = ch.begin + threadIdx.x; p < ch.end; p += blockDim.x) neumaier(sum, comp, cols[c * stride + p]);
     ss[threadIdx.x] = sum;
@@ -1453,10 +1467,16 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const double *cols, size_
 }
 
 // out[s * ncols + c] = sum over the chunks of system s (chunk ranges [cbeg[s], cbeg[s+1]))
-__global__ void k_reduce_systems(const double2 *partial, int ncols, const int *cbeg, int nsys, double *out) {
+__global__ void k_reduce_systems(const double2 *partial, int ncols, const int *cbeg, int nsys, double *out, int nmax) {
   const int s = blockIdx.x;
   if (s >= nsys) return;
   for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+    if (c >= ncols - nmax) {
+      double m = 0;
+      for (int k = cbeg[s]; k < cbeg[s + 1]; k++) m = fmax(m, partial[(size_t)k * ncols + c].x);
+      out[(size_t)s * ncols + c] = m;
+      continue;
+    }
     double sum = 0, comp = 0;
     for (int k = cbeg[s]; k < cbeg[s + 1]; k++) {
       const double2 v = partial[(size_t)k * ncols + c];
@@ -1465,6 +1485,71 @@ __global__ void k_reduce_systems(const double2 *partial, int ncols, const int *c
     }
     out[(size_t)s * ncols + c] = sum + comp;
   }
+}
+
+// ---------------------------------------------------------------------------------------
+// FIRE minimisation (MinFire::iterate, LMFF/min_fire.cpp:104-460, semi-implicit Euler
+// integrator with the half step back).  The scalar decisions are made per system on the host
+// from the statistics below; the per-atom update is one kernel.
+
+struct FireStep {
+  double dtv, dtf;         // position and velocity step (dtf = dtv * ftm2v)
+  double scale1, scale2;   // velocity mixing v = scale1 v + scale2 f (when mix)
+  double back;             // x -= back * v before v is zeroed (half step back)
+  int mix, zero, active, pad;
+};
+
+__global__ void k_zero_v(int n, double4 *v) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) v[p] = {0.0, 0.0, 0.0, 0.0};
+}
+
+// per atom: v.f, v.v, f.f (summed per system) and max|v_c|, |f|^2, max|f_c| (maxed per system)
+__global__ void k_fire_stats(int n, const double4 *v, const double4 *f, double *cols) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double4 a = v[p], b = f[p];
+  const double ff = b.x * b.x + b.y * b.y + b.z * b.z;
+  cols[0 * (size_t)n + p] = a.x * b.x + a.y * b.y + a.z * b.z;
+  cols[1 * (size_t)n + p] = a.x * a.x + a.y * a.y + a.z * a.z;
+  cols[2 * (size_t)n + p] = ff;
+  cols[3 * (size_t)n + p] = fmax(fmax(fabs(a.x), fabs(a.y)), fabs(a.z));
+  cols[4 * (size_t)n + p] = ff;
+  cols[5 * (size_t)n + p] = fmax(fmax(fabs(b.x), fabs(b.y)), fabs(b.z));
+}
+
+__global__ void k_fire_step(int n, double4 *x, double4 *v, const double4 *f, const int *sys, const FireStep *fs,
+                            const double *rmass_type, const double4 *xhold, double far_sq, int *farflag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const FireStep q = fs[sys[p]];
+  double4 xp = x[p], vp = v[p];
+  if (q.active) {
+    const double4 fp = f[p];
+    if (q.zero) {
+      xp.x -= q.back * vp.x;
+      xp.y -= q.back * vp.y;
+      xp.z -= q.back * vp.z;
+      vp.x = vp.y = vp.z = 0.0;
+    }
+    const double dtfm = q.dtf / rmass_type[(int)xp.w - 1];
+    vp.x += dtfm * fp.x;
+    vp.y += dtfm * fp.y;
+    vp.z += dtfm * fp.z;
+    if (q.mix) {
+      vp.x = q.scale1 * vp.x + q.scale2 * fp.x;
+      vp.y = q.scale1 * vp.y + q.scale2 * fp.y;
+      vp.z = q.scale1 * vp.z + q.scale2 * fp.z;
+    }
+    xp.x += q.dtv * vp.x;
+    xp.y += q.dtv * vp.y;
+    xp.z += q.dtv * vp.z;
+    x[p] = xp;
+    v[p] = vp;
+  }
+  const double4 h = xhold[p];
+  const double dx = xp.x - h.x, dy = xp.y - h.y, dz = xp.z - h.z;
+  if (dx * dx + dy * dy + dz * dz > far_sq && *(volatile int *)farflag == 0) atomicOr(farflag, 1);
 }
 
 // max squared displacement since the last build, per system (neigh_modify check yes)

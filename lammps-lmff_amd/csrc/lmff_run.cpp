@@ -5,9 +5,13 @@
 //   fix nve | fix langevin / thermo / run
 // executes with the force field, the neighbor build and the integrator on the GPU.
 // Output follows LAMMPS' log format (thermo table, "Loop time", "Performance:" as in
-// LMFF/finish.cpp:124,162).  Extra command: "lmff_precision fp64|mixed".
+// LMFF/finish.cpp:124,162).  Also: compute pe/atom and stress/atom, dump custom (text,
+// DumpCustom::header_item, LMFF/dump_custom.cpp:422-440; atoms sorted by id), write_data
+// (WriteData, LMFF/write_data.cpp:236-420, atom_style full with velocities) and
+// reset_timestep.  Extra command: "lmff_precision fp64|mixed".
 //
 // usage: lmff_run -in deck [-var NAME VALUE ...]
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <memory>
@@ -29,6 +33,15 @@
 namespace {
 
 constexpr double BOLTZ = 8.617343e-5, MVV2E = 1.0364269e-4, NKTV2P = 1.6021765e6;
+
+struct Dump {
+  std::string id, file;
+  int64_t every = 1;
+  std::vector<std::string> fields;
+  std::string fmt_float = "%g";
+  FILE *fp = nullptr;
+  int64_t last = -1;  // a step is written once (a run's set-up repeats the previous run's end)
+};
 
 struct Data {
   int64_t n = 0;
@@ -103,6 +116,92 @@ Data read_data(const std::string &path) {
   return d;
 }
 
+// write_data: header (WriteData::header), Masses, Atoms # full (AtomVecFull::write_data; image
+// flags 0: positions are kept wrapped), Velocities (AtomVec::write_vel)
+void write_data(const std::string &path, const Data &d, int64_t step, const std::vector<double> &x,
+                const std::vector<double> &v) {
+  FILE *fp = std::fopen(path.c_str(), "w");
+  if (!fp) die("Cannot open data file " + path);
+  std::fprintf(fp, "LAMMPS data file via write_data, lmff_run, timestep = %lld\n\n", (long long)step);
+  std::fprintf(fp, "%lld atoms\n%d atom types\n\n", (long long)d.n, d.ntypes);
+  const char *ax[3] = {"x", "y", "z"};
+  for (int k = 0; k < 3; k++) std::fprintf(fp, "%-1.16e %-1.16e %slo %shi\n", d.lo[k], d.hi[k], ax[k], ax[k]);
+  std::fprintf(fp, "\nMasses\n\n");
+  for (int t = 1; t <= d.ntypes; t++) std::fprintf(fp, "%d %g\n", t, d.mass.count(t) ? d.mass.at(t) : 0.0);
+  std::vector<int64_t> order(d.n);
+  for (int64_t i = 0; i < d.n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return d.tag[a] < d.tag[b]; });
+  std::fprintf(fp, "\nAtoms # full\n\n");
+  for (int64_t i : order)
+    std::fprintf(fp, "%lld %lld %d %-1.16e %-1.16e %-1.16e %-1.16e 0 0 0\n", (long long)d.tag[i], (long long)d.mol[i],
+                 d.type[i], 0.0, x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+  std::fprintf(fp, "\nVelocities\n\n");
+  for (int64_t i : order)
+    std::fprintf(fp, "%lld %-1.16e %-1.16e %-1.16e\n", (long long)d.tag[i], v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  std::fclose(fp);
+}
+
+// one snapshot of dump custom: columns id mol type x y z vx vy vz fx fy fz and c_ID / c_ID[k] /
+// c_ID[*] of compute pe/atom (eV) and stress/atom (compute_stress_atom.cpp:254-340: minus
+// (m v_a v_b + per-atom virial) in pressure*volume units, order xx yy zz xy xz yz)
+void write_dump(Dump &dp, const Data &d, int64_t step, const std::string &boundstr, const std::vector<double> &x,
+                const std::vector<double> &v, const std::vector<double> &f, const std::vector<double> &eatom,
+                const std::vector<double> &vatom, const std::map<std::string, std::string> &computes) {
+  if (!dp.fp) {
+    dp.fp = std::fopen(dp.file.c_str(), "w");
+    if (!dp.fp) die("Cannot open dump file " + dp.file);
+  }
+  std::vector<int64_t> order(d.n);
+  for (int64_t i = 0; i < d.n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return d.tag[a] < d.tag[b]; });
+  // expand c_ID[*] into its six columns
+  std::vector<std::string> cols;
+  for (const auto &c : dp.fields) {
+    if (c.size() > 5 && c.compare(c.size() - 3, 3, "[*]") == 0) {
+      for (int k = 1; k <= 6; k++) cols.push_back(c.substr(0, c.size() - 3) + "[" + std::to_string(k) + "]");
+    } else {
+      cols.push_back(c);
+    }
+  }
+  FILE *fp = dp.fp;
+  std::fprintf(fp, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ATOMS\n%lld\n", (long long)step, (long long)d.n);
+  std::fprintf(fp, "ITEM: BOX BOUNDS %s\n", boundstr.c_str());
+  for (int k = 0; k < 3; k++) std::fprintf(fp, "%-1.16e %-1.16e\n", d.lo[k], d.hi[k]);
+  std::fprintf(fp, "ITEM: ATOMS");
+  for (const auto &c : cols) std::fprintf(fp, " %s", c.c_str());
+  std::fprintf(fp, "\n");
+  for (int64_t i : order) {
+    bool first = true;
+    for (const auto &c : cols) {
+      if (!first) std::fputc(' ', fp);
+      first = false;
+      if (c == "id") { std::fprintf(fp, "%lld", (long long)d.tag[i]); continue; }
+      if (c == "mol") { std::fprintf(fp, "%lld", (long long)d.mol[i]); continue; }
+      if (c == "type") { std::fprintf(fp, "%d", d.type[i]); continue; }
+      double val = 0.0;
+      if (c == "x" || c == "y" || c == "z") val = x[3 * i + (c[0] - 'x')];
+      else if (c == "vx" || c == "vy" || c == "vz") val = v[3 * i + (c[1] - 'x')];
+      else if (c == "fx" || c == "fy" || c == "fz") val = f[3 * i + (c[1] - 'x')];
+      else {
+        const size_t br = c.find('[');
+        const std::string id = c.substr(2, br == std::string::npos ? std::string::npos : br - 2);
+        const std::string &style = computes.at(id);
+        if (style == "pe/atom") {
+          val = eatom[i];
+        } else {
+          const int k = br == std::string::npos ? 1 : std::stoi(c.substr(br + 1));
+          static const int a[6] = {0, 1, 2, 0, 0, 1}, b[6] = {0, 1, 2, 1, 2, 2};
+          const double m = d.mass.at(d.type[i]);
+          val = -NKTV2P * (MVV2E * m * v[3 * i + a[k - 1]] * v[3 * i + b[k - 1]] + vatom[6 * i + (k - 1)]);
+        }
+      }
+      std::fprintf(fp, dp.fmt_float.c_str(), val);
+    }
+    std::fputc('\n', fp);
+  }
+  std::fflush(fp);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -131,6 +230,12 @@ int main(int argc, char **argv) {
   std::unique_ptr<lmff::PairLMFFHip> pair;
   int sys = -1;
   int64_t step = 0;
+  std::map<std::string, std::string> computes;  // id -> pe/atom | stress/atom
+  std::vector<Dump> dumps;
+  std::string bstr[3] = {"p", "p", "p"};
+  std::string min_style = "cg";
+  lmff_fire_params fire;
+  lmff_fire_defaults(&fire);
 
   auto subst = [&](std::string s) {
     for (size_t p; (p = s.find("${")) != std::string::npos;) {
@@ -165,6 +270,7 @@ int main(int argc, char **argv) {
     if (lmff_system_add(ctx, data.n, data.lo, data.hi, periodic, data.type.data(), data.tag.data(), data.mol.data(),
                         data.x.data(), data.v.data(), &sys))
       die(lmff_last_error(ctx));
+    if (step > 0 && lmff_system_set_step(ctx, sys, step)) die(lmff_last_error(ctx));
   };
 
   std::string line;
@@ -182,7 +288,10 @@ int main(int argc, char **argv) {
     } else if (cmd == "dimension") {
       if (w.size() < 2 || w[1] != "3") die("only dimension 3 is supported");
     } else if (cmd == "boundary") {
-      for (int k = 0; k < 3 && k + 1 < (int)w.size(); k++) periodic[k] = w[k + 1] == "p" ? 1 : 0;
+      for (int k = 0; k < 3 && k + 1 < (int)w.size(); k++) {
+        periodic[k] = w[k + 1] == "p" ? 1 : 0;
+        bstr[k] = w[k + 1];
+      }
     } else if (cmd == "variable") {
       if (w.size() >= 4 && !vars.count(w[1])) vars[w[1]] = w[3];
     } else if (cmd == "read_data") {
@@ -254,7 +363,30 @@ int main(int argc, char **argv) {
       lmff_ctx *ctx = pair->context();
       if (!have_fix) std::printf("WARNING: No fixes defined, atoms won't move\n");
       if (lmff_system_integrator(ctx, sys, style, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
-      if (lmff_evaluate(ctx, 1, 2)) die(lmff_last_error(ctx));
+      bool want_peratom = false;
+      for (const auto &dp : dumps)
+        for (const auto &c : dp.fields)
+          if (c.rfind("c_", 0) == 0) want_peratom = true;
+      std::string boundstr;
+      for (int k = 0; k < 3; k++) boundstr += (k ? " " : "") + (bstr[k].size() == 1 ? bstr[k] + bstr[k] : bstr[k]);
+      std::vector<double> hx(3 * data.n), hv(3 * data.n), hf(3 * data.n), he, hw;
+      auto snapshot = [&](int64_t at) {
+        bool any = false;
+        for (const auto &dp : dumps) any = any || (at % dp.every == 0 && dp.last != at);
+        if (!any) return;
+        if (lmff_system_get(ctx, sys, hx.data(), hv.data(), hf.data())) die(lmff_last_error(ctx));
+        if (want_peratom) {
+          he.assign(data.n, 0.0);
+          hw.assign(6 * data.n, 0.0);
+          if (lmff_system_peratom(ctx, sys, he.data(), hw.data())) die(lmff_last_error(ctx));
+        }
+        for (auto &dp : dumps)
+          if (at % dp.every == 0 && dp.last != at) {
+            write_dump(dp, data, at, boundstr, hx, hv, hf, he, hw, computes);
+            dp.last = at;
+          }
+      };
+      if (lmff_evaluate(ctx, want_peratom ? 3 : 1, want_peratom ? 6 : 2)) die(lmff_last_error(ctx));
       lmff_thermo t0;
       lmff_system_thermo(ctx, sys, &t0);
       double vol = 1;
@@ -264,30 +396,132 @@ int main(int argc, char **argv) {
       };
       std::printf("Step Temp E_pair TotEng Press\n");
       std::printf("%8lld %12.6g %16.10g %16.10g %12.6g\n", (long long)step, t0.temp, t0.pe, t0.pe + t0.ke, press(t0));
+      snapshot(step);
+      if (lmff_peratom_on_last_step(ctx, want_peratom ? 1 : 0)) die(lmff_last_error(ctx));
+      // the run is cut at thermo and dump steps; each piece is one lmff_run (device-resident)
       const int64_t te = thermo_every > 0 ? thermo_every : nsteps;
-      const int64_t nrec = nsteps / te + (nsteps % te ? 1 : 0);
-      std::vector<lmff_thermo> rec((size_t)std::max<int64_t>(nrec, 1));
-      auto c0 = std::chrono::steady_clock::now();
-      if (lmff_run(ctx, nsteps, te, 1, rec.data(), nrec)) die(lmff_last_error(ctx));
-      double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
-      for (int64_t r = 0; r < nrec; r++) {
-        const auto &t = rec[r];
-        std::printf("%8lld %12.6g %16.10g %16.10g %12.6g\n", (long long)(step + (r + 1) * te > step + nsteps ? step + nsteps : step + (r + 1) * te),
-                    t.temp, t.pe, t.pe + t.ke, press(t));
+      const int64_t end = step + nsteps;
+      double secs = 0;
+      int64_t at = step;
+      while (at < end) {
+        int64_t next = std::min(end, (at / te + 1) * te);
+        for (const auto &dp : dumps) next = std::min(next, (at / dp.every + 1) * dp.every);
+        const int64_t len = next - at;
+        lmff_thermo t;
+        auto c0 = std::chrono::steady_clock::now();
+        if (lmff_run(ctx, len, len, 1, &t, 1)) die(lmff_last_error(ctx));
+        secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+        at = next;
+        if (at % te == 0 || at == end)
+          std::printf("%8lld %12.6g %16.10g %16.10g %12.6g\n", (long long)at, t.temp, t.pe, t.pe + t.ke, press(t));
+        snapshot(at);
       }
-      step += nsteps;
+      if (lmff_peratom_on_last_step(ctx, 0)) die(lmff_last_error(ctx));
+      step = end;
       std::printf("Loop time of %g on 1 procs for %lld steps with %lld atoms\n\n", secs, (long long)nsteps,
                   (long long)data.n);
       const double tps = secs > 0 ? nsteps / secs : 0;
       std::printf("Performance: %.3f ns/day, %.3f hours/ns, %.3f timesteps/s\n", tps * dt * 86400.0 / 1000.0,
                   tps > 0 ? 1.0 / (tps * dt * 3.6) : 0.0, tps);
       std::printf("Performance: %.4g atom-steps/s\n", tps * data.n);
-    } else if (cmd == "group" || cmd == "compute" || cmd == "thermo_style" || cmd == "dump" || cmd == "dump_modify" ||
-               cmd == "replicate" || cmd == "reset_timestep" || cmd == "minimize" || cmd == "unfix") {
+    } else if (cmd == "compute") {
+      if (w.size() >= 4 && (w[3] == "pe/atom" || w[3] == "stress/atom")) computes[w[1]] = w[3];
+      else std::printf("WARNING: compute %s ignored by lmff_run\n", w.size() > 3 ? w[3].c_str() : "?");
+    } else if (cmd == "dump") {
+      if (w.size() < 6 || w[3] != "custom") die("lmff_run supports dump ID group custom N file fields...");
+      Dump dp;
+      dp.id = w[1];
+      dp.every = std::stoll(w[4]);
+      if (dp.every <= 0) die("Illegal dump command");
+      dp.file = w[5];
+      for (size_t k = 6; k < w.size(); k++) {
+        const std::string &c = w[k];
+        static const char *plain[] = {"id", "mol", "type", "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"};
+        bool ok = std::find_if(std::begin(plain), std::end(plain), [&](const char *p) { return c == p; }) != std::end(plain);
+        if (!ok && c.rfind("c_", 0) == 0) {
+          const size_t br = c.find('[');
+          const std::string id = c.substr(2, br == std::string::npos ? std::string::npos : br - 2);
+          if (!computes.count(id)) die("Could not find dump custom compute ID " + id);
+          ok = computes[id] == "pe/atom" ? br == std::string::npos : true;
+        }
+        if (!ok) die("Invalid attribute " + c + " in dump custom command");
+        dp.fields.push_back(c);
+      }
+      dumps.push_back(dp);
+    } else if (cmd == "dump_modify") {
+      for (auto &dp : dumps)
+        if (dp.id == w.at(1))
+          for (size_t k = 2; k + 1 < w.size(); k++)
+            if (w[k] == "format" && k + 2 < w.size() && w[k + 1] == "float") dp.fmt_float = w[k + 2];
+      // atoms are always written sorted by id (dump_modify sort id)
+    } else if (cmd == "undump") {
+      for (auto it = dumps.begin(); it != dumps.end(); ++it)
+        if (it->id == w.at(1)) {
+          if (it->fp) std::fclose(it->fp);
+          dumps.erase(it);
+          break;
+        }
+    } else if (cmd == "reset_timestep") {
+      step = std::stoll(w.at(1));
+      if (step < 0) die("Timestep must be >= 0");
+      if (pair && lmff_system_set_step(pair->context(), sys, step)) die(lmff_last_error(pair->context()));
+    } else if (cmd == "write_data") {
+      if (!have_data) die("Write_data command before simulation box is defined");
+      std::vector<double> x = data.x, v = data.v;
+      if (pair && lmff_system_get(pair->context(), sys, x.data(), v.data(), nullptr)) die(lmff_last_error(pair->context()));
+      write_data(w.at(1), data, step, x, v);
+    } else if (cmd == "min_style") {
+      min_style = w.at(1);
+    } else if (cmd == "min_modify") {
+      for (size_t k = 1; k + 1 < w.size(); k += 2) {
+        const std::string &key = w[k], &val = w[k + 1];
+        if (key == "dmax") fire.dmax = std::stod(val);
+        else if (key == "delaystep") fire.delaystep = std::stoi(val);
+        else if (key == "dtgrow") fire.dtgrow = std::stod(val);
+        else if (key == "dtshrink") fire.dtshrink = std::stod(val);
+        else if (key == "alpha0") fire.alpha0 = std::stod(val);
+        else if (key == "alphashrink") fire.alphashrink = std::stod(val);
+        else if (key == "tmax") fire.tmax = std::stod(val);
+        else if (key == "tmin") fire.tmin = std::stod(val);
+        else if (key == "halfstepback") fire.halfstepback = val == "yes";
+        else if (key == "initialdelay") fire.delaystep_start = val == "yes";
+        else if (key == "vdfmax") fire.max_vdotf_negatif = std::stoi(val);
+        else if (key == "norm") fire.norm = val == "max" ? 1 : val == "inf" ? 2 : 0;
+        else std::printf("WARNING: min_modify %s ignored by lmff_run\n", key.c_str());
+      }
+    } else if (cmd == "minimize") {
+      // minimize etol ftol maxiter maxeval (LMFF/minimize.cpp), FIRE on the device
+      if (w.size() < 5) die("Illegal minimize command");
+      if (min_style != "fire")
+        std::printf("WARNING: min_style %s is not available in lmff_run; minimizing with fire\n", min_style.c_str());
+      setup_pair();
+      lmff_ctx *ctx = pair->context();
+      if (lmff_system_integrator(ctx, sys, style, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
+      lmff_min_stats ms;
+      auto c0 = std::chrono::steady_clock::now();
+      if (lmff_minimize(ctx, std::stod(w[1]), std::stod(w[2]), std::stoll(w[3]), &fire, &ms, 1))
+        die(lmff_last_error(ctx));
+      const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+      static const char *stops[] = {"max iterations", "max force evaluations", "energy tolerance", "force tolerance",
+                                    "search direction is not downhill", "linesearch alpha is zero", "forces are zero",
+                                    "quadratic factors are zero", "trust region too small", "HFTN minimizer error",
+                                    "walltime limit reached", "max iterations with v.f negative"};
+      step += ms.iterations;
+      std::printf("Loop time of %g on 1 procs for %lld steps with %lld atoms\n\n", secs, (long long)ms.iterations,
+                  (long long)data.n);
+      std::printf("Minimization stats:\n  Stopping criterion = %s\n", stops[ms.stop]);
+      std::printf("  Energy initial, next-to-last, final = \n    %18.15g %18.15g %18.15g\n", ms.e_initial,
+                  ms.e_next_to_last, ms.e_final);
+      std::printf("  Force two-norm initial, final = %g %g\n", ms.fnorm2_initial, ms.fnorm2_final);
+      std::printf("  Force max component initial, final = %g %g\n", ms.fnorminf_initial, ms.fnorminf_final);
+      std::printf("  Iterations, force evaluations = %lld %lld\n\n", (long long)ms.iterations, (long long)ms.evaluations);
+    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "replicate" || cmd == "unfix") {
       std::printf("WARNING: %s ignored by lmff_run\n", cmd.c_str());
     } else {
       die("Unknown command: " + line);
     }
   }
+  for (auto &dp : dumps)
+    if (dp.fp) std::fclose(dp.fp);
   return 0;
 }

@@ -29,8 +29,31 @@ SYMBOLS = [
     "lmff_system_remove", "lmff_system_integrator", "lmff_system_natoms", "lmff_system_get", "lmff_system_set",
     "lmff_system_peratom", "lmff_evaluate", "lmff_system_thermo", "lmff_run", "lmff_profile",
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
-    "lmff_pair_stats",
+    "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
+    "lmff_minimize",
 ]
+
+STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
+        11: "max iterations with v.f negative"}
+
+
+class FireParams(C.Structure):
+    _fields_ = [("dmax", C.c_double), ("delaystep", C.c_int), ("dtgrow", C.c_double), ("dtshrink", C.c_double),
+                ("alpha0", C.c_double), ("alphashrink", C.c_double), ("tmax", C.c_double), ("tmin", C.c_double),
+                ("halfstepback", C.c_int), ("delaystep_start", C.c_int), ("max_vdotf_negatif", C.c_int),
+                ("norm", C.c_int)]
+
+
+class MinStats(C.Structure):
+    _fields_ = [("stop", C.c_int), ("iterations", C.c_int64), ("evaluations", C.c_int64),
+                ("e_initial", C.c_double), ("e_next_to_last", C.c_double), ("e_final", C.c_double),
+                ("fnorm2_initial", C.c_double), ("fnorm2_final", C.c_double),
+                ("fnorminf_initial", C.c_double), ("fnorminf_final", C.c_double), ("dt_final", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["stop_text"] = STOP.get(self.stop, str(self.stop))
+        return d
 
 
 class LmffError(RuntimeError):
@@ -97,6 +120,12 @@ def load(path: str = LIB_PATH):
     L.lmff_debug_ilp_list.argtypes = [vp, C.c_int, C.c_int64, _i32, C.c_int, _int]
     L.lmff_debug_normals.argtypes = [vp, C.c_int, _d, _i32]
     L.lmff_pair_stats.argtypes = [vp, _i64]
+    L.lmff_system_set_step.argtypes = [vp, C.c_int, C.c_int64]
+    L.lmff_peratom_on_last_step.argtypes = [vp, C.c_int]
+    L.lmff_fire_defaults.argtypes = [C.POINTER(FireParams)]
+    L.lmff_fire_defaults.restype = None
+    L.lmff_minimize.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.POINTER(FireParams), C.POINTER(MinStats),
+                                C.c_int]
     _lib = L
     return L
 
@@ -239,6 +268,27 @@ class Context:
         w = np.zeros((n, 6)) if virial else None
         self._chk(self.L.lmff_system_peratom(self.h, int(sid), _p(e, C.c_double), _p(w, C.c_double)))
         return e, w
+
+    def set_step(self, sid: int, step: int):
+        """reset_timestep for one system (the Langevin noise is keyed by the step)."""
+        self._chk(self.L.lmff_system_set_step(self.h, int(sid), int(step)))
+
+    def peratom_on_last_step(self, enable: bool = True):
+        """The last step of every run also tallies per-atom energy and virial."""
+        self._chk(self.L.lmff_peratom_on_last_step(self.h, 1 if enable else 0))
+
+    def minimize(self, etol: float, ftol: float, maxiter: int, nsys: int = 1, **fire):
+        """min_style fire + minimize etol ftol maxiter (every resident system); keyword
+        arguments override LAMMPS' FIRE defaults (dmax, delaystep, ..., norm 0/1/2)."""
+        p = FireParams()
+        self.L.lmff_fire_defaults(C.byref(p))
+        for k, v in fire.items():
+            if not hasattr(p, k):
+                raise ValueError(f"unknown FIRE parameter {k}")
+            setattr(p, k, v)
+        out = (MinStats * max(nsys, 1))()
+        self._chk(self.L.lmff_minimize(self.h, float(etol), float(ftol), int(maxiter), C.byref(p), out, int(nsys)))
+        return [out[k].as_dict() for k in range(nsys)]
 
     def evaluate(self, eflag: int = 1, vflag: int = 2):
         self._chk(self.L.lmff_evaluate(self.h, int(eflag), int(vflag)))

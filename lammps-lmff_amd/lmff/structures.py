@@ -316,3 +316,49 @@ def read_lammps_data(path: str) -> System:
     if vel:
         s.v = np.array([vel[int(t)] for t in s.tag])
     return s
+
+
+def write_lammps_data(s: System, path: str, step: int = 0) -> None:
+    """Writes ``s`` as an ``atom_style full`` data file in the layout of LAMMPS' ``write_data``
+    (LMFF/write_data.cpp:236-420, AtomVecFull::write_data): header with the timestep, counts,
+    box, ``Masses``, ``Atoms # full`` rows ``id mol type q x y z ix iy iz`` (q and image flags
+    0) and ``Velocities``, numbers in ``%-1.16e`` so that a read gives the same doubles back."""
+    order = np.argsort(s.tag, kind="stable")
+    with open(path, "w") as fh:
+        fh.write(f"LAMMPS data file via write_data, lmff, timestep = {int(step)}\n\n")
+        fh.write(f"{s.n} atoms\n{len(s.elements)} atom types\n\n")
+        for d, ax in enumerate("xyz"):
+            fh.write(f"{s.boxlo[d]:<1.16e} {s.boxhi[d]:<1.16e} {ax}lo {ax}hi\n")
+        fh.write("\nMasses\n\n")
+        for t, m in enumerate(s.mass_per_type, start=1):
+            fh.write(f"{t} {m:g}\n")
+        fh.write("\nAtoms # full\n\n")
+        for i in order:
+            x = s.x[i]
+            fh.write(f"{int(s.tag[i])} {int(s.mol[i])} {int(s.type[i])} {0.0:<1.16e} "
+                     f"{x[0]:<1.16e} {x[1]:<1.16e} {x[2]:<1.16e} 0 0 0\n")
+        if s.v is not None:
+            fh.write("\nVelocities\n\n")
+            for i in order:
+                v = s.v[i]
+                fh.write(f"{int(s.tag[i])} {v[0]:<1.16e} {v[1]:<1.16e} {v[2]:<1.16e}\n")
+
+
+def read_dump_custom(path: str):
+    """Reads a text ``dump custom`` file (DumpCustom::header_item, LMFF/dump_custom.cpp:422-440):
+    returns a list of (timestep, column names, float array [natoms][ncolumns])."""
+    out = []
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    k = 0
+    while k < len(lines):
+        if lines[k].startswith("ITEM: TIMESTEP"):
+            step = int(lines[k + 1])
+            n = int(lines[k + 3])
+            cols = lines[k + 8].split()[2:]
+            rows = np.array([[float(w) for w in lines[k + 9 + r].split()] for r in range(n)]).reshape(n, len(cols))
+            out.append((step, cols, rows))
+            k += 9 + n
+        else:
+            k += 1
+    return out

@@ -195,3 +195,102 @@ def tap(r: float, rcut: float) -> float:
 
 def dtap(r: float, rcut: float) -> float:
     return lib().orc_dtap(r, rcut)
+
+
+FIRE_DEFAULTS = dict(dmax=0.1, delaystep=20, dtgrow=1.1, dtshrink=0.5, alpha0=0.25, alphashrink=0.99, tmax=10.0,
+                     tmin=0.02, halfstepback=1, delaystep_start=1, max_vdotf_negatif=2000, norm=0)
+
+
+def fire_minimize(o: Oracle, x0, mass_per_atom, dt: float, etol: float, ftol: float, maxiter: int,
+                  skin: float = 2.0, **params):
+    """FIRE on the oracle's forces: MinFire::iterate (LMFF/min_fire.cpp:104-460) with the
+    default semi-implicit Euler integrator and half step back, velocities zeroed at the start
+    (MinFire::setup_style, :83-88), lists rebuilt when an atom has moved more than skin/2
+    (Min::init sets every 1, delay 0, check yes: LMFF/min.cpp:190-196; Neighbor::check_distance).
+    Energy and force tolerances as in :420-455 (EPS_ENERGY 1e-8).  Returns the stop code
+    (0 max iterations, 2 energy tolerance, 3 force tolerance, 11 v.f negative), counts,
+    energies and the final positions."""
+    p = dict(FIRE_DEFAULTS)
+    p.update(params)
+    ftm2v = 1.0 / 1.0364269e-4
+    x = np.array(x0, dtype=np.float64)
+    m = np.asarray(mass_per_atom, dtype=np.float64)[:, None]
+    v = np.zeros_like(x)
+    o.put_x(x)
+    o.pbc()
+    o.build_list()
+    x = o.get_x()
+    xhold = x.copy()
+    r = o.compute(1, 0)
+    f, ecur = r["f"], r["energy"]
+    e_init = ecur
+    dtcur, dtmax, dtmin, alpha = dt, p["tmax"] * dt, p["tmin"] * dt, p["alpha0"]
+    ts = last_negative = start = 0
+    negatif = 0
+    dtv = 0.0
+    stop, niter, neval = 0, 0, 0
+    eprev = ecur
+
+    def fnorm(ff):
+        if p["norm"] == 1:
+            return (ff * ff).sum(1).max()
+        if p["norm"] == 2:
+            return np.abs(ff).max()
+        return (ff * ff).sum()
+
+    for _ in range(maxiter):
+        ts += 1
+        niter += 1
+        vdotf = float((v * f).sum())
+        mix = False
+        if vdotf > 0.0:
+            negatif = 0
+            vdotv, fdotf = float((v * v).sum()), float((f * f).sum())
+            scale1 = 1.0 - alpha
+            scale2 = 0.0 if fdotf <= 1e-20 else alpha * np.sqrt(vdotv / fdotf)
+            mix = True
+            if ts - last_negative > p["delaystep"]:
+                dtcur = min(dtcur * p["dtgrow"], dtmax)
+                alpha *= p["alphashrink"]
+        else:
+            last_negative = ts
+            if not (ts - start < p["delaystep"] and p["delaystep_start"]):
+                alpha = p["alpha0"]
+                if dtcur * p["dtshrink"] >= dtmin:
+                    dtcur *= p["dtshrink"]
+            negatif += 1
+            if p["max_vdotf_negatif"] > 0 and negatif > p["max_vdotf_negatif"]:
+                stop = 11
+                break
+            if p["halfstepback"]:
+                x -= 0.5 * dtv * v
+            v[:] = 0.0
+        vmax = float(np.abs(v).max()) if v.size else 0.0
+        dtv = dtcur
+        if dtv * vmax > p["dmax"]:
+            dtv = p["dmax"] / vmax
+        v += (dtv * ftm2v / m) * f
+        if mix:
+            v = scale1 * v + scale2 * f
+        x += dtv * v
+        if ((x - xhold) ** 2).sum(1).max() > (0.5 * skin) ** 2:
+            o.put_x(x)
+            o.pbc()
+            o.build_list()
+            x = o.get_x()
+            xhold = x.copy()
+        else:
+            o.put_x(x)
+        r = o.compute(1, 0)
+        f = r["f"]
+        neval += 1
+        eprev, ecur = ecur, r["energy"]
+        if etol > 0.0 and ts - last_negative > p["delaystep"] and \
+                abs(ecur - eprev) < etol * 0.5 * (abs(ecur) + abs(eprev) + 1.0e-8):
+            stop = 2
+            break
+        if ftol > 0.0 and fnorm(f) < ftol * ftol:
+            stop = 3
+            break
+    return {"stop": stop, "iterations": niter, "evaluations": neval, "e_initial": e_init, "e_final": ecur,
+            "e_next_to_last": eprev, "fnorm2_final": float(np.sqrt((f * f).sum())), "x": x, "f": f}

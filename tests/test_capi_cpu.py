@@ -61,3 +61,20 @@ def test_generators_shapes():
     r = np.sqrt((d ** 2).sum(-1))
     nb = ((r < 1.6) & (r > 0.1) & same).sum(1)
     assert (nb == 3).all()
+
+
+def test_write_data_round_trip(tmp_path):
+    """write_lammps_data -> read_lammps_data gives back the same atoms, bit for bit."""
+    from lmff import structures as S
+
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=8)
+    path = str(tmp_path / "d.data")
+    S.write_lammps_data(s, path, step=120)
+    r = S.read_lammps_data(path)
+    assert open(path).readline().strip().endswith("timestep = 120")
+    assert r.n == s.n
+    o = np.argsort(s.tag)
+    assert np.array_equal(r.tag, s.tag[o]) and np.array_equal(r.mol, s.mol[o]) and np.array_equal(r.type, s.type[o])
+    assert np.array_equal(r.x, s.x[o]) and np.array_equal(r.v, s.v[o])
+    assert np.array_equal(r.boxlo, s.boxlo) and np.array_equal(r.boxhi, s.boxhi)
+    assert r.meta["masses"] == {t + 1: float(f"{m:g}") for t, m in enumerate(s.mass_per_type)}

@@ -325,3 +325,70 @@ def test_nve_drift_fixture_is_the_current_oracle(oracle_lib):
     late = e[len(e) // 2:]
     slope = np.polyfit(np.array(fx["step"][len(e) // 2:], dtype=float), late, 1)[0]
     assert abs(slope) * 5000 < 3 * late.std() + 1e-6
+
+
+class _Harmonic:
+    """Stands in for the oracle: E = k/2 |x - c|^2 per atom, F = -k (x - c)."""
+
+    def __init__(self, c, k=2.0):
+        self.c, self.k, self.x = np.asarray(c, dtype=float), k, None
+
+    def put_x(self, x):
+        self.x = np.array(x, dtype=float)
+
+    def pbc(self):
+        pass
+
+    def build_list(self):
+        pass
+
+    def get_x(self):
+        return self.x.copy()
+
+    def compute(self, eflag=1, vflag=0):
+        d = self.x - self.c
+        return {"f": -self.k * d, "energy": 0.5 * self.k * float((d * d).sum())}
+
+
+def test_fire_restatement_first_iterations_by_hand():
+    """Iteration 1 starts from v = 0 (v.f = 0: the 'not downhill' branch, no dt change inside the
+    initial delay) and takes x1 = x0 + dt^2 ftm2v/m f0; iteration 2 has v.f > 0 and mixes
+    v = (1-a) v' + a |v'| f/|f| with a = alpha0 (no dt growth before delaystep)."""
+    import oracle
+
+    ftm2v, m, dt, k = 1.0 / 1.0364269e-4, 12.0, 0.001, 2.0
+    x0 = np.array([[0.03, -0.02, 0.01]])
+    h = _Harmonic(np.zeros((1, 3)), k)
+    r1 = oracle.fire_minimize(h, x0, [m], dt, 0.0, 0.0, 1, dmax=10.0)
+    f0 = -k * x0
+    v1 = dt * ftm2v / m * f0
+    x1 = x0 + dt * v1
+    assert np.allclose(r1["x"], x1, rtol=1e-14, atol=0)
+    r2 = oracle.fire_minimize(h, x0, [m], dt, 0.0, 0.0, 2, dmax=10.0)
+    f1 = -k * x1
+    vp = v1 + dt * ftm2v / m * f1
+    a = 0.25
+    v2 = (1 - a) * vp + a * np.linalg.norm(v1) / np.linalg.norm(f1) * f1
+    x2 = x1 + dt * v2
+    assert np.allclose(r2["x"], x2, rtol=1e-13, atol=0)
+    # with a small dmax the second step is dtv = dmax / max|v1_c| instead of dt
+    dmax = 1e-6
+    r3 = oracle.fire_minimize(h, x0, [m], dt, 0.0, 0.0, 2, dmax=dmax)
+    dtv = dmax / np.abs(v1).max()
+    assert dt * np.abs(v1).max() > dmax
+    vp = v1 + dtv * ftm2v / m * f1
+    v2 = (1 - a) * vp + a * np.linalg.norm(v1) / np.linalg.norm(f1) * f1
+    assert np.allclose(r3["x"], x1 + dtv * v2, rtol=1e-13, atol=0)
+
+
+def test_fire_restatement_converges_to_the_minimum():
+    import oracle
+
+    rng = np.random.default_rng(3)
+    c = rng.normal(size=(20, 3))
+    h = _Harmonic(c, 5.0)
+    r = oracle.fire_minimize(h, c + rng.normal(scale=0.2, size=c.shape), np.full(20, 12.0), 0.001, 0.0, 1e-8, 20000)
+    assert r["stop"] == 3
+    assert np.abs(r["x"] - c).max() < 1e-8
+    r = oracle.fire_minimize(h, c + 0.1, np.full(20, 12.0), 0.001, 1e-12, 0.0, 20000)
+    assert r["stop"] == 2 and r["e_final"] < 1e-8

@@ -1,0 +1,116 @@
+"""The deck runner (lmff_run) end to end on the GPU: dump custom with compute pe/atom and
+stress/atom, write_data, and a restart from that file with reset_timestep.  The dumped
+per-atom columns are checked against Pair::compute (lmff_pair_compute) on the dumped
+positions; the restarted run must follow the uninterrupted one."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import POT, REPO
+from lmff import capi, structures as S
+
+pytestmark = pytest.mark.gpu
+
+RUNNER = os.path.join(REPO, "lammps-lmff_amd", "lmff_run")
+NKTV2P, MVV2E = 1.6021765e6, 1.0364269e-4
+
+
+def deck(data, extra_top, body):
+    return "\n".join([
+        "units metal", "atom_style full", "boundary p p f", extra_top,
+        f"read_data {data}",
+        "pair_style hybrid/overlay tersoff ilp/graphene/hbn 16.0 1",
+        f"pair_coeff * * tersoff {os.path.join(POT, 'BNC.tersoff')} C B N",
+        f"pair_coeff * * ilp/graphene/hbn {os.path.join(POT, 'BNCH.ILP')} C B N",
+        "neighbor 2.0 bin", "neigh_modify every 10 delay 0 check no", "timestep 0.001",
+        "fix 1 all nve", body, ""])
+
+
+def run_deck(tmp_path, name, text):
+    path = tmp_path / name
+    path.write_text(text)
+    print("running", name, flush=True)
+    r = subprocess.run([RUNNER, "-in", str(path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_dump_computes_write_data_and_restart(tmp_path):
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=61)
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    out = run_deck(tmp_path, "a.in", deck(data, "", "\n".join([
+        "compute pe all pe/atom", "compute st all stress/atom NULL",
+        "dump d1 all custom 10 a.dump id type x y z vx vy vz fx fy fz c_pe c_st[*]",
+        "dump_modify d1 format float %.17g", "thermo 10", "run 20", "write_data mid.data", "run 20"])))
+    frames = S.read_dump_custom(str(tmp_path / "a.dump"))
+    assert [f[0] for f in frames] == [0, 10, 20, 30, 40]
+    cols = frames[0][1]
+    assert cols[:11] == ["id", "type", "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"]
+    assert cols[11:] == ["c_pe"] + [f"c_st[{k}]" for k in range(1, 7)]
+    # E_pair of step 0 from the thermo table equals the sum of c_pe
+    e0 = float([l for l in out.splitlines() if l.strip().startswith("0 ")][0].split()[2])
+    assert abs(frames[0][2][:, 11].sum() - e0) < 1e-9 * abs(e0)
+
+    # per-atom columns of step 20 against one Pair::compute on the dumped positions
+    step, _, a = frames[2]
+    t = s.copy()
+    order = np.argsort(s.tag)
+    t.x = a[:, 2:5]
+    t.tag, t.type, t.mol = s.tag[order], s.type[order], s.mol[order]
+    with capi.Context(0) as ctx:
+        capi.setup_for(ctx, t)
+        r = ctx.pair_compute(t, 3, 6)
+    m = t.mass_per_type[t.type - 1]
+    fscale = np.abs(r["f"]).max()
+    assert np.abs(a[:, 8:11] - r["f"]).max() < 1e-9 * fscale
+    assert np.abs(a[:, 11] - r["eatom"]).max() < 1e-10 * np.abs(r["eatom"]).max()
+    v = a[:, 5:8]
+    ia, ib = [0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]
+    stress = -NKTV2P * (MVV2E * m[:, None] * v[:, ia] * v[:, ib] + r["vatom"])
+    assert np.abs(a[:, 12:18] - stress).max() < 1e-9 * np.abs(stress).max()
+
+    # write_data at step 20 holds exactly the dumped state
+    mid = S.read_lammps_data(str(tmp_path / "mid.data"))
+    assert open(tmp_path / "mid.data").readline().strip().endswith("timestep = 20")
+    assert np.array_equal(mid.x, a[:, 2:5]) and np.array_equal(mid.v, a[:, 5:8])
+
+    # restart: read the mid-run file, reset the step, run the remaining 20 steps
+    run_deck(tmp_path, "b.in", deck(str(tmp_path / "mid.data"), "", "\n".join([
+        "reset_timestep 20", "dump d2 all custom 20 b.dump id x y z", "dump_modify d2 format float %.17g",
+        "thermo 10", "run 20"])))
+    fb = S.read_dump_custom(str(tmp_path / "b.dump"))
+    assert [f[0] for f in fb] == [20, 40]
+    assert np.array_equal(fb[0][2][:, 1:4], a[:, 2:5])
+    assert np.abs(fb[1][2][:, 1:4] - frames[4][2][:, 2:5]).max() < 1e-8
+
+
+def test_minimize_from_a_deck(tmp_path):
+    """min_style fire + minimize etol ftol maxiter maxeval in a deck: the log's Minimization
+    stats (LAMMPS layout) report the FIRE run of the library."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=71, jitter=0.08)
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    out = run_deck(tmp_path, "m.in", deck(data, "", "\n".join([
+        "min_style fire", "minimize 0.0 0.1 400 100000", "write_data relaxed.data"])))
+    lines = out.splitlines()
+    k = lines.index("Minimization stats:")
+    assert lines[k + 1].strip() == "Stopping criterion = force tolerance"
+    e = [float(t) for t in lines[k + 3].split()]
+    it = [int(t) for t in lines[k + 6].split("=")[1].split()]
+    with capi.Context(0) as ctx:
+        capi.setup_for(ctx, s)
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+        st = ctx.minimize(0.0, 0.1, 400)[0]
+    assert it == [st["iterations"], st["evaluations"]]
+    assert abs(e[2] - st["e_final"]) < 1e-12 * abs(e[2]) + 1e-9
+    r = S.read_lammps_data(str(tmp_path / "relaxed.data"))
+    assert open(tmp_path / "relaxed.data").readline().strip().endswith(f"timestep = {st['iterations']}")
+    assert r.n == s.n
