@@ -131,7 +131,10 @@ typedef struct {
   int norm;                 /* force norm for ftol: 0 two, 1 max, 2 inf (min_modify norm) */
 } lmff_fire_params;
 
-enum { LMFF_STOP_MAXITER = 0, LMFF_STOP_ETOL = 2, LMFF_STOP_FTOL = 3, LMFF_STOP_MAXVDOTF = 11 };
+enum {
+  LMFF_STOP_MAXITER = 0, LMFF_STOP_MAXEVAL = 1, LMFF_STOP_ETOL = 2, LMFF_STOP_FTOL = 3, LMFF_STOP_DOWNHILL = 4,
+  LMFF_STOP_ZEROALPHA = 5, LMFF_STOP_ZEROFORCE = 6, LMFF_STOP_ZEROQUAD = 7, LMFF_STOP_MAXVDOTF = 11
+};
 
 typedef struct {
   int stop;                 /* LMFF_STOP_* (the reference's stop_condition codes) */
@@ -147,6 +150,13 @@ void lmff_fire_defaults(lmff_fire_params *p);
 /* params NULL: defaults; out[k] for the k-th resident system (in order of addition) */
 int lmff_minimize(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, const lmff_fire_params *params,
                   lmff_min_stats *out, int out_cap);
+
+/* min_style cg (the reference's default) with the quadratic line search: MinCG::iterate and
+ * MinLineSearch::linemin_quadratic (LMFF/min_cg.cpp:40-190, LMFF/min_linesearch.cpp:324-503);
+ * dmax is min_modify dmax (0.1), norm as in lmff_fire_params.  The context must hold exactly one
+ * system.  Velocities are not touched; the step counter advances by the iterations. */
+int lmff_minimize_cg(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, int64_t maxeval, double dmax,
+                     int norm, lmff_min_stats *out);
 
 /* Thermo record of one system (metal units). */
 typedef struct {

@@ -143,6 +143,9 @@ struct Ctx {
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
   int ntp = 0;  // Tersoff parameter records
+  // conjugate-gradient vectors (x0, g, h), carried through rebuilds while a CG minimisation runs
+  DBuf cg_x0, cg_g, cg_h;
+  bool cg_active = false;
   // atom ranges whose last-step force is restored after the next set-up force evaluation
   std::vector<std::pair<int, int>> restore_f;
   int scap = 64, icap = 0;
@@ -558,6 +561,13 @@ struct Ctx {
     permutei(mol);
     permutei(tag);
     permutei(orig);
+    if (cg_active) {
+      permute4(cg_x0);
+      permute4(cg_g);
+      permute4(cg_h);
+      hipLaunchKernelGGL(k_image_follow, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)x.p,
+                         cg_x0.as<double4>(), sysid.as<int>(), systab.as<SysDev>());
+    }
     hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
                        key2.as<uint32_t>(), cell_start.as<int>());
     HIPCHK(hipMemcpyAsync(xhold.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
@@ -684,6 +694,160 @@ struct Ctx {
                        chunks.as<Chunk>(), partial.as<double2>(), nmax);
     hipLaunchKernelGGL(k_reduce_systems, dim3((unsigned)sys.size()), dim3(64), 0, stream, (const double2 *)partial.p,
                        ncols, cbeg.as<int>(), (int)sys.size(), dst_dev, nmax);
+  }
+
+  // ---------------------------------------------------------------- CG minimisation
+  // MinCG::iterate (LMFF/min_cg.cpp:40-190) with linemin_quadratic and alpha_step
+  // (LMFF/min_linesearch.cpp:324-503, 837-868); one system per context (the line search's
+  // evaluation sequence is the system's own).  Lists as Min::init: every 1, delay 0, check yes.
+  void minimize_cg(double etol, double ftol, int64_t maxiter, int64_t maxeval, double dmax, int norm,
+                   lmff_min_stats &st) {
+    relayout();
+    if (sys.size() != 1) fail(LMFF_ERR_STATE, "min_style cg needs exactly one system in the context");
+    st = lmff_min_stats{};
+    if (n == 0) return;
+    const double EMACH = 1.0e-8, EPS_QUAD = 1.0e-28, ALPHA_MAX = 1.0, ALPHA_REDUCE = 0.5, BACKTRACK_SLOPE = 0.4,
+                 QUADRATIC_TOL = 0.1, eps_energy = 1.0e-8;
+    const int64_t nlimit = std::min<int64_t>(2147483647, 3 * (int64_t)n);
+    cg_x0.ensure(n * sizeof(double4));
+    cg_g.ensure(n * sizeof(double4));
+    cg_h.ensure(n * sizeof(double4));
+    const int sv_every = every, sv_delay = delay, sv_check = check;
+    every = 1;
+    delay = 0;
+    check = 1;
+    cg_active = true;
+    struct Restore {
+      Ctx &c;
+      int e, d, k;
+      ~Restore() { c.every = e; c.delay = d; c.check = k; c.cg_active = false; }
+    } restore{*this, sv_every, sv_delay, sv_check};
+    DBuf out;
+    out.ensure(sizeof(double) * 16);
+    double hst[9], hd[2];
+    const int nb = (int)((n + BS - 1) / BS);
+    auto eval = [&]() {  // energy, then f.f, f.h, f.g, max|h_c|, max|f|^2, max|f_c| of the new forces
+      ago++;
+      if (rebuild_due()) rebuild();
+      force(1, 0);
+      reduce(3, out.as<double>());
+      hipLaunchKernelGGL(k_cg_stats, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p,
+                         (const double4 *)cg_g.p, (const double4 *)cg_h.p, th.as<double>());
+      reduce(6, out.as<double>() + 3, 3);
+      HIPCHK(hipMemcpyAsync(hst, out.p, sizeof hst, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      check_errors();
+      return hst[0] + hst[1] + hst[2];
+    };
+    int64_t neval = 0;
+    const double hm = 0.5 * ilp_margin();
+    auto alpha_step = [&](double alpha) {
+      hipLaunchKernelGGL(k_cg_alpha_step, dim3(nb), dim3(BS), 0, stream, (int)n, x.as<double4>(),
+                         (const double4 *)cg_x0.p, (const double4 *)cg_h.p, alpha, (const double4 *)xhold.p, hm * hm,
+                         farflag.as<int>());
+      neval++;
+      return eval();
+    };
+    auto fnorm = [&]() { return norm == 1 ? hst[7] : norm == 2 ? hst[8] : hst[3]; };
+    // Min::setup: lists and forces of the start (not counted as an evaluation)
+    if (need_rebuild) rebuild();
+    ago = -1;  // eval() counts this evaluation's step
+    double ecurrent = eval();
+    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p, cg_g.as<double4>(),
+                       cg_h.as<double4>(), 0.0, 0, th.as<double>());
+    double gg = hst[3];
+    double fdoth = hst[3], hmax = hst[8];  // h = f
+    st.e_initial = st.e_next_to_last = ecurrent;
+    st.fnorm2_initial = std::sqrt(hst[3]);
+    st.fnorminf_initial = hst[8];
+    st.stop = LMFF_STOP_MAXITER;
+    // quadratic line search along h from the current point; returns 0 or a stop code
+    auto linemin = [&](double eoriginal) -> int {
+      const double fdothall = fdoth;
+      if (fdothall <= 0.0) return LMFF_STOP_DOWNHILL;
+      const double alphamax = std::min(ALPHA_MAX, dmax / hmax);
+      if (hmax == 0.0) return LMFF_STOP_ZEROFORCE;
+      HIPCHK(hipMemcpyAsync(cg_x0.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+      double alpha = alphamax, fhprev = fdothall, engprev = eoriginal, alphaprev = 0.0;
+      while (true) {
+        ecurrent = alpha_step(alpha);
+        const double fh = hst[4];
+        const double delfh = fh - fhprev;
+        if (std::fabs(fh) < EPS_QUAD || std::fabs(delfh) < EPS_QUAD) {
+          ecurrent = alpha_step(0.0);
+          return LMFF_STOP_ZEROQUAD;
+        }
+        const double relerr = std::fabs(1.0 - (0.5 * (alpha - alphaprev) * (fh + fhprev) + ecurrent) / engprev);
+        const double alpha0 = alpha - (alpha - alphaprev) * fh / delfh;
+        if (relerr <= QUADRATIC_TOL && alpha0 > 0.0 && alpha0 < alphamax) {
+          ecurrent = alpha_step(alpha0);
+          if (ecurrent - eoriginal < EMACH) return 0;
+        }
+        const double de_ideal = -BACKTRACK_SLOPE * alpha * fdothall;
+        const double de = ecurrent - eoriginal;
+        if (de <= de_ideal) return 0;
+        fhprev = fh;
+        engprev = ecurrent;
+        alphaprev = alpha;
+        alpha *= ALPHA_REDUCE;
+        if (alpha <= 0.0 || de_ideal >= -EMACH) {
+          ecurrent = alpha_step(0.0);
+          return LMFF_STOP_ZEROALPHA;
+        }
+      }
+    };
+    int64_t niter = 0;
+    double eprevious = ecurrent;
+    for (int64_t iter = 0; iter < maxiter; iter++) {
+      niter++;
+      eprevious = ecurrent;
+      const int rc = linemin(ecurrent);
+      if (rc) {
+        st.stop = rc;
+        break;
+      }
+      if (neval >= maxeval) {
+        st.stop = LMFF_STOP_MAXEVAL;
+        break;
+      }
+      if (std::fabs(ecurrent - eprevious) < etol * 0.5 * (std::fabs(ecurrent) + std::fabs(eprevious) + eps_energy)) {
+        st.stop = LMFF_STOP_ETOL;
+        break;
+      }
+      const double dot0 = hst[3], dot1 = hst[5];
+      if (ftol > 0.0 && fnorm() < ftol * ftol) {
+        st.stop = LMFF_STOP_FTOL;
+        break;
+      }
+      double beta = std::max(0.0, (dot0 - dot1) / gg);
+      if ((niter + 1) % nlimit == 0) beta = 0.0;
+      gg = dot0;
+      hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p, cg_g.as<double4>(),
+                         cg_h.as<double4>(), beta, 1, th.as<double>());
+      reduce(2, out.as<double>() + 12, 1);
+      HIPCHK(hipMemcpyAsync(hd, out.as<double>() + 12, sizeof hd, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (hd[0] <= 0.0) {
+        hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p, cg_g.as<double4>(),
+                           cg_h.as<double4>(), 0.0, 2, th.as<double>());
+        fdoth = dot0;   // h = g = f
+        hmax = hst[8];
+      } else {
+        fdoth = hd[0];
+        hmax = hd[1];
+      }
+    }
+    st.iterations = niter;
+    st.evaluations = neval;
+    st.e_next_to_last = eprevious;
+    st.e_final = ecurrent;
+    st.fnorm2_final = std::sqrt(hst[3]);
+    st.fnorminf_final = hst[8];
+    sync_steps_from_device();
+    sys[0].step += niter;
+    upload_systab();
+    gen++;
+    forces_valid = true;
   }
 
   // ---------------------------------------------------------------- FIRE minimisation
@@ -1327,6 +1491,18 @@ int lmff_minimize(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, cons
     c.minimize(etol, ftol, maxiter, p, st);
     if (out)
       for (size_t k = 0; k < st.size() && (int)k < out_cap; k++) out[k] = st[k];
+  });
+}
+
+int lmff_minimize_cg(lmff_ctx *ctx, double etol, double ftol, int64_t maxiter, int64_t maxeval, double dmax, int norm,
+                     lmff_min_stats *out) {
+  return guard(ctx, [&] {
+    if (etol < 0.0 || ftol < 0.0 || maxiter < 0 || maxeval < 0) fail(LMFF_ERR_ARG, "Illegal minimize command");
+    if (!(dmax > 0.0)) fail(LMFF_ERR_ARG, "Illegal min_modify command");
+    if (norm < 0 || norm > 2) fail(LMFF_ERR_ARG, "Illegal min_modify command");
+    lmff_min_stats st;
+    ctx->c.minimize_cg(etol, ftol, maxiter, maxeval, dmax, norm, st);
+    if (out) *out = st;
   });
 }
 

@@ -1552,6 +1552,80 @@ __global__ void k_fire_step(int n, double4 *x, double4 *v, const double4 *f, con
   if (dx * dx + dy * dy + dz * dz > far_sq && *(volatile int *)farflag == 0) atomicOr(farflag, 1);
 }
 
+// ---------------------------------------------------------------------------------------
+// Conjugate gradients with the quadratic line search (MinCG::iterate, LMFF/min_cg.cpp:40-190;
+// MinLineSearch::linemin_quadratic and alpha_step, LMFF/min_linesearch.cpp:324-503,837-868).
+// Per-atom vectors: x0 (start of the line search), g (last gradient), h (search direction).
+
+// per atom: f.f, f.h, f.g (summed) and max|h_c|, |f|^2, max|f_c| (maxed)
+__global__ void k_cg_stats(int n, const double4 *f, const double4 *g, const double4 *h, double *cols) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double4 a = f[p], b = g[p], c = h[p];
+  const double ff = a.x * a.x + a.y * a.y + a.z * a.z;
+  cols[0 * (size_t)n + p] = ff;
+  cols[1 * (size_t)n + p] = a.x * c.x + a.y * c.y + a.z * c.z;
+  cols[2 * (size_t)n + p] = a.x * b.x + a.y * b.y + a.z * b.z;
+  cols[3 * (size_t)n + p] = fmax(fmax(fabs(c.x), fabs(c.y)), fabs(c.z));
+  cols[4 * (size_t)n + p] = ff;
+  cols[5 * (size_t)n + p] = fmax(fmax(fabs(a.x), fabs(a.y)), fabs(a.z));
+}
+
+// mode 0: h = g = f;  mode 1: g = f, h = g + beta h, cols <- g.h and max|h_c|;  mode 2: h = g
+__global__ void k_cg_dir(int n, const double4 *f, double4 *g, double4 *h, double beta, int mode, double *cols) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (mode == 2) {
+    h[p] = g[p];
+    return;
+  }
+  const double4 a = f[p];
+  double4 c = mode == 0 ? a : h[p];
+  if (mode == 1) {
+    c.x = a.x + beta * c.x;
+    c.y = a.y + beta * c.y;
+    c.z = a.z + beta * c.z;
+    cols[0 * (size_t)n + p] = a.x * c.x + a.y * c.y + a.z * c.z;
+    cols[1 * (size_t)n + p] = fmax(fmax(fabs(c.x), fabs(c.y)), fabs(c.z));
+  }
+  g[p] = a;
+  h[p] = c;
+}
+
+// x = x0 + alpha h (x = x0 for alpha <= 0), keeping x.w (type); raises the far flag
+__global__ void k_cg_alpha_step(int n, double4 *x, const double4 *x0, const double4 *h, double alpha,
+                                const double4 *xhold, double far_sq, int *farflag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double4 xp = x0[p];
+  xp.w = x[p].w;
+  if (alpha > 0.0) {
+    const double4 c = h[p];
+    xp.x += alpha * c.x;
+    xp.y += alpha * c.y;
+    xp.z += alpha * c.z;
+  }
+  x[p] = xp;
+  const double4 q = xhold[p];
+  const double dx = xp.x - q.x, dy = xp.y - q.y, dz = xp.z - q.z;
+  if (dx * dx + dy * dy + dz * dz > far_sq && *(volatile int *)farflag == 0) atomicOr(farflag, 1);
+}
+
+// after a rebuild wrapped x into the box: x0 follows by the same periodic image
+// (FixMinimize::reset_coords, LMFF/fix_minimize.cpp)
+__global__ void k_image_follow(int n, const double4 *x, double4 *x0, const int *sys, const SysDev *st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const SysDev &s = st[sys[p]];
+  const double4 a = x[p];
+  double4 b = x0[p];
+  double *bb = &b.x;
+  const double aa[3] = {a.x, a.y, a.z};
+  for (int d = 0; d < 3; d++)
+    if (s.per[d]) bb[d] += s.prd[d] * rint((aa[d] - bb[d]) / s.prd[d]);
+  x0[p] = b;
+}
+
 // max squared displacement since the last build, per system (neigh_modify check yes)
 __global__ void k_displacement(int n, const double4 *x, const double4 *xhold, const int *sys, double trigger_sq,
                                int *flag) {

@@ -490,17 +490,21 @@ int main(int argc, char **argv) {
         else std::printf("WARNING: min_modify %s ignored by lmff_run\n", key.c_str());
       }
     } else if (cmd == "minimize") {
-      // minimize etol ftol maxiter maxeval (LMFF/minimize.cpp), FIRE on the device
+      // minimize etol ftol maxiter maxeval (LMFF/minimize.cpp) on the device: min_style cg (the
+      // default, quadratic line search) or fire
       if (w.size() < 5) die("Illegal minimize command");
-      if (min_style != "fire")
-        std::printf("WARNING: min_style %s is not available in lmff_run; minimizing with fire\n", min_style.c_str());
+      if (min_style != "fire" && min_style != "cg")
+        std::printf("WARNING: min_style %s is not available in lmff_run; minimizing with cg\n", min_style.c_str());
       setup_pair();
       lmff_ctx *ctx = pair->context();
       if (lmff_system_integrator(ctx, sys, style, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
       lmff_min_stats ms;
       auto c0 = std::chrono::steady_clock::now();
-      if (lmff_minimize(ctx, std::stod(w[1]), std::stod(w[2]), std::stoll(w[3]), &fire, &ms, 1))
-        die(lmff_last_error(ctx));
+      const int rc = min_style == "fire"
+                         ? lmff_minimize(ctx, std::stod(w[1]), std::stod(w[2]), std::stoll(w[3]), &fire, &ms, 1)
+                         : lmff_minimize_cg(ctx, std::stod(w[1]), std::stod(w[2]), std::stoll(w[3]), std::stoll(w[4]),
+                                            fire.dmax, fire.norm, &ms);
+      if (rc) die(lmff_last_error(ctx));
       const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
       static const char *stops[] = {"max iterations", "max force evaluations", "energy tolerance", "force tolerance",
                                     "search direction is not downhill", "linesearch alpha is zero", "forces are zero",

@@ -30,11 +30,12 @@ SYMBOLS = [
     "lmff_system_peratom", "lmff_evaluate", "lmff_system_thermo", "lmff_run", "lmff_profile",
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
     "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
-    "lmff_minimize",
+    "lmff_minimize", "lmff_minimize_cg",
 ]
 
 STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
-        11: "max iterations with v.f negative"}
+        4: "search direction is not downhill", 5: "linesearch alpha is zero", 6: "forces are zero",
+        7: "quadratic factors are zero", 11: "max iterations with v.f negative"}
 
 
 class FireParams(C.Structure):
@@ -126,6 +127,8 @@ def load(path: str = LIB_PATH):
     L.lmff_fire_defaults.restype = None
     L.lmff_minimize.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.POINTER(FireParams), C.POINTER(MinStats),
                                 C.c_int]
+    L.lmff_minimize_cg.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_double, C.c_int,
+                                   C.POINTER(MinStats)]
     _lib = L
     return L
 
@@ -289,6 +292,13 @@ class Context:
         out = (MinStats * max(nsys, 1))()
         self._chk(self.L.lmff_minimize(self.h, float(etol), float(ftol), int(maxiter), C.byref(p), out, int(nsys)))
         return [out[k].as_dict() for k in range(nsys)]
+
+    def minimize_cg(self, etol: float, ftol: float, maxiter: int, maxeval: int, dmax: float = 0.1, norm: int = 0):
+        """min_style cg (quadratic line search) + minimize etol ftol maxiter maxeval; one system."""
+        out = MinStats()
+        self._chk(self.L.lmff_minimize_cg(self.h, float(etol), float(ftol), int(maxiter), int(maxeval), float(dmax),
+                                          int(norm), C.byref(out)))
+        return out.as_dict()
 
     def evaluate(self, eflag: int = 1, vflag: int = 2):
         self._chk(self.L.lmff_evaluate(self.h, int(eflag), int(vflag)))

@@ -41,6 +41,7 @@ class Job:
     sys_id: int = -1
     trace: list = field(default_factory=list)
     result: dict | None = None
+    status: str = "pending"        # pending -> queued -> running -> done | cancelled
 
     @property
     def natoms(self) -> int:
@@ -94,6 +95,21 @@ class Server:
         self.resident: dict[int, Job] = {}
         self.reserved = 0.0
         self.log = []
+        self._cancel: set[int] = set()
+        self.cancelled: list[Job] = []
+
+    def cancel(self, job_id: int):
+        """Requests cancellation: a queued job never starts, a running one leaves at the next tick
+        boundary (its slot and HBM reservation are freed, no result is kept)."""
+        self._cancel.add(job_id)
+
+    def snapshot(self, job_id: int):
+        """Positions and velocities of a running job now (caller's atom order)."""
+        for job in self.resident.values():
+            if job.job_id == job_id:
+                x, v, _ = self.ctx.get(job.sys_id)
+                return {"step": job.done_steps, "x": x, "v": v}
+        raise KeyError(f"job {job_id} is not running")
 
     def close(self):
         self.ctx.close()
@@ -106,6 +122,7 @@ class Server:
         self.ctx.integrator(sid, job.style, dt=self.dt, t_target=job.t_target, damp=job.damp, seed=job.seed)
         job.sys_id = sid
         job.admitted_at = now
+        job.status = "running"
         self.resident[sid] = job
         self.reserved += self._job_bytes(job)
 
@@ -118,10 +135,20 @@ class Server:
         self.reserved -= self._job_bytes(job)
         del self.resident[job.sys_id]
         job.finished_at = now
+        job.status = "done"
 
-    def run(self, jobs, clock=None, time_scale: float = 1.0):
+    def _drop(self, job: Job, now: float):
+        self.ctx.remove_system(job.sys_id)
+        self.reserved -= self._job_bytes(job)
+        del self.resident[job.sys_id]
+        job.finished_at = now
+        job.status = "cancelled"
+        self.cancelled.append(job)
+
+    def run(self, jobs, clock=None, time_scale: float = 1.0, on_tick=None):
         """Serves `jobs` (sorted by arrival).  Arrival times are in seconds of wall clock
-        (scaled by time_scale).  Returns summary statistics."""
+        (scaled by time_scale).  on_tick(server, now), if given, is called after every tick
+        (it may cancel jobs or take snapshots).  Returns summary statistics."""
         clock = clock or time.perf_counter
         pending = sorted(jobs, key=lambda j: j.arrival)
         queue = []   # arrived, not admitted (FIFO)
@@ -132,8 +159,19 @@ class Server:
         while pi < len(pending) or queue or self.resident:
             now = (clock() - t0) / time_scale
             while pi < len(pending) and pending[pi].arrival <= now:
+                pending[pi].status = "queued"
                 queue.append(pending[pi])
                 pi += 1
+            # cancellations: queued jobs never start, running ones leave now
+            if self._cancel:
+                for job in [j for j in queue if j.job_id in self._cancel]:
+                    queue.remove(job)
+                    job.status = "cancelled"
+                    job.finished_at = now
+                    self.cancelled.append(job)
+                for job in [j for j in self.resident.values() if j.job_id in self._cancel]:
+                    self._drop(job, now)
+                self._cancel -= {j.job_id for j in self.cancelled}
             # retire finished jobs
             for job in [j for j in self.resident.values() if j.done_steps >= j.steps]:
                 self._retire(job, now)
@@ -161,10 +199,13 @@ class Server:
                 job.trace.append(rec[k][-1])
                 atom_steps += job.natoms * self.tick
             self.log.append((now, nsys, sum(j.natoms for j in self.resident.values())))
+            if on_tick is not None:
+                on_tick(self, (clock() - t0) / time_scale)
         t_end = (clock() - t0) / time_scale
         lat = np.array([j.finished_at - j.arrival for j in finished])
         return {
             "jobs": len(finished),
+            "cancelled": len(self.cancelled),
             "makespan_s": t_end,
             "atom_steps": atom_steps,
             "atom_steps_per_s": atom_steps / t_end if t_end > 0 else float("nan"),

@@ -294,3 +294,129 @@ def fire_minimize(o: Oracle, x0, mass_per_atom, dt: float, etol: float, ftol: fl
             break
     return {"stop": stop, "iterations": niter, "evaluations": neval, "e_initial": e_init, "e_final": ecur,
             "e_next_to_last": eprev, "fnorm2_final": float(np.sqrt((f * f).sum())), "x": x, "f": f}
+
+
+def cg_minimize(o: Oracle, x0, etol: float, ftol: float, maxiter: int, maxeval: int, skin: float = 2.0,
+                dmax: float = 0.1, norm: int = 0):
+    """Conjugate gradients on the oracle's forces: MinCG::iterate (LMFF/min_cg.cpp:40-190,
+    Polak-Ribiere beta >= 0, restart every 3N iterations and when g.h <= 0) with
+    MinLineSearch::linemin_quadratic and alpha_step (LMFF/min_linesearch.cpp:324-503, 837-868:
+    alphamax = min(1, dmax/max|h|), backtracking by 1/2 with slope 0.4, quadratic projection
+    when the secant model fits within 0.1).  Lists are rebuilt when an atom has moved more than
+    skin/2 (Min::init); the start of the line search follows a wrap by its periodic image
+    (FixMinimize::reset_coords).  Stop codes as Min::stopstrings."""
+    EMACH, EPS_QUAD = 1.0e-8, 1.0e-28
+    x = np.array(x0, dtype=np.float64)
+    o.put_x(x)
+    o.pbc()
+    o.build_list()
+    x = o.get_x()
+    xhold = x.copy()
+    L = np.asarray(o._keep[4]) - np.asarray(o._keep[3])
+    per = np.asarray(o._keep[5]).astype(bool)
+    state = {"x": x, "xhold": xhold, "neval": 0}
+
+    def evaluate(xn):
+        if ((xn - state["xhold"]) ** 2).sum(1).max() > (0.5 * skin) ** 2:
+            o.put_x(xn)
+            o.pbc()
+            o.build_list()
+            xw = o.get_x()
+            state["xhold"] = xw.copy()
+            state["wrap"] = xw - xn
+            xn = xw
+        else:
+            o.put_x(xn)
+        state["x"] = xn
+        r = o.compute(1, 0)
+        return r["energy"], r["f"]
+
+    def fnorm(ff):
+        if norm == 1:
+            return (ff * ff).sum(1).max()
+        if norm == 2:
+            return np.abs(ff).max()
+        return (ff * ff).sum()
+
+    ecur, f = evaluate(x)
+    e_init = ecur
+    g = f.copy()
+    h = f.copy()
+    gg = float((f * f).sum())
+    nlimit = min(2147483647, 3 * len(x))
+
+    def alpha_step(xs, alpha):
+        state["neval"] += 1
+        xn = xs + alpha * h if alpha > 0.0 else xs.copy()
+        return evaluate(xn)
+
+    def image_follow(xs):
+        d = state["x"] - xs
+        for k in range(3):
+            if per[k]:
+                xs[:, k] += L[k] * np.rint(d[:, k] / L[k])
+        return xs
+
+    def linemin(eoriginal):
+        nonlocal f
+        fdothall = float((f * h).sum())
+        if fdothall <= 0.0:
+            return 4, eoriginal
+        hmax = float(np.abs(h).max())
+        if hmax == 0.0:
+            return 6, eoriginal
+        alphamax = min(1.0, dmax / hmax)
+        xs = state["x"].copy()
+        alpha, fhprev, engprev, alphaprev = alphamax, fdothall, eoriginal, 0.0
+        while True:
+            e, f = alpha_step(xs, alpha)
+            xs = image_follow(xs)
+            fh = float((f * h).sum())
+            delfh = fh - fhprev
+            if abs(fh) < EPS_QUAD or abs(delfh) < EPS_QUAD:
+                e, f = alpha_step(xs, 0.0)
+                return 7, e
+            relerr = abs(1.0 - (0.5 * (alpha - alphaprev) * (fh + fhprev) + e) / engprev)
+            a0 = alpha - (alpha - alphaprev) * fh / delfh
+            if relerr <= 0.1 and 0.0 < a0 < alphamax:
+                e, f = alpha_step(xs, a0)
+                xs = image_follow(xs)
+                if e - eoriginal < EMACH:
+                    return 0, e
+            de_ideal = -0.4 * alpha * fdothall
+            if e - eoriginal <= de_ideal:
+                return 0, e
+            fhprev, engprev, alphaprev = fh, e, alpha
+            alpha *= 0.5
+            if alpha <= 0.0 or de_ideal >= -EMACH:
+                e, f = alpha_step(xs, 0.0)
+                return 5, e
+
+    stop, niter, eprev = 0, 0, ecur
+    for _ in range(maxiter):
+        niter += 1
+        eprev = ecur
+        rc, ecur = linemin(ecur)
+        if rc:
+            stop = rc
+            break
+        if state["neval"] >= maxeval:
+            stop = 1
+            break
+        if abs(ecur - eprev) < etol * 0.5 * (abs(ecur) + abs(eprev) + 1.0e-8):
+            stop = 2
+            break
+        dot0, dot1 = float((f * f).sum()), float((f * g).sum())
+        if ftol > 0.0 and fnorm(f) < ftol * ftol:
+            stop = 3
+            break
+        beta = max(0.0, (dot0 - dot1) / gg)
+        if (niter + 1) % nlimit == 0:
+            beta = 0.0
+        gg = dot0
+        g = f.copy()
+        h = g + beta * h
+        if float((g * h).sum()) <= 0.0:
+            h = g.copy()
+    return {"stop": stop, "iterations": niter, "evaluations": state["neval"], "e_initial": e_init, "e_final": ecur,
+            "e_next_to_last": eprev, "fnorm2_final": float(np.sqrt((f * f).sum())), "x": state["x"], "f": f}

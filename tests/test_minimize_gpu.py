@@ -79,3 +79,45 @@ def test_fire_parameter_errors():
             ctx.minimize(0.0, 0.1, 10, tmax=0.01)
         with pytest.raises(capi.LmffError, match="dtgrow"):
             ctx.minimize(0.0, 0.1, 10, dtgrow=0.9)
+
+
+def gpu_cg(s, etol, ftol, maxiter, maxeval, skin=2.0):
+    with capi.Context(0) as ctx:
+        capi.setup_for(ctx, s, skin=skin)
+        sid = ctx.add_system(s)
+        st = ctx.minimize_cg(etol, ftol, maxiter, maxeval)
+        x = ctx.get(sid)[0]
+    return st, x
+
+
+@pytest.mark.parametrize("skin,etol,maxiter", [(2.0, 1e-5, 1000), (0.2, 1e-8, 60)])
+def test_cg_follows_the_oracle(skin, etol, maxiter):
+    """minimize etol 1e-5 1000 10000 as the reference's decks use it; then a 0.2 A skin, so that
+    lists are rebuilt (and atoms wrapped) in the middle of line searches."""
+    import os
+
+    import oracle
+    from conftest import POT
+
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=71, jitter=0.08)
+    o = oracle.Oracle(3, os.path.join(POT, "BNC.tersoff"), s.elements, os.path.join(POT, "BNCH.ILP"), s.elements,
+                      skin=skin, threads=8)
+    o.set_system(s)
+    ref = oracle.cg_minimize(o, s.x, etol, 1e-5, maxiter, 10000, skin=skin)
+    g, x = gpu_cg(s, etol, 1e-5, maxiter, 10000, skin=skin)
+    print("oracle", {k: v for k, v in ref.items() if k not in ("x", "f")}, "gpu", g)
+    assert g["stop"] == ref["stop"]
+    assert g["iterations"] == ref["iterations"] and g["evaluations"] == ref["evaluations"]
+    assert abs(g["e_final"] - ref["e_final"]) < 1e-9 * abs(ref["e_final"])
+    assert g["e_final"] < g["e_initial"]
+    assert wrapped_diff(x, ref["x"], s) < 1e-7
+
+
+def test_cg_needs_one_system():
+    a = S.layered_stack(15, 9, ["G", "BN"], seed=75)
+    with capi.Context(0) as ctx:
+        capi.setup_for(ctx, a)
+        ctx.add_system(a)
+        ctx.add_system(a.copy())
+        with pytest.raises(capi.LmffError, match="exactly one system"):
+            ctx.minimize_cg(1e-5, 1e-5, 10, 100)

@@ -332,6 +332,8 @@ class _Harmonic:
 
     def __init__(self, c, k=2.0):
         self.c, self.k, self.x = np.asarray(c, dtype=float), k, None
+        # box record as Oracle keeps it (type, tag, mol, lo, hi, periodic): an open box
+        self._keep = (None, None, None, np.full(3, -100.0), np.full(3, 100.0), np.zeros(3, dtype=np.int32))
 
     def put_x(self, x):
         self.x = np.array(x, dtype=float)
@@ -392,3 +394,28 @@ def test_fire_restatement_converges_to_the_minimum():
     assert np.abs(r["x"] - c).max() < 1e-8
     r = oracle.fire_minimize(h, c + 0.1, np.full(20, 12.0), 0.001, 1e-12, 0.0, 20000)
     assert r["stop"] == 2 and r["e_final"] < 1e-8
+
+
+def test_cg_restatement_on_harmonic_wells():
+    """On E = k/2 |x - c|^2 the first quadratic line search is exact: from h = f it tries
+    alpha = min(1, dmax / max|h|), the secant model fits (relerr 0), and alpha0 = 1/k lands on
+    c; the next force check stops on ftol after 1 iteration and 2 evaluations.  On an
+    anisotropic well (k per coordinate) Polak-Ribiere CG reaches the minimum."""
+    import oracle
+
+    rng = np.random.default_rng(5)
+    c = rng.normal(size=(10, 3))
+    h = _Harmonic(c, 5.0)
+    r = oracle.cg_minimize(h, c + 0.03 * rng.uniform(-1, 1, size=c.shape), 0.0, 1e-10, 100, 1000)
+    assert (r["stop"], r["iterations"], r["evaluations"]) == (3, 1, 2)
+    assert np.abs(r["x"] - c).max() < 1e-13
+
+    class Aniso(_Harmonic):
+        def compute(self, eflag=1, vflag=0):
+            d = self.x - self.c
+            kk = np.array([1.0, 4.0, 9.0])
+            return {"f": -kk * d, "energy": 0.5 * float((kk * d * d).sum())}
+
+    a = Aniso(c)
+    r = oracle.cg_minimize(a, c + 0.05 * rng.uniform(-1, 1, size=c.shape), 0.0, 1e-9, 200, 10000)
+    assert r["stop"] == 3 and np.abs(r["x"] - c).max() < 1e-8

@@ -114,3 +114,22 @@ def test_minimize_from_a_deck(tmp_path):
     r = S.read_lammps_data(str(tmp_path / "relaxed.data"))
     assert open(tmp_path / "relaxed.data").readline().strip().endswith(f"timestep = {st['iterations']}")
     assert r.n == s.n
+
+
+def test_default_cg_minimize_from_a_deck(tmp_path):
+    """minimize 1.0e-5 1.0e-5 1000 10000 with the default min_style (cg), as in the reference's
+    Cases decks: the log reports the library's CG run."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=71, jitter=0.08)
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    out = run_deck(tmp_path, "c.in", deck(data, "", "minimize 1.0e-5 1.0e-5 1000 10000"))
+    lines = out.splitlines()
+    k = lines.index("Minimization stats:")
+    with capi.Context(0) as ctx:
+        capi.setup_for(ctx, s)
+        ctx.add_system(s)
+        st = ctx.minimize_cg(1e-5, 1e-5, 1000, 10000)
+    assert lines[k + 1].strip() == "Stopping criterion = " + st["stop_text"]
+    assert [int(t) for t in lines[k + 6].split("=")[1].split()] == [st["iterations"], st["evaluations"]]

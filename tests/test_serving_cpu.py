@@ -2,6 +2,7 @@
 admission by HBM budget in arrival order, retirement at tick boundaries, one packed run per
 tick covering every resident job, and latency bookkeeping."""
 import numpy as np
+import pytest
 
 from lmff import serving, structures as S
 
@@ -93,3 +94,31 @@ def test_job_stream_is_seeded_and_in_range():
     for j in a:
         m, n = j.system.meta["m"], j.system.meta["n"]
         assert 1 <= n < m <= 12
+
+
+def test_cancel_queued_and_running_jobs_and_snapshots():
+    """on_tick hook: after the first tick, cancel one running and one queued job and take a
+    snapshot of another running one; cancelled jobs free their slots and are not counted."""
+    fake = FakeCtx(bytes_per_atom=1000.0)
+    jobs = jobs_of([1080] * 5, [40, 40, 40, 20, 20], [0.0] * 5)
+    srv = serving.Server(ctx=fake, hbm_budget_bytes=1080 * 1000 * 2.5, tick=10)   # two at a time
+    snaps = []
+
+    def on_tick(server, now):
+        if len(server.log) == 1:
+            running = sorted(j.job_id for j in server.resident.values())
+            assert running == [0, 1]
+            server.cancel(1)        # running
+            server.cancel(3)        # still queued
+            snaps.append(server.snapshot(0))
+            with pytest.raises(KeyError):
+                server.snapshot(2)
+
+    st = srv.run(jobs, clock=Clock(), on_tick=on_tick)
+    status = {j.job_id: j.status for j in jobs}
+    assert status == {0: "done", 1: "cancelled", 2: "done", 3: "cancelled", 4: "done"}
+    assert st["jobs"] == 3 and st["cancelled"] == 2
+    assert jobs[1].done_steps == 10 and jobs[3].done_steps == 0 and jobs[1].result is None
+    assert snaps[0]["step"] == 10 and snaps[0]["x"].shape == (1080, 3)
+    assert not fake.systems and srv.reserved == 0.0
+    assert st["atom_steps"] == sum(j.natoms * j.done_steps for j in jobs)
