@@ -37,6 +37,11 @@ FP32_VALU_PEAK = 157.3  # TFLOP/s, MI355X FP32 vector
 ILP_PAIR_FLOP = 120     # algorithmic FLOP per ILP ordered pair (exp, sqrt, divide count 1 each)
 
 
+def log(msg):
+    """Progress on stderr (the result line is the only stdout output)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def build_ctx(s, precision):
     ctx = capi.Context(0)
     capi.setup_for(ctx, s, precision=precision, skin=2.0, every=10, delay=0, check=0)
@@ -140,15 +145,19 @@ def _cpu_model():
 
 def md_compare(s, steps, warmup):
     """FP64 and mixed contexts on the same input: single-shot forces, then timed resident runs."""
+    log(f"{s.n} atoms: single-shot Pair::compute, FP64 and mixed")
     r64 = one_shot(s, capi.LMFF_FP64, repeat=4)
     rmx = one_shot(s, capi.LMFF_MIXED)
+    log(f"FP64 resident run: {warmup} warmup + {steps} timed steps")
     ctx, sid = build_ctx(s, capi.LMFF_FP64)
     stats = ctx.pair_stats()
     res64 = timed_run(ctx, sid, s, steps, warmup, profile=True)
     ctx.close()
+    log(f"mixed resident run: {warmup} warmup + {steps} timed steps")
     ctxm, sidm = build_ctx(s, capi.LMFF_MIXED)
     resmx = timed_run(ctxm, sidm, s, steps, warmup, profile=True)
     ctxm.close()
+    log("runs done")
     df = rmx["f"] - r64["f"]
     mixed_err = {"max_abs": float(np.abs(df).max()), "rms": float(np.sqrt((df ** 2).sum(1).mean()))}
     return r64, rmx, stats, res64, resmx, mixed_err
@@ -215,6 +224,7 @@ def main():
     cpu = None
     oracle_err = None
     if not args.no_cpu:
+        log("CPU baseline (oracle)")
         cpu, fref = cpu_baseline(s)
         d = r64["f"] - fref["f"]
         rmsf = np.sqrt((fref["f"] ** 2).sum(1).mean())
@@ -325,8 +335,11 @@ def serving_bench(args):
         seq_srv.close()
         return packed, seq
 
+    log(f"capacity: {len(jobs)} jobs packed, then one at a time")
     packed, seq = capacity(jobs)
+    log(f"capacity: {len(small)} minimal-cell jobs")
     packed_s, seq_s = capacity(small)
+    log("latency run")
     # latency at an offered load of about half the packed capacity
     rate = args.rate or 0.5 * packed["jobs"] / packed["makespan_s"]
     srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=10)
