@@ -192,7 +192,8 @@ int lmff_profile_read(lmff_ctx *ctx, double *ms_total, double *ms_per_family, in
 
 /* HBM accounting: bytes currently held and the per-atom estimate used for admission. */
 int lmff_memory(lmff_ctx *ctx, int64_t *bytes_in_use, double *bytes_per_atom_estimate);
-/* Bytes a new system of natoms would add (lists sized from the current density estimate). */
+/* Bytes a new system of natoms would add: every per-atom array, with list rows at the
+ * context's current capacities or, if larger, what a stack of nlayers_hint sheets needs. */
 int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint);
 
 /* ILP list of one atom (caller order index) as sorted (caller-order j, sx, sy, sz) rows, for

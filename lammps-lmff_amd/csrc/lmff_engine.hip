@@ -179,7 +179,8 @@ struct Ctx {
                    &slist, &scnt, &ilist, &icnt, &ocnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
-                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag};
+                   &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
+                   &cg_x0, &cg_g, &cg_h};
     for (DBuf *b : all) b->tally = &bytes_in_use;
   }
   ~Ctx() {
@@ -410,6 +411,35 @@ struct Ctx {
   static void adopt(DBuf &dst, DBuf &src) {
     std::swap(dst.p, src.p);
     std::swap(dst.bytes, src.bytes);
+  }
+
+  // Bytes per atom of everything alloc_atoms() sizes (term for term), plus the radix sort's
+  // alternate key/value buffers and list rows of scap + icap entries.
+  static double atom_bytes(int scap_, int icap_) {
+    const double d4 = 9 * 32.0;                                     // x v f xhold tmp4 nrm ft fil S
+    const double i32 = 15 * 4.0;                                    // mol tag sysid orig tmpi key key2 val
+                                                                    // perm scnt icnt ocnt tcnt rtcnt rncnt
+    const double f64 = 8.0 * (27 + 9 + 9 + 6 + NTH + 6 + MAXSH + 3 * MAXSH * MAXSH);  // dn dndri vet T th
+                                                                    // vatom eb fb
+    const double u32 = 4.0 * (4 + 3 * MAXSH);                       // nnb tsh rtc rnc
+    const double sort = 8.0;
+    return d4 + i32 + f64 + u32 + sort + 4.0 * ((double)scap_ + (double)icap_);
+  }
+  // ILP row capacity the list build settles on for the worst atom of an AB-stacked pile of
+  // nlayers sheets 3.2 A apart with at most 0.39 atoms/A^2 per sheet (graphene 0.382, hBN
+  // 0.370): the atoms of the other sheets within cutneigh, grown the way build_lists() grows.
+  static int ilp_row_estimate(double cutneigh, int nlayers) {
+    const double rho = 0.39, gap = 3.2, pi = 3.141592653589793;
+    double worst = 0.0;
+    for (int p = 0; p < nlayers; p++) {
+      double cnt = 0.0;
+      for (int q = 0; q < nlayers; q++) {
+        const double dz = gap * (p - q);
+        if (q != p && dz * dz < cutneigh * cutneigh) cnt += pi * (cutneigh * cutneigh - dz * dz) * rho;
+      }
+      worst = std::max(worst, cnt);
+    }
+    return ((int)(worst * 1.15) + 64 + 63) / 64 * 64;
   }
 
   void alloc_atoms(size_t m) {
@@ -1748,22 +1778,17 @@ int lmff_memory(lmff_ctx *ctx, int64_t *bytes, double *per_atom) {
   return guard(ctx, [&] {
     auto &c = ctx->c;
     if (bytes) *bytes = (int64_t)c.bytes_in_use;
-    if (per_atom) {
-      const double fixed = 5 * 32 + 11 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + MAXSH * 8 + MAXSH * MAXSH * 24 + 64 + 48 +
-                           NTH * 8 + 48 + 32;
-      *per_atom = fixed + 4.0 * (c.scap + std::max(c.icap, 448));
-    }
+    if (per_atom) *per_atom = Ctx::atom_bytes(c.scap, std::max(c.icap, 64));
   });
 }
 
 int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint) {
   if (!ctx) return -1;
   auto &c = ctx->c;
-  const double fixed = 5 * 32 + 11 * 4 + 27 * 8 + 9 * 8 * 2 + 16 + MAXSH * 4 + 4 + MAXSH * 8 + MAXSH * MAXSH * 24 + 64 + 48 +
-                       NTH * 8 + 48 + 32;
-  const int layers = std::max(nlayers_hint, 2);
-  const double list = 4.0 * (c.scap + (c.icap > 0 ? c.icap : 448 * std::min(layers - 1, 5)));
-  return (int64_t)(natoms * (fixed + list));
+  // List rows are shared by every packed system, so a new system pays the context's current
+  // capacities, or what its own stack needs if that is more.
+  const int icap = std::max(c.icap, Ctx::ilp_row_estimate(c.cutneigh, std::max(nlayers_hint, 2)));
+  return (int64_t)std::ceil((double)natoms * Ctx::atom_bytes(std::max(c.scap, 64), icap));
 }
 
 int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, int capacity, int *nout) {

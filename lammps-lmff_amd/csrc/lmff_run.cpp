@@ -1,8 +1,8 @@
 // lmff_run: runs the subset of a LAMMPS input deck that drives the LMFF force path on one
 // MI355X, so an unchanged deck of the reference's form
-//   units metal / atom_style full / boundary / read_data / mass / pair_style hybrid/overlay
-//   tersoff ilp/graphene/hbn / pair_coeff / neighbor / neigh_modify / timestep / velocity /
-//   fix nve | fix langevin / thermo / run
+//   units metal / atom_style full / boundary / variable (index, equal) / read_data / mass /
+//   replicate / pair_style hybrid/overlay tersoff|lmff ilp/graphene/hbn / pair_coeff /
+//   neighbor / neigh_modify / timestep / velocity / fix nve | fix langevin / thermo / run
 // executes with the force field, the neighbor build and the integrator on the GPU.
 // Output follows LAMMPS' log format (thermo table, "Loop time", "Performance:" as in
 // LMFF/finish.cpp:124,162).  Also: compute pe/atom and stress/atom, dump custom (text,
@@ -20,6 +20,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
+#include <cctype>
 #include <iostream>
 #include <map>
 #include <random>
@@ -48,6 +50,7 @@ struct Data {
   int ntypes = 0;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   std::vector<double> x, v;
+  std::vector<int> img;  // image flags (3 per atom) from the Atoms section, 0 when absent
   std::vector<int32_t> type;
   std::vector<int64_t> tag, mol;
   std::map<int, double> mass;
@@ -101,6 +104,7 @@ Data read_data(const std::string &path) {
       d.mol.push_back(std::stoll(w[1]));
       d.type.push_back(std::stoi(w[2]));
       for (int k = 0; k < 3; k++) d.x.push_back(std::stod(w[4 + k]));
+      for (int k = 0; k < 3; k++) d.img.push_back(w.size() >= 10 ? std::stoi(w[7 + k]) : 0);
     } else if (section == "Velocities") {
       vel_id.push_back(std::stoll(w[0]));
       vel.push_back({std::stod(w[1]), std::stod(w[2]), std::stod(w[3])});
@@ -115,6 +119,168 @@ Data read_data(const std::string &path) {
   }
   return d;
 }
+
+// replicate nx ny nz (Replicate::command, LMFF/replicate.cpp): atoms are unwrapped with their
+// image flags (137), the box grows to n times each side (212-214), copies are made in ix, iy, iz
+// loop order (384-386), shifted by whole old box lengths, remapped into the new box (533-544),
+// and renumbered with id + (iz*ny*nx + iy*nx + ix)*maxtag and, for molecule ids > 0,
+// mol + (iz*ny*nx + iy*nx + ix)*maxmol (556-571; maxtag/maxmol at 83-100).
+void replicate(Data &d, int nx, int ny, int nz, const int periodic[3]) {
+  if (nx < 1 || ny < 1 || nz < 1) die("Illegal replicate command");
+  const int nrep = nx * ny * nz;
+  const double prd[3] = {d.hi[0] - d.lo[0], d.hi[1] - d.lo[1], d.hi[2] - d.lo[2]};
+  int64_t maxtag = 0, maxmol = 0;
+  for (int64_t i = 0; i < d.n; i++) {
+    maxtag = std::max(maxtag, d.tag[i]);
+    maxmol = std::max(maxmol, d.mol[i]);
+  }
+  if (maxtag + (int64_t)(nrep - 1) * d.n >= ((int64_t)1 << 53)) die("Replicated system atom IDs are too big");
+  Data r = d;
+  r.x.clear(); r.v.clear(); r.img.clear(); r.type.clear(); r.tag.clear(); r.mol.clear();
+  const int nrep_k[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; k++) r.hi[k] = d.lo[k] + nrep_k[k] * prd[k];
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++) {
+        const int64_t rep = (int64_t)iz * ny * nx + (int64_t)iy * nx + ix;
+        const int shift[3] = {ix, iy, iz};
+        for (int64_t i = 0; i < d.n; i++) {
+          for (int k = 0; k < 3; k++) {
+            const int im0 = d.img.size() == 3 * (size_t)d.n ? d.img[3 * i + k] : 0;
+            double xk = d.x[3 * i + k] + im0 * prd[k] + shift[k] * prd[k];
+            int im = 0;
+            if (periodic[k]) {
+              const double L = r.hi[k] - r.lo[k];
+              while (xk < r.lo[k]) { xk += L; im--; }
+              while (xk >= r.hi[k]) { xk -= L; im++; }
+            }
+            r.x.push_back(xk);
+            r.img.push_back(im);
+            r.v.push_back(d.v[3 * i + k]);
+          }
+          r.type.push_back(d.type[i]);
+          r.tag.push_back(d.tag[i] + rep * maxtag);
+          r.mol.push_back(d.mol[i] > 0 ? d.mol[i] + rep * maxmol : d.mol[i]);
+        }
+      }
+  r.n = d.n * nrep;
+  d = std::move(r);
+}
+
+// Equal-style variable formulas (Variable::evaluate, LMFF/variable.cpp:1185): numbers, PI,
+// v_name, + - * / % ^ with LAMMPS' precedence (variable.cpp:108-116: unary minus binds
+// tighter than ^, and ^ groups left to right), parentheses and the math functions sqrt exp
+// ln log abs sin cos tan asin acos atan atan2 ceil floor round.
+struct Formula {
+  const std::string &s;
+  std::function<double(const std::string &)> var;
+  size_t p = 0;
+  void skip() { while (p < s.size() && std::isspace((unsigned char)s[p])) p++; }
+  [[noreturn]] void bad() { die("Invalid syntax in variable formula: " + s); }
+  double add() {
+    double a = mul();
+    for (skip(); p < s.size() && (s[p] == '+' || s[p] == '-'); skip()) {
+      const char op = s[p++];
+      const double b = mul();
+      a = op == '+' ? a + b : a - b;
+    }
+    return a;
+  }
+  double mul() {
+    double a = pw();
+    for (skip(); p < s.size() && (s[p] == '*' || s[p] == '/' || s[p] == '%'); skip()) {
+      const char op = s[p++];
+      const double b = pw();
+      if (op == '*') a *= b;
+      else if (b == 0.0) die("Divide by 0 in variable formula");
+      else a = op == '/' ? a / b : std::fmod(a, b);
+    }
+    return a;
+  }
+  double pw() {
+    double a = unary();
+    for (skip(); p < s.size() && s[p] == '^'; skip()) {
+      p++;
+      const double b = unary();
+      if (b == 0.0) a = 1.0;
+      else if (a == 0.0 && b < 0.0) die("Invalid power expression in variable formula");
+      else a = std::pow(a, b);
+    }
+    return a;
+  }
+  double unary() {
+    skip();
+    if (p < s.size() && s[p] == '-') { p++; return -unary(); }
+    if (p < s.size() && s[p] == '+') { p++; return unary(); }
+    return primary();
+  }
+  double primary() {
+    skip();
+    if (p >= s.size()) bad();
+    if (s[p] == '(') {
+      p++;
+      const double a = add();
+      skip();
+      if (p >= s.size() || s[p] != ')') bad();
+      p++;
+      return a;
+    }
+    if (std::isdigit((unsigned char)s[p]) || s[p] == '.') {
+      size_t q = p;
+      while (q < s.size() && (std::isdigit((unsigned char)s[q]) || s[q] == '.')) q++;
+      if (q < s.size() && (s[q] == 'e' || s[q] == 'E')) {
+        q++;
+        if (q < s.size() && (s[q] == '+' || s[q] == '-')) q++;
+        while (q < s.size() && std::isdigit((unsigned char)s[q])) q++;
+      }
+      const double a = std::stod(s.substr(p, q - p));
+      p = q;
+      return a;
+    }
+    size_t q = p;
+    while (q < s.size() && (std::isalnum((unsigned char)s[q]) || s[q] == '_')) q++;
+    if (q == p) bad();
+    const std::string word = s.substr(p, q - p);
+    p = q;
+    if (word.compare(0, 2, "v_") == 0) return var(word.substr(2));
+    if (word == "PI") return 3.14159265358979323846;
+    skip();
+    if (p >= s.size() || s[p] != '(') die("Invalid thermo keyword '" + word + "' in variable formula");
+    p++;
+    const double a = add();
+    skip();
+    double b = 0.0;
+    const bool two = p < s.size() && s[p] == ',';
+    if (two) { p++; b = add(); skip(); }
+    if (p >= s.size() || s[p] != ')') bad();
+    p++;
+    if (two && word == "atan2") return std::atan2(a, b);
+    if (two) die("Invalid math function in variable formula: " + word);
+    if (word == "sqrt") { if (a < 0.0) die("Sqrt of negative value in variable formula"); return std::sqrt(a); }
+    if (word == "exp") return std::exp(a);
+    if (word == "ln" || word == "log") {
+      if (a <= 0.0) die("Log of zero/negative value in variable formula");
+      return word == "ln" ? std::log(a) : std::log10(a);
+    }
+    if (word == "abs") return std::fabs(a);
+    if (word == "sin") return std::sin(a);
+    if (word == "cos") return std::cos(a);
+    if (word == "tan") return std::tan(a);
+    if (word == "asin") return std::asin(a);
+    if (word == "acos") return std::acos(a);
+    if (word == "atan") return std::atan(a);
+    if (word == "ceil") return std::ceil(a);
+    if (word == "floor") return std::floor(a);
+    if (word == "round") return std::round(a);
+    die("Invalid math function in variable formula: " + word);
+  }
+  double eval() {
+    const double a = add();
+    skip();
+    if (p != s.size()) bad();
+    return a;
+  }
+};
 
 // write_data: header (WriteData::header), Masses, Atoms # full (AtomVecFull::write_data; image
 // flags 0: positions are kept wrapped), Velocities (AtomVec::write_vel)
@@ -237,13 +403,34 @@ int main(int argc, char **argv) {
   lmff_fire_params fire;
   lmff_fire_defaults(&fire);
 
+  std::map<std::string, std::string> formulas;  // equal-style variables
+  int depth = 0;
+  std::function<double(const std::string &)> value = [&](const std::string &name) -> double {
+    if (vars.count(name)) return std::stod(vars[name]);
+    if (!formulas.count(name)) die("Invalid variable name in variable formula: " + name);
+    if (++depth > 64) die("Variable formula refers to itself: " + name);
+    const double a = Formula{formulas[name], value}.eval();
+    depth--;
+    return a;
+  };
+  // ${name}: an index-style value as written, an equal-style one as its formula's value in
+  // "%.15g" (Variable::retrieve, LMFF/variable.cpp:871)
   auto subst = [&](std::string s) {
     for (size_t p; (p = s.find("${")) != std::string::npos;) {
       size_t q = s.find('}', p);
       if (q == std::string::npos) die("Invalid variable name in " + s);
       std::string name = s.substr(p + 2, q - p - 2);
-      if (!vars.count(name)) die("Substitution for illegal variable " + name);
-      s = s.substr(0, p) + vars[name] + s.substr(q + 1);
+      std::string val;
+      if (vars.count(name)) {
+        val = vars[name];
+      } else if (formulas.count(name)) {
+        char b[64];
+        std::snprintf(b, sizeof b, "%.15g", value(name));
+        val = b;
+      } else {
+        die("Substitution for illegal variable " + name);
+      }
+      s = s.substr(0, p) + val + s.substr(q + 1);
     }
     return s;
   };
@@ -293,7 +480,33 @@ int main(int argc, char **argv) {
         bstr[k] = w[k + 1];
       }
     } else if (cmd == "variable") {
-      if (w.size() >= 4 && !vars.count(w[1])) vars[w[1]] = w[3];
+      // index: the first definition (or -var on the command line) wins; equal: redefinable
+      if (w.size() < 4) die("Illegal variable command");
+      if (w[2] == "equal") {
+        if (vars.count(w[1])) die("Cannot redefine variable as a different style");
+        std::string f;
+        for (size_t k = 3; k < w.size(); k++) f += w[k];
+        if (f.size() >= 2 && (f.front() == '"' || f.front() == '\'') && f.back() == f.front())
+          f = f.substr(1, f.size() - 2);
+        formulas[w[1]] = f;
+      } else if (w[2] == "index") {
+        if (!vars.count(w[1])) vars[w[1]] = w[3];
+      } else {
+        die("lmff_run supports variable styles index and equal");
+      }
+    } else if (cmd == "print") {
+      // print "text" (Input::print): the substituted text, quotes removed
+      std::string t = line.substr(line.find("print") + 5);
+      const size_t a = t.find_first_not_of(" \t"), b = t.find_last_not_of(" \t\r");
+      t = a == std::string::npos ? "" : t.substr(a, b - a + 1);
+      if (t.size() >= 2 && (t.front() == '"' || t.front() == '\'') && t.back() == t.front()) t = t.substr(1, t.size() - 2);
+      std::printf("%s\n", t.c_str());
+    } else if (cmd == "replicate") {
+      if (!have_data) die("Replicate command before simulation box is defined");
+      if (sys >= 0) die("lmff_run supports replicate only before the first run or minimize");
+      if (w.size() < 4) die("Illegal replicate command");
+      replicate(data, std::stoi(w[1]), std::stoi(w[2]), std::stoi(w[3]), periodic);
+      std::printf("  %lld atoms after replicate\n", (long long)data.n);
     } else if (cmd == "read_data") {
       data = read_data(w.at(1));
       have_data = true;
@@ -301,8 +514,9 @@ int main(int argc, char **argv) {
     } else if (cmd == "mass") {
       data.mass[std::stoi(w.at(1))] = std::stod(w.at(2));
     } else if (cmd == "pair_style") {
-      if (w.size() < 4 || w[1] != "hybrid/overlay" || w[2] != "tersoff" || w[3] != "ilp/graphene/hbn")
-        die("lmff_run supports pair_style hybrid/overlay tersoff ilp/graphene/hbn <cut> [tap]");
+      if (w.size() < 5 || w[1] != "hybrid/overlay" || (w[2] != "tersoff" && w[2] != "lmff") ||
+          w[3] != "ilp/graphene/hbn" || w.size() > 6)
+        die("lmff_run supports pair_style hybrid/overlay tersoff|lmff ilp/graphene/hbn <cut> [tap]");
       pair_args.assign(w.begin() + 4, w.end());
     } else if (cmd == "pair_coeff") {
       coeffs.push_back(std::vector<std::string>(w.begin() + 1, w.end()));
@@ -519,7 +733,7 @@ int main(int argc, char **argv) {
       std::printf("  Force two-norm initial, final = %g %g\n", ms.fnorm2_initial, ms.fnorm2_final);
       std::printf("  Force max component initial, final = %g %g\n", ms.fnorminf_initial, ms.fnorminf_final);
       std::printf("  Iterations, force evaluations = %lld %lld\n\n", (long long)ms.iterations, (long long)ms.evaluations);
-    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "replicate" || cmd == "unfix") {
+    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "unfix") {
       std::printf("WARNING: %s ignored by lmff_run\n", cmd.c_str());
     } else {
       die("Unknown command: " + line);

@@ -31,7 +31,9 @@ void PairLMFFHip::settings(int narg, char **arg) {
   have_settings_ = true;
 }
 
-// PairTersoff::coeff (pair_tersoff.cpp:293-358) / PairILPGrapheneHBN::coeff (ilp.cpp:155-219)
+// PairTersoff::coeff (pair_tersoff.cpp:293-358), PairLMFF::coeff (LMFF/pair_lmff.cpp:2432-2497)
+// and PairILPGrapheneHBN::coeff (ilp.cpp:155-219).  "lmff" takes a Tersoff file like
+// "tersoff" does; the two deck forms differ only in which atoms ILP sees (apply_ilp).
 void PairLMFFHip::coeff(int narg, char **arg) {
   if (narg != 4 + ntypes_) throw PairError(LMFF_ERR_ARG, "Incorrect args for pair coefficients");
   if (std::strcmp(arg[0], "*") != 0 || std::strcmp(arg[1], "*") != 0)
@@ -40,7 +42,10 @@ void PairLMFFHip::coeff(int narg, char **arg) {
   const std::string style = arg[2];
   const char *file = arg[3];
   std::vector<const char *> elems(arg + 4, arg + 4 + ntypes_);
-  if (style == "tersoff") {
+  if (style == "tersoff" || style == "lmff") {
+    const int form = style == "tersoff" ? 1 : 2;
+    if (form_ != 0 && form_ != form)
+      throw PairError(LMFF_ERR_ARG, "Pair coeff for hybrid has invalid style: " + style);
     check(lmff_coeff_tersoff(ctx_, file, elems.data()));
     TersoffTable t;
     std::string err;
@@ -48,12 +53,32 @@ void PairLMFFHip::coeff(int narg, char **arg) {
     read_tersoff(file, t, err);
     tersoff_cutmax_ = t.cutmax;
     have_tersoff_ = true;
+    form_ = form;
+    lmff_null_.assign((size_t)ntypes_ + 1, false);
+    if (form == 2)
+      for (int t = 1; t <= ntypes_; t++) lmff_null_[t] = std::strcmp(elems[t - 1], "NULL") == 0;
+    if (have_ilp_) apply_ilp();
   } else if (style == "ilp/graphene/hbn") {
-    check(lmff_coeff_ilp(ctx_, file, elems.data()));
+    ilp_file_ = file;
+    ilp_elems_.assign(elems.begin(), elems.end());
+    apply_ilp();
     have_ilp_ = true;
   } else {
     throw PairError(LMFF_ERR_ARG, "Pair coeff for hybrid has invalid style: " + style);
   }
+}
+
+// In the lmff form the fused style computes ILP itself, over the neighbor list PairHybrid
+// hands the lmff sub-style; that list skips every type the lmff pair_coeff maps to NULL
+// (LMFF/pair_hybrid.cpp:571-619), while the ilp/graphene/hbn sub-style only loads parameters.
+// So ILP leaves those types out as well.  In the stock form ILP keeps its own mapping.
+void PairLMFFHip::apply_ilp() {
+  std::vector<const char *> el;
+  for (int t = 1; t <= ntypes_; t++) {
+    const bool skip = form_ == 2 && t < (int)lmff_null_.size() && lmff_null_[t];
+    el.push_back(skip ? "NULL" : ilp_elems_[t - 1].c_str());
+  }
+  check(lmff_coeff_ilp(ctx_, ilp_file_.c_str(), el.data()));
 }
 
 void PairLMFFHip::init_style(double skin, int every, int delay, int check_yes, const std::vector<double> &mass) {

@@ -1,5 +1,6 @@
-// PairLMFFHip: host C++ adaptor with the LAMMPS Pair surface for
-//   pair_style hybrid/overlay tersoff ilp/graphene/hbn <cut> [tap_flag]
+// PairLMFFHip: host C++ adaptor with the LAMMPS Pair surface for either deck form
+//   pair_style hybrid/overlay tersoff ilp/graphene/hbn <cut> [tap_flag]   (stock styles)
+//   pair_style hybrid/overlay lmff ilp/graphene/hbn <cut> [tap_flag]      (LMFF fused style)
 // over the C ABI of include/lmff_hip.h.
 //
 // Mirrors the reference interface (LMFF/pair.h:154-181): settings(narg, arg), coeff(narg, arg),
@@ -41,9 +42,9 @@ class PairLMFFHip {
   PairLMFFHip(const PairLMFFHip &) = delete;
   PairLMFFHip &operator=(const PairLMFFHip &) = delete;
 
-  // pair_style hybrid/overlay tersoff ilp/graphene/hbn <args>: args after "ilp/graphene/hbn"
+  // pair_style hybrid/overlay tersoff|lmff ilp/graphene/hbn <args>: args after "ilp/graphene/hbn"
   void settings(int narg, char **arg);
-  // pair_coeff * * tersoff <file> e1..en   |   pair_coeff * * ilp/graphene/hbn <file> e1..en
+  // pair_coeff * * tersoff|lmff <file> e1..en   |   pair_coeff * * ilp/graphene/hbn <file> e1..en
   void coeff(int narg, char **arg);
   // neighbor/neigh_modify/mass state the reference reads in init
   void init_style(double skin, int every, int delay, int check, const std::vector<double> &mass);
@@ -63,7 +64,12 @@ class PairLMFFHip {
 
  private:
   void check(int rc) const;
+  void apply_ilp();
   lmff_ctx *ctx_ = nullptr;
+  int form_ = 0;                         // 0 not yet known, 1 tersoff, 2 lmff
+  std::vector<bool> lmff_null_;          // types the lmff sub-style maps to NULL
+  std::string ilp_file_;
+  std::vector<std::string> ilp_elems_;
   int ntypes_;
   double cut_global_ = 16.0, tersoff_cutmax_ = 0.0, skin_ = 2.0;
   bool have_settings_ = false, have_tersoff_ = false, have_ilp_ = false;

@@ -32,7 +32,7 @@ def oracle_for(s, tap=1, telems=None, ielems=None, threads=8):
         else:
             telems, ielems, tf = list(s.elements), list(s.elements), BNC
     else:
-        tf = CT if telems[0] == "C(O)" else BNC
+        tf = CT if "C(O)" in telems else BNC
     o = oracle.Oracle(len(telems), tf, telems, ILP, ielems, tap_flag=tap, threads=threads)
     o.set_system(s)
     o.build_list()
@@ -45,7 +45,7 @@ def ctx_for(s, precision=capi.LMFF_FP64, tap=1, telems=None, ielems=None, masses
         capi.setup_for(ctx, s, precision=precision, tap=tap)
     else:
         capi.setup_graphene(ctx, len(telems), tuple(telems), tuple(ielems),
-                            "C.opt.tersoff" if telems[0] == "C(O)" else "BNC.tersoff",
+                            "C.opt.tersoff" if "C(O)" in telems else "BNC.tersoff",
                             tuple(masses), tap=tap, precision=precision)
     return ctx
 
@@ -91,11 +91,24 @@ def test_fp64_forces_energy_virial(name):
     assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
 
 
-@pytest.mark.parametrize("tap", [0, 1])
-def test_fp64_reference_six_layer_deck(tap):
-    """The reference's own 5,016-atom six-layer graphene data file (periodic in x, y and z)."""
+SIX_LAYER_MAPS = {
+    # every layer in both sub-styles
+    "all": (["C(O)"] * 6, ["C"] * 6),
+    # Cases/case_5016/in.replicate.ilp_tersoff: the bottom layer (type 1) has no Tersoff
+    # but full ILP
+    "stock_null": (["NULL"] + ["C(O)"] * 5, ["C"] * 6),
+    # Cases/case_5016/in.replicate (pair_style lmff): the fused style's skip list drops
+    # type 1 from ILP too, so the effective ILP map is NULL for it
+    "lmff_null": (["NULL"] + ["C(O)"] * 5, ["NULL"] + ["C"] * 5),
+}
+
+
+@pytest.mark.parametrize("tap,maps", [(0, "all"), (1, "all"), (0, "stock_null"), (0, "lmff_null")])
+def test_fp64_reference_six_layer_deck(tap, maps):
+    """The reference's own 5,016-atom six-layer graphene data file (periodic in x, y and z),
+    with the element maps of its decks."""
     s = reference_stack()
-    el_t, el_i = ["C(O)"] * 6, ["C"] * 6
+    el_t, el_i = SIX_LAYER_MAPS[maps]
     r = oracle_for(s, tap=tap, telems=el_t, ielems=el_i).compute(3, 6)
     with ctx_for(s, tap=tap, telems=el_t, ielems=el_i, masses=[12.0107] * 6) as ctx:
         g = ctx.pair_compute(s, 3, 6)
@@ -336,3 +349,34 @@ def test_full_size_c2_properties():
     rms_err = math.sqrt(((m["f"] - g["f"]) ** 2).sum(1).mean())
     assert rms_err <= 1e-5
     assert abs(m["energy"] - g["energy"]) <= 1e-6 * abs(g["energy"])
+
+
+def test_pair_compute_list_reuse_follows_the_oracle():
+    """LAMMPS' loop through the Pair boundary: positions uploaded every step, the list built
+    once (new_list=1) and reused (new_list=0) for 30 steps at 3000 K, so atoms move past the ILP
+    rows' split margin and the far flag must be raised from the uploaded positions.  Velocity
+    Verlet on the host; positions and energies follow the oracle's run with the same schedule."""
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=81, temperature=3000.0)
+    o = oracle_for(s)
+    x_ref, v_ref, th = o.run_nve(s.x, s.v, s.mass_per_type, 0.001, 30, rebuild_every=30, thermo_every=10)
+    dt, ftm2v = 0.001, 1.0 / 1.0364269e-4
+    m = s.masses()[:, None]
+    x, v = s.x.copy(), s.v.copy()
+    pe = []
+    with ctx_for(s) as ctx:
+        f = ctx.pair_compute(s, 1, 0, new_list=1, x=x)["f"]
+        for step in range(1, 31):
+            v += 0.5 * dt * ftm2v / m * f
+            x += dt * v
+            r = ctx.pair_compute(s, 1, 0, new_list=0, x=x)
+            f = r["f"]
+            v += 0.5 * dt * ftm2v / m * f
+            if step % 10 == 0:
+                pe.append(r["energy"])
+    L = s.boxhi - s.boxlo
+    d = x - x_ref
+    d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
+    disp = x - s.x
+    assert np.sqrt((disp ** 2).sum(1)).max() > 0.5
+    assert np.abs(d).max() < 1e-8
+    np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)

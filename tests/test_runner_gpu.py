@@ -2,13 +2,14 @@
 stress/atom, write_data, and a restart from that file with reset_timestep.  The dumped
 per-atom columns are checked against Pair::compute (lmff_pair_compute) on the dumped
 positions; the restarted run must follow the uninterrupted one."""
+import math
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import POT, REPO
+from conftest import GOLDEN, POT, REPO
 from lmff import capi, structures as S
 
 pytestmark = pytest.mark.gpu
@@ -133,3 +134,69 @@ def test_default_cg_minimize_from_a_deck(tmp_path):
         st = ctx.minimize_cg(1e-5, 1e-5, 1000, 10000)
     assert lines[k + 1].strip() == "Stopping criterion = " + st["stop_text"]
     assert [int(t) for t in lines[k + 6].split("=")[1].split()] == [st["iterations"], st["evaluations"]]
+
+
+def six_layer_deck(pair_lines, extra=""):
+    return "\n".join([
+        "units metal", "atom_style full", "boundary p p p",
+        f"read_data {os.path.join(GOLDEN, 'inf0.data')}", extra,
+        *pair_lines, "neighbor 2.0 bin", "neigh_modify every 10 delay 0 check no", "timestep 0.001",
+        "fix 1 all nve", "thermo 10", "run 0", ""])
+
+
+def step0_epair(out):
+    return float([l for l in out.splitlines() if l.strip().startswith("0 ")][0].split()[2])
+
+
+def printed_close(printed, value):
+    """The thermo table prints E_pair with 10 significant digits (%16.10g)."""
+    ulp10 = 10.0 ** (math.floor(math.log10(abs(value))) - 9)
+    return abs(printed - value) <= 0.5 * ulp10 + 1e-10 * abs(value)
+
+
+def test_lmff_deck_form_and_replicate(tmp_path):
+    """pair_style hybrid/overlay lmff ilp/graphene/hbn, as in Cases/case_5016/in.replicate,
+    against the oracle with the maps that form implies; the stock tersoff form with the same
+    pair_coeff lines; and replicate: two copies of the periodic cell hold exactly twice the
+    energy."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    import oracle
+
+    ct, ilp = os.path.join(POT, "C.opt.tersoff"), os.path.join(POT, "BNCH.ILP")
+    null_t = "NULL " + " ".join(["C(O)"] * 5)
+    lmff = run_deck(tmp_path, "l.in", six_layer_deck([
+        "pair_style hybrid/overlay lmff ilp/graphene/hbn 16.0 0",
+        f"pair_coeff * * lmff {ct} {null_t}", f"pair_coeff * * ilp/graphene/hbn {ilp} C C C C C C"]))
+    stock = run_deck(tmp_path, "s.in", six_layer_deck([
+        "pair_style hybrid/overlay tersoff ilp/graphene/hbn 16.0 0",
+        f"pair_coeff * * tersoff {ct} {null_t}", f"pair_coeff * * ilp/graphene/hbn {ilp} C C C C C C"]))
+    s = S.read_lammps_data(os.path.join(GOLDEN, "inf0.data"))
+    for out, el_i in ((lmff, ["NULL"] + ["C"] * 5), (stock, ["C"] * 6)):
+        o = oracle.Oracle(6, ct, ["NULL"] + ["C(O)"] * 5, ilp, el_i, tap_flag=0, threads=8)
+        o.set_system(s)
+        o.build_list()
+        e = o.compute(1, 0)["energy"]
+        assert printed_close(step0_epair(out), e)
+    # without NULL the two forms are one force field: identical logs
+    full = [f"pair_coeff * * ilp/graphene/hbn {ilp} C C C C C C"]
+    a = run_deck(tmp_path, "a.in", six_layer_deck(["pair_style hybrid/overlay lmff ilp/graphene/hbn 16.0 0",
+                                                   f"pair_coeff * * lmff {ct} " + " ".join(["C(O)"] * 6)] + full))
+    b = run_deck(tmp_path, "b.in", six_layer_deck(["pair_style hybrid/overlay tersoff ilp/graphene/hbn 16.0 0",
+                                                   f"pair_coeff * * tersoff {ct} " + " ".join(["C(O)"] * 6)] + full))
+    assert step0_epair(a) == step0_epair(b)
+    # replicate 2 1 1 through an equal-style variable.  LAMMPS offsets molecule ids per copy
+    # (replicate.cpp:560-571), so the two halves of each sheet become different molecules and
+    # ILP acts across the seam: the energy is not twice the cell's, it is the oracle's on the
+    # replicated atoms.
+    r = run_deck(tmp_path, "r.in", six_layer_deck(
+        ["pair_style hybrid/overlay lmff ilp/graphene/hbn 16.0 0",
+         f"pair_coeff * * lmff {ct} " + " ".join(["C(O)"] * 6)] + full,
+        extra="variable SX index 2\nvariable DX equal ${SX}*1\nreplicate ${DX} 1 1\nwrite_data rep.data"))
+    assert "10032 atoms after replicate" in r
+    rep = S.read_lammps_data(str(tmp_path / "rep.data"))
+    assert rep.n == 10032 and sorted(set(rep.mol.tolist())) == list(range(1, 13))
+    o = oracle.Oracle(6, ct, ["C(O)"] * 6, ilp, ["C"] * 6, tap_flag=0, threads=8)
+    o.set_system(rep)
+    o.build_list()
+    assert printed_close(step0_epair(r), o.compute(1, 0)["energy"])

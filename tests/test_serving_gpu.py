@@ -29,3 +29,24 @@ def test_stream_results_equal_alone_bitwise():
         assert j.result["e_interlayer"] == a["e_interlayer"]
         assert j.result["e_interlayer"] < 0
     assert st["latency_p99_s"] >= st["latency_p50_s"] > 0
+
+
+def test_hbm_estimate_covers_what_a_job_allocates():
+    """The server admits by lmff_system_bytes: for bilayer jobs the estimate must cover what the
+    library actually allocates for them (without being far above it)."""
+    from lmff import structures as S
+
+    for s in (S.layered_stack(30, 20, ["G", "BN"], seed=3), S.twisted_bilayer(5, 3, seed=4, target_atoms=12000)):
+        with capi.Context(0) as ctx:
+            capi.setup_for(ctx, s)
+            est = ctx.system_bytes(s.n, 2)
+            b0, _ = ctx.memory()
+            sid = ctx.add_system(s)
+            ctx.integrator(sid, capi.LMFF_LANGEVIN, dt=0.001, t_target=300.0, damp=1.0, seed=5)
+            ctx.run(20, 10)
+            b1, _ = ctx.memory()
+        used = b1 - b0
+        print(s.n, "atoms: estimate", est, "allocated", used)
+        # per-context tables (parameters, cells, chunk sums) are the only bytes outside the estimate
+        assert used <= est + 4 * 2 ** 20
+        assert est <= 1.2 * used + 4 * 2 ** 20
