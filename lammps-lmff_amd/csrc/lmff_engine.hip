@@ -153,7 +153,7 @@ struct Ctx {
   int end_bit = 1;
   int nchunks = 0;
   // derived cutoffs
-  double cutneigh = 18.0, short_cut = 4.1;
+  double cutneigh = 18.0, short_cut = 4.1, short_reach = 2.1;
   // graph
   hipGraphExec_t graph = nullptr;
   int graph_len = 0;
@@ -276,7 +276,8 @@ struct Ctx {
     for (double c : ilp.cutILPsq) rc_max = std::max(rc_max, std::sqrt(c));
     const double cutforce = std::max(ilp_cut, ters.cutmax);
     cutneigh = cutforce + skin;
-    short_cut = std::max(ters.cutmax, rc_max) + skin;
+    short_reach = std::max(ters.cutmax, rc_max);
+    short_cut = short_reach + skin;
     params_dirty = false;
     gen++;
   }
@@ -530,6 +531,9 @@ struct Ctx {
     P.tersoff_cutmaxsq = ters.cutmax * ters.cutmax;
     P.ilp_cutsq = ilp_cut * ilp_cut;
     P.ilp_splitsq = (ilp_cut + ilp_margin()) * (ilp_cut + ilp_margin());
+    // the per-step tests reach to cutmax and rcut widened by 1e-12 in r^2 (reverse candidates)
+    const double sr = short_reach * (1.0 + 1e-9) + ilp_margin();
+    P.short_splitsq = sr * sr;
     P.farflag = farflag.as<int>();
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
@@ -537,7 +541,8 @@ struct Ctx {
     return P;
   }
 
-  // ILP rows keep pairs with r < rc + margin at the build in front; the rest is only walked once
+  // ILP rows keep pairs with r < rc + margin at the build in front (short lists: r < max(Tersoff
+  // cutmax, ILP normal rcut) + margin); the rest is only walked once
   // some atom has moved more than margin/2 (the far flag).  The margin trades the length of the
   // front segment against how soon thermal motion raises the flag (0.3 A of displacement is
   // beyond what a 300 K atom travels in a 10-step rebuild interval); results do not depend on it.
@@ -633,6 +638,7 @@ struct Ctx {
       bool grow = false;
       if (o[0] > 0) {
         scap = ((int)(o[0] * 1.25) + 63) / 64 * 64;
+        if (scap > 0xffff) fail(LMFF_ERR_MEMORY, "short neighbor list of %d entries per atom", o[0]);
         grow = true;
       }
       if (o[1] > 0) {

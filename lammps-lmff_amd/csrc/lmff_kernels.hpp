@@ -86,7 +86,7 @@ struct DevPtrs {
   int *mol, *tag, *sys;
   const SysDev *sys_tab;
   // lists
-  uint32_t *slist; int *scnt; int scap;
+  uint32_t *slist; int *scnt; int scap;  // scnt: front count | back count << 16
   // ILP rows: a front segment [0, icnt) and a back segment [icap - ocnt, icap)
   uint32_t *ilist; int *icnt, *ocnt; int icap;
   // per step
@@ -98,8 +98,9 @@ struct DevPtrs {
   uint32_t *rtc; int *rtcnt;  // [n][MAXSH] atoms that may hold i in their Tersoff list
   uint32_t *rnc; int *rncnt;  // [n][MAXSH] atoms that may hold i as a normal neighbour
   double *eb;         // [n][MAXSH] bond energy 1/2 b_ij f_A
-  double *fb;         // [n][MAXSH][MAXSH][3] Tersoff force of bond (i, jj) on its kk-th neighbour
-  double *wb;         // [n][MAXSH][MAXSH][6] per-atom virial shares of the same (per-atom virial only)
+  double *fb;         // per atom a block of MAXSH*MAXSH*3: the Tersoff force of bond (i, jj) on i's
+                      // t-th neighbour at (t*ns + jj)*3, ns = tcnt[i] (see bond_slot)
+  double *wb;         // the same with 6 per-atom virial shares per entry (per-atom virial only)
   double4 *ft;        // [n] Tersoff force of i's own bonds and repulsive halves
   double4 *fil, *S;
   double *T;          // [n][6] (per-atom virial only)
@@ -114,6 +115,7 @@ struct DevPtrs {
   int *ovf;               // [0] max short count, [1] max ILP count
   double cutneighsq, shortsq, tersoff_cutmaxsq, ilp_cutsq, inv_rcut, rcut;
   double ilp_splitsq;     // ILP rows hold entries with r^2 < ilp_splitsq (at build time) first
+  double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
   int tap_flag;
 };
@@ -426,12 +428,12 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // the front, the rest from the back.  The per-step pair kernel walks the front segment and
   // then the back one in chunks of 64, so the chunks past the split radius have (almost) no
   // pair in range and skip the pair arithmetic as a whole.
-  int ns = 0, ni = 0, no = 0;
+  int ns = 0, nsb = 0, ni = 0, no = 0;
   uint32_t *irow = P.ilist + (size_t)i * P.icap;
   int r = 0;
   for (int v0 = 0; v0 < total; v0 += 64) {
     const int v = v0 + lane;
-    bool in_i = false, in_o = false, in_s = false;
+    bool in_i = false, in_o = false, in_s = false, in_sb = false;
     uint32_t ent = 0;
     if (v < total) {
       while (run_off[wv][r + 1] <= v) r++;
@@ -450,27 +452,33 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
         const bool ilp = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && ((ilp_mask >> tj) & 1u);
         in_i = ilp && rsq < P.ilp_splitsq;
         in_o = ilp && !(rsq < P.ilp_splitsq);
-        in_s = (rsq <= P.shortsq);
+        in_s = (rsq <= P.shortsq) && rsq < P.short_splitsq;
+        in_sb = (rsq <= P.shortsq) && !(rsq < P.short_splitsq);
       }
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const unsigned long long bi = __ballot(in_i), bo = __ballot(in_o), bs = __ballot(in_s);
-    const int pi = __popcll(bi & lt), po = __popcll(bo & lt), ps = __popcll(bs & lt);
+    const unsigned long long bi = __ballot(in_i), bo = __ballot(in_o), bs = __ballot(in_s), bsb = __ballot(in_sb);
+    const int pi = __popcll(bi & lt), po = __popcll(bo & lt), ps = __popcll(bs & lt), psb = __popcll(bsb & lt);
     if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
     if (in_o && no + po < P.icap) irow[P.icap - 1 - (no + po)] = ent;
     // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
-    // readers coalesce
+    // readers coalesce; it is split like the ILP rows (front from slot 0, back from scap-1)
     if (in_s && ns + ps < P.scap) P.slist[(size_t)(ns + ps) * P.n + i] = ent;
+    if (in_sb && nsb + psb < P.scap) P.slist[(size_t)(P.scap - 1 - (nsb + psb)) * P.n + i] = ent;
     ni += __popcll(bi);
     no += __popcll(bo);
     ns += __popcll(bs);
+    nsb += __popcll(bsb);
   }
   if (lane == 0) {
     const int front = ni < P.icap ? ni : P.icap;
     P.icnt[i] = front;
     P.ocnt[i] = no < P.icap - front ? no : P.icap - front;
     ni += no;
-    P.scnt[i] = ns < P.scap ? ns : P.scap;
+    const int sfront = ns < P.scap ? ns : P.scap;
+    const int sback = nsb < P.scap - sfront ? nsb : P.scap - sfront;
+    P.scnt[i] = sfront | (sback << 16);
+    ns += nsb;
     // only an overflowing row reports its size (a per-wave atomic on one address would
     // serialize every wave of the launch)
     if (ni > P.icap) atomicMax(&P.ovf[1], ni);
@@ -515,7 +523,11 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   const double4 xi = P.x[i];
   const int ti = (int)xi.w - 1;
   const int mi = P.mol[i];
-  const int cnt = P.scnt[i];
+  // the back segment (r beyond the split at the build) can hold a neighbour within reach
+  // only after some atom has moved more than half the split margin since the build
+  const int sc = P.scnt[i];
+  const int sfront = sc & 0xffff, cnt = sfront + (*P.farflag ? (sc >> 16) : 0);
+  const int sback_off = P.scap - 1 + sfront;  // slot of entry k >= sfront: sback_off - k
   const uint32_t *row = P.slist + i;  // transposed: entry k at row[k * n]
   const size_t sstride = (size_t)P.n;
   int ns = 0, nn = 0, nrt = 0, nrn = 0;
@@ -534,7 +546,10 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
     uint32_t eb_[NB];
     double4 xb_[NB];
 #pragma unroll
-    for (int b = 0; b < NB; b++) eb_[b] = (k0 + b < cnt) ? row[(k0 + b) * sstride] : 0u;
+    for (int b = 0; b < NB; b++) {
+      const int k = k0 + b;
+      eb_[b] = (k < cnt) ? row[(size_t)(k < sfront ? k : sback_off - k) * sstride] : 0u;
+    }
 #pragma unroll
     for (int b = 0; b < NB; b++)
       if (k0 + b < cnt) xb_[b] = P.x[e_idx(eb_[b])];
@@ -1038,6 +1053,13 @@ __device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R r
   }
 }
 
+// Entry of bond (i, jj)'s force on i's t-th Tersoff neighbour in fb (times 3) and wb (times
+// 6): target-major with the atom's own bond count as the stride, so that all of one atom's
+// entries are contiguous and the gather reads one run of ns entries per (i, target).
+__device__ __forceinline__ size_t bond_slot(int i, int t, int jj, int ns) {
+  return (size_t)i * (MAXSH * MAXSH) + (size_t)(t * ns + jj);
+}
+
 // displacement x_j(image) - x_i of a list entry, its square and j's type (the Tersoff kernels
 // recompute these where needed instead of keeping per-neighbour arrays, which would live in
 // scratch memory because their indices are not compile-time constants)
@@ -1099,7 +1121,6 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   for (int jj = quad; jj < ns; jj += 4) {
     const uint32_t ej = row[jj];
     const size_t slot = (size_t)i * MAXSH + jj;
-    double *fbj = P.fb + slot * MAXSH * 3;
     double dj[3], rsqj;
     int tj;
     nb_del(P.x, ej, prd, xi, dj, rsqj, tj);
@@ -1133,9 +1154,10 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     const TersP<RT> &p = tp[t3[(ti * MAXT + tj) * MAXT + tj]];
     if (rsqj >= p.cutsq) {
       for (int kk = 0; kk < ns; kk++) {
-        fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
+        double *fv = P.fb + bond_slot(i, kk, jj, ns) * 3;
+        fv[0] = 0.0; fv[1] = 0.0; fv[2] = 0.0;
         if (VA)
-          for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + kk) * 6 + c] = 0.0;
+          for (int c = 0; c < 6; c++) P.wb[bond_slot(i, kk, jj, ns) * 6 + c] = 0.0;
       }
       if (EF) P.eb[slot] = 0.0;
       continue;
@@ -1191,10 +1213,11 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
       int tk;
       nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
       const TersP<double> &q = tp[t3[(ti * MAXT + tj) * MAXT + tk]];
+      double *fv = P.fb + bond_slot(i, kk, jj, ns) * 3;
       if (rsqk >= q.cutsq) {
-        fbj[kk * 3 + 0] = 0.0; fbj[kk * 3 + 1] = 0.0; fbj[kk * 3 + 2] = 0.0;
+        fv[0] = 0.0; fv[1] = 0.0; fv[2] = 0.0;
         if (VA)
-          for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + kk) * 6 + c] = 0.0;
+          for (int c = 0; c < 6; c++) P.wb[bond_slot(i, kk, jj, ns) * 6 + c] = 0.0;
         continue;
       }
       double fi[3], fj[3], fk[3];
@@ -1205,7 +1228,7 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
       fjx += fj[0];
       fjy += fj[1];
       fjz += fj[2];
-      fbj[kk * 3 + 0] = fk[0]; fbj[kk * 3 + 1] = fk[1]; fbj[kk * 3 + 2] = fk[2];
+      fv[0] = fk[0]; fv[1] = fk[1]; fv[2] = fk[2];
       if (VF || VA) {
         // v_tally3: delr1 (x) fj + delr2 (x) fk, a third to each of i, j, k
         const double w[6] = {dj[0] * fj[0] + dk[0] * fk[0], dj[1] * fj[1] + dk[1] * fk[1],
@@ -1216,13 +1239,14 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
           const double t = third * w[c];
           vo[c] += t;
           vj[c] += t;
-          if (VA) P.wb[(slot * MAXSH + kk) * 6 + c] = t;
+          if (VA) P.wb[bond_slot(i, kk, jj, ns) * 6 + c] = t;
         }
       }
     }
-    fbj[jj * 3 + 0] = fjx; fbj[jj * 3 + 1] = fjy; fbj[jj * 3 + 2] = fjz;
+    double *fjv = P.fb + bond_slot(i, jj, jj, ns) * 3;
+    fjv[0] = fjx; fjv[1] = fjy; fjv[2] = fjz;
     if (VA)
-      for (int c = 0; c < 6; c++) P.wb[(slot * MAXSH + jj) * 6 + c] = vj[c];
+      for (int c = 0; c < 6; c++) P.wb[bond_slot(i, jj, jj, ns) * 6 + c] = vj[c];
   }
   // quad reduction, the same order in every lane
 #pragma unroll
@@ -1323,16 +1347,15 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
       for (int b = 0; b < ni; b++)
         if (e_idx(irow[b]) == a) { slot_a = b; break; }
       if (slot_a < 0) continue;
-      const size_t base = (size_t)i * MAXSH;
+      const double *fv = P.fb + bond_slot(i, slot_a, 0, ni) * 3;
       for (int jj = 0; jj < ni; jj++) {
-        const double *fv = P.fb + ((base + jj) * MAXSH + slot_a) * 3;
-        F.x += fv[0];
-        F.y += fv[1];
-        F.z += fv[2];
+        F.x += fv[3 * jj + 0];
+        F.y += fv[3 * jj + 1];
+        F.z += fv[3 * jj + 2];
         if (VA)
-          for (int c = 0; c < 6; c++) va[c] += P.wb[((base + jj) * MAXSH + slot_a) * 6 + c];
+          for (int c = 0; c < 6; c++) va[c] += P.wb[bond_slot(i, slot_a, jj, ni) * 6 + c];
       }
-      if (EF) et += 0.5 * P.eb[base + slot_a];
+      if (EF) et += 0.5 * P.eb[(size_t)i * MAXSH + slot_a];
     }
   }
   // quad reduction, the same order in every lane
@@ -1683,7 +1706,7 @@ __global__ void k_pair_stats(DevPtrs<R> P, unsigned long long *out) {
   atomicAdd(&out[0], (unsigned long long)cnt);
   atomicAdd(&out[1], np);
   atomicAdd(&out[2], (unsigned long long)P.tcnt[i]);
-  atomicAdd(&out[3], (unsigned long long)P.scnt[i]);
+  atomicAdd(&out[3], (unsigned long long)((P.scnt[i] & 0xffff) + (P.scnt[i] >> 16)));
 }
 
 }  // namespace lmff
