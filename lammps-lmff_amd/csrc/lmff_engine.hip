@@ -538,6 +538,14 @@ struct Ctx {
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
+    {
+      const double rc = P.rcut, rc4 = rc * rc * rc * rc;
+      P.tap_poly[0] = -35.0 / rc4;
+      P.tap_poly[1] = 84.0 / (rc4 * rc);
+      P.tap_poly[2] = -70.0 / (rc4 * rc * rc);
+      P.tap_poly[3] = 20.0 / (rc4 * rc * rc * rc);
+      P.tap_poly[4] = 140.0 / (rc4 * rc * rc * rc);
+    }
     return P;
   }
 

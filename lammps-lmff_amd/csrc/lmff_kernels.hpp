@@ -118,6 +118,9 @@ struct DevPtrs {
   double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
   int tap_flag;
+  // Tap(r) = 1 + r^4 (c4 + r (c5 + r (c6 + r c7))) and dTap/dr = d7 r^3 (r - rcut)^3:
+  // {c4, c5, c6, c7, d7}
+  double tap_poly[5];
 };
 
 // ---------------------------------------------------------------------------------------
@@ -717,39 +720,34 @@ struct Tap {
   R t, dt;
 };
 
+// Tap(x) = 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1 with x = r/rcut (pair_ilp_graphene_hbn.h:80-117)
+// written in r, 1 + r^4 (c4 + r (c5 + r (c6 + r c7))) with c_k = a_k / rcut^k, and its
+// derivative 140 x^3 (x - 1)^3 / rcut = d7 r^3 (r - rcut)^3 with d7 = 140 / rcut^7 (the
+// reference's expanded dTap polynomial, factored).  The coefficients are launch arguments,
+// so they stay in scalar registers.
 template <typename R>
-__device__ __forceinline__ Tap<R> calc_tap(R r, R inv_rcut, int tap_flag) {
+__device__ __forceinline__ Tap<R> calc_tap(R r, const double *poly, double rcut, int tap_flag) {
   Tap<R> o;
   if (!tap_flag) {
     o.t = 1;
     o.dt = 0;
     return o;
   }
-  const R x = r * inv_rcut;
-  // Tap(x) = 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1 (pair_ilp_graphene_hbn.h:80-117), Horner
-  R t = (R)20 * x + (R)-70;
-  t = t * x + (R)84;
-  t = t * x + (R)-35;
-  t = t * x;
-  t = t * x;
-  t = t * x;
-  t = t * x + (R)1;
-  R d = (R)140 * x + (R)-420;
-  d = d * x + (R)420;
-  d = d * x + (R)-140;
-  d = d * x;
-  d = d * x;
-  d = d * x;
-  o.t = t;
-  o.dt = d * inv_rcut;
+  const R r2 = r * r;
+  R p = (R)poly[3] * r + (R)poly[2];
+  p = p * r + (R)poly[1];
+  p = p * r + (R)poly[0];
+  o.t = (r2 * r2) * p + (R)1;
+  const R y = r - (R)rcut;
+  o.dt = (R)poly[4] * (r2 * r) * (y * y * y);
   return o;
 }
 
-// minimum resident waves per SIMD requested for the plain (force-only) ILP pair kernel: five
-// waves fit its registers without spilling and hide more of the gather latency (measured 10 %
-// faster than the four the compiler picks by itself); the tally variants keep their registers
+// minimum resident waves per SIMD requested for the plain (force-only) ILP pair kernel: with
+// the prefetched chunk in registers four waves fit without spilling (five spill and lose 20 %);
+// the tally variants keep their registers
 #ifndef LMFF_ILP_MIN_WAVES
-#define LMFF_ILP_MIN_WAVES 5
+#define LMFF_ILP_MIN_WAVES 4
 #endif
 
 template <typename R, bool EF, bool VF, bool VA>
@@ -775,13 +773,22 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
   const int cnt = nfront + nback, back_off = P.icap - P.ocnt[i] - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
   const int itag = P.tag[i];
-  const R inv_rcut = (R)P.inv_rcut;
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
   double w[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0}, tt[6] = {0, 0, 0, 0, 0, 0};
+  // software pipeline: the entry two chunks ahead and the position one chunk ahead are in
+  // flight while a chunk computes (15 % off the FP32 kernel, 2 % off the FP64 one)
+  uint32_t e_n1 = 0, e_n2 = 0;
+  double4 x_n1 = {0, 0, 0, 0};
+  if (lane < cnt) e_n1 = row[ilp_slot(lane, nfront, back_off)];
+  if (lane + 64 < cnt) e_n2 = row[ilp_slot(lane + 64, nfront, back_off)];
+  if (lane < cnt) x_n1 = P.x[e_idx(e_n1)];
   for (int k = lane; k < cnt; k += 64) {
-    const uint32_t e = row[ilp_slot(k, nfront, back_off)];
+    const uint32_t e = e_n1;
+    const double4 xj = x_n1;
+    e_n1 = e_n2;
+    if (k + 64 < cnt) x_n1 = P.x[e_idx(e_n1)];
+    if (k + 128 < cnt) e_n2 = row[ilp_slot(k + 128, nfront, back_off)];
     const int j = e_idx(e);
-    const double4 xj = P.x[j];
     double gx, gy, gz;
     // x_j + 0*prd == x_j exactly, so the image arithmetic is skipped when no lane needs it
     if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
@@ -800,17 +807,17 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
     const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd, rsq = (R)rsqd;
     const R rinv = M::rsqrt(rsq);
     const R r = rsq * rinv;
-    const Tap<R> tp = calc_tap<R>(r, inv_rcut, P.tap_flag);
+    const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
     // ---- van der Waals, one half per ordered pair ----
     const IlpPair<R> *pv = &pij;
     if (!pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
+    // E = -C6 / (r^6 (1 + e)), e = exp(-d (r/seff - 1)) (ilp.cpp:510-530); fpv = -dE/dr / r
     const R r2inv = rinv * rinv;
-    const R r6inv = r2inv * r2inv * r2inv;
-    const R r8inv = r6inv * r2inv;
-    const R TS = (R)1 + M::ex(-pv->d * (r * pv->inv_seff - (R)1));
-    const R TSinv = M::rcp(TS);
-    const R Vvdw = -pv->C6 * r6inv * TSinv;
-    const R fpv = (R)-6 * pv->C6 * r8inv * TSinv + pv->C6_d_seff * (TS - (R)1) * TSinv * TSinv * r8inv * r;
+    const R edamp = M::ex(-pv->d * (r * pv->inv_seff - (R)1));
+    const R TSinv = M::rcp((R)1 + edamp);
+    const R w6 = r2inv * r2inv * r2inv * TSinv;
+    const R Vvdw = -pv->C6 * w6;
+    const R fpv = r2inv * w6 * (pv->C6_d_seff * r * edamp * TSinv - (R)6 * pv->C6);
     // ---- repulsion (i,j) with n_i ----
     const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
     const R pn1 = nix * dx + niy * dy + niz * dz;
