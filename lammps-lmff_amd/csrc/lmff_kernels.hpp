@@ -982,6 +982,12 @@ __device__ __forceinline__ R t_gijk_d(R costheta, const TersP<R> &p) {
 }
 template <typename R>
 __device__ __forceinline__ R t_exdelr(R rij, R rik, const TersP<R> &p, R &ex_delr_d) {
+  // lam3 = 0 (every entry of C.opt.tersoff and BNC.tersoff): the reference's exp(0) = 1 and
+  // a zero derivative, without evaluating them
+  if (p.lam3 == (R)0) {
+    ex_delr_d = 0;
+    return 1;
+  }
   R arg;
   if (p.powermint == 3) {
     const R t = p.lam3 * (rij - rik);
@@ -1018,6 +1024,24 @@ __device__ __forceinline__ R t_bij_d(R zeta, const TersP<R> &p) {
   if (tmp < p.c3) return (R)-0.5 * p.beta * M::pw(tmp, p.powern - (R)1);
   const R tmp_n = M::pw(tmp, p.powern);
   return (R)-0.5 * M::pw((R)1 + tmp_n, (R)-1 - ((R)1 / ((R)2 * p.powern))) * tmp_n / zeta;
+}
+
+// b_ij and d b_ij / d zeta together (ters_bij + ters_bij_d, the same branches): in the
+// common branch both share (1 + (beta zeta)^n), and pow(1 + t, -1 - 1/2n) is taken as
+// pow(1 + t, -1/2n) / (1 + t)
+template <typename R>
+__device__ __forceinline__ void t_bij_both(R zeta, const TersP<R> &p, R &bij, R &bij_d) {
+  using M = MathOps<R>;
+  const R tmp = p.beta * zeta;
+  if (tmp > p.c1 || tmp > p.c2 || tmp < p.c4 || tmp < p.c3) {
+    bij = t_bij(zeta, p);
+    bij_d = t_bij_d(zeta, p);
+    return;
+  }
+  const R tmp_n = M::pw(tmp, p.powern);
+  const R base = (R)1 + tmp_n;
+  bij = M::pw(base, (R)-1 / ((R)2 * p.powern));
+  bij_d = (R)-0.5 * (bij / base) * tmp_n / zeta;
 }
 
 // attractive(): forces on i, j, k of the zeta term of bond (i,j) through neighbour k
@@ -1194,9 +1218,10 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
       fa = -p.bigb * e2 * t_fc(r, p);
       fa_d = p.bigb * e2 * (p.lam2 * t_fc(r, p) - t_fc_d(r, p));
     }
-    const RT bij = t_bij(zeta, p);
+    RT bij, bij_d;
+    t_bij_both(zeta, p, bij, bij_d);
     const double fzv = (double)((RT)0.5 * bij * fa_d / r);
-    const double pref = (double)((RT)-0.5 * fa * t_bij_d(zeta, p));
+    const double pref = (double)((RT)-0.5 * fa * bij_d);
     if (EF) {
       const double eb = (double)((RT)0.5 * bij * fa);
       P.eb[slot] = eb;
