@@ -10,7 +10,10 @@ it, with its force error against FP64.
 
 Other workloads (not the default line):
   --workload multilayer   10-layer alternating G/hBN stack, 260x260 cells per layer
-                          (2,704,000 atoms): FP64 vs mixed step time and NVE drift
+                          (2,704,000 atoms): FP64 vs mixed step time and NVE drift; with
+                          --layers 4 --nx 1118 --ny 1118 the 19,998,784-atom 4-layer stack.
+                          Its cpu_baseline leg runs the oracle on a seeded patch of 1 % of
+                          the box and reports the GPU's FP64 force error on that patch
   --workload serving      stream of twisted-bilayer Langevin jobs (2k-80k atoms, log-uniform),
                           packed server vs the same jobs run one at a time: throughput, p50/p99
 
@@ -116,6 +119,56 @@ def cpu_baseline(s, seconds_budget=20.0):
             "sample": f"{steps} force evaluations + 1 list build of the same {s.n}-atom system "
                       f"(oracle/lmff_oracle.c, OpenMP), integration excluded",
             "cpu_model": _cpu_model()}, f_ref
+
+
+def patch_baseline(s, f_gpu, seed, frac=0.01, halo=24.0):
+    """cpu_baseline leg of the multilayer workloads: the oracle on a seeded rectangular patch
+    covering `frac` of the periodic x-y box, cut out with a `halo`-wide rim (open boundaries).
+    The force on an atom depends only on atoms within rc + two normal-neighbour hops (16 + 2 +
+    2 A < 24 A), so every atom of the patch interior sees the environment it has in the full
+    system: the oracle's forces there are the reference values for the GPU's FP64 forces on
+    those atoms (the spot check), and its time is the CPU baseline for this sample."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle  # noqa: E402
+
+    oracle.build()
+    rng = np.random.default_rng(seed)
+    L = s.boxhi - s.boxlo
+    c = s.boxlo[:2] + rng.random(2) * L[:2]
+    w = 0.5 * L[:2] * np.sqrt(frac)
+    d = s.x[:, :2] - c
+    d -= L[:2] * np.round(d / L[:2])
+    keep = np.nonzero((np.abs(d) < w + halo).all(1))[0]
+    inner = (np.abs(d[keep]) < w).all(1)
+    x = s.x[keep].copy()
+    x[:, :2] = c + d[keep]
+    lo = np.array([c[0] - w[0] - halo - 1.0, c[1] - w[1] - halo - 1.0, s.boxlo[2]])
+    hi = np.array([c[0] + w[0] + halo + 1.0, c[1] + w[1] + halo + 1.0, s.boxhi[2]])
+    sub = S.System(x=x, type=s.type[keep].copy(), tag=s.tag[keep].copy(), mol=s.mol[keep].copy(), boxlo=lo,
+                   boxhi=hi, periodic=(0, 0, s.periodic[2]), elements=list(s.elements))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pot = capi.POTENTIALS
+    o = oracle.Oracle(3, os.path.join(pot, "BNC.tersoff"), s.elements, os.path.join(pot, "BNCH.ILP"), s.elements,
+                      threads=cores)
+    o.set_system(sub)
+    t0 = time.perf_counter()
+    o.build_list()
+    t_list = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    r = o.compute(1, 0)
+    t_force = time.perf_counter() - t1
+    fr = r["f"][inner]
+    fg = f_gpu[keep[inner]]
+    rms = float(np.sqrt((fr ** 2).sum(1).mean()))
+    err = {"atoms_checked": int(inner.sum()), "fraction_of_system": float(inner.sum()) / s.n,
+           "max_abs": float(np.abs(fg - fr).max()), "rel_max_over_rms": float(np.abs(fg - fr).max()) / rms,
+           "patch_seed": seed}
+    per_step = t_force + t_list / 10.0
+    base = {"value": sub.n / per_step, "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 force evaluation + 1 list build of a {sub.n}-atom patch (seeded, {frac:.0%} of the "
+                      f"box plus a {halo:g} A rim) of the same stack (oracle/lmff_oracle.c, OpenMP), "
+                      f"integration excluded", "cpu_model": _cpu_model()}
+    return base, err
 
 
 def profiled_traffic(kernel, nx, ny):
@@ -283,7 +336,7 @@ def multilayer_bench(args):
     msmx = resmx["gpu_ms"] / args.steps
     d64 = (res64["e1"] - res64["e0"]) / n
     dmx = (resmx["e1"] - resmx["e0"]) / n
-    return {
+    out = {
         "metric": "atom-steps/s FP64 vs mixed (multilayer)", "value": n / (ms64 * 1e-3), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic (seeded generator)",
@@ -298,7 +351,12 @@ def multilayer_bench(args):
         "ilp_pairs": stats["ilp_pairs"], "ilp_list_entries": stats["ilp_entries"],
         "fp64_ilp_tflops": stats["ilp_pairs"] * ILP_PAIR_FLOP / (res64["fam_ms"][2] * 1e-3) / 1e12,
         "wall_s": {"fp64": res64["wall_s"], "mixed": resmx["wall_s"]},
+        "cpu_baseline": None, "oracle_force_error": None,
     }
+    if not args.no_cpu:
+        log("CPU baseline and FP64 spot check: oracle on a seeded 1 % patch")
+        out["cpu_baseline"], out["oracle_force_error"] = patch_baseline(s, r64["f"], seed=20240 + args.layers)
+    return out
 
 
 def serving_bench(args):

@@ -6,6 +6,7 @@ differences, the strain derivative of the energy, brute-force neighbor sets and
 invariants.  CPU only.
 """
 import math
+import sys
 import os
 
 import numpy as np
@@ -419,3 +420,22 @@ def test_cg_restatement_on_harmonic_wells():
     a = Aniso(c)
     r = oracle.cg_minimize(a, c + 0.05 * rng.uniform(-1, 1, size=c.shape), 0.0, 1e-9, 200, 10000)
     assert r["stop"] == 3 and np.abs(r["x"] - c).max() < 1e-8
+
+
+# ---------------------------------------------------------------- locality of the force
+def test_patch_with_a_24_A_rim_reproduces_the_full_system(oracle_lib):
+    """The multilayer benchmarks spot-check the GPU on a cut-out patch (bench.patch_baseline):
+    an atom's force depends only on atoms within rc + two normal-neighbour hops, so the oracle
+    on the patch plus a 24 A rim (open x, y boundaries) must give the interior atoms the forces
+    of the full periodic system."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    s = S.layered_stack(60, 40, ["G", "BN", "G", "BN"], seed=31, jitter=0.05)
+    full = make_oracle(s, threads=os.cpu_count() or 1).compute(1, 0)["f"]
+    _, err = bench.patch_baseline(s, full, seed=5, frac=0.04)
+    assert err["atoms_checked"] > 1000
+    assert err["rel_max_over_rms"] < 1e-12
+    # a rim shorter than the interaction range does not: the check can fail
+    _, short = bench.patch_baseline(s, full, seed=5, frac=0.04, halo=6.0)
+    assert short["rel_max_over_rms"] > 1e-4
