@@ -1047,11 +1047,10 @@ __device__ __forceinline__ void t_bij_both(R zeta, const TersP<R> &p, R &bij, R 
 // attractive(): forces on i, j, k of the zeta term of bond (i,j) through neighbour k
 // (pair_tersoff.cpp:642-661 -> ters_zetaterm_d 738-795, costheta_d 797-811)
 template <typename R>
-__device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R rsqij, R rsqik, const R *delrij,
+__device__ __forceinline__ void t_attractive(const TersP<R> &p, R prefactor, R rij, R rik, const R *delrij,
                                              const R *delrik, R *fi, R *fj, R *fk) {
-  using M = MathOps<R>;
-  const R rij = M::sq(rsqij), rijinv = (R)1 / rij;
-  const R rik = M::sq(rsqik), rikinv = (R)1 / rik;
+  const R rijinv = (R)1 / rij;
+  const R rikinv = (R)1 / rik;
   R rij_hat[3], rik_hat[3];
   for (int d = 0; d < 3; d++) {
     rij_hat[d] = rijinv * delrij[d];
@@ -1144,6 +1143,25 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   const int itag = P.tag[i];
   const uint32_t *row = P.tsh + (size_t)i * MAXSH;
   const size_t n = (size_t)P.n;
+  // the quad stages i's neighbours once (x_k - x_i, r^2, r, type per slot); both loops of
+  // every bond read them from here instead of gathering and recomputing them per triplet
+  __shared__ double s_nb[64][MAXSH][6];
+  double(*nbr)[6] = s_nb[threadIdx.x >> 2];
+  for (int kk = quad; kk < ns; kk += 4) {
+    double d[3], rsq;
+    int t;
+    nb_del(P.x, row[kk], prd, xi, d, rsq, t);
+    nbr[kk][0] = d[0];
+    nbr[kk][1] = d[1];
+    nbr[kk][2] = d[2];
+    nbr[kk][3] = rsq;
+    nbr[kk][4] = M::sq(rsq);
+    nbr[kk][5] = (double)t;
+  }
+  // read by the other lanes of this quad only (one wave)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   double4 F = {0.0, 0.0, 0.0, 0.0};
   double et = 0.0;
   double vg[6] = {0, 0, 0, 0, 0, 0};  // global virial of i's bonds (VF)
@@ -1152,19 +1170,21 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   for (int jj = quad; jj < ns; jj += 4) {
     const uint32_t ej = row[jj];
     const size_t slot = (size_t)i * MAXSH + jj;
-    double dj[3], rsqj;
-    int tj;
-    nb_del(P.x, ej, prd, xi, dj, rsqj, tj);
+    const double dj[3] = {nbr[jj][0], nbr[jj][1], nbr[jj][2]};
+    const double rsqj = nbr[jj][3], rj = nbr[jj][4];
+    const int tj = (int)nbr[jj][5];
     // ---- repulsion, i's half; the parameter is the one of whichever atom computes the pair
     //      in the reference (tag parity), so both halves agree ----
     {
-      double gx, gy, gz;
-      ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
-      const bool skip = tag_skip(itag, P.tag[e_idx(ej)], xi.x, xi.y, xi.z, gx, gy, gz);
+      // the coordinates only decide between two images of the same atom
+      const int jtag = P.tag[e_idx(ej)];
+      double gx = 0.0, gy = 0.0, gz = 0.0;
+      if (jtag == itag) ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
+      const bool skip = tag_skip(itag, jtag, xi.x, xi.y, xi.z, gx, gy, gz);
       const int pr = skip ? t3[(tj * MAXT + ti) * MAXT + ti] : t3[(ti * MAXT + tj) * MAXT + tj];
       const TersP<double> &p = tp[pr];
       if (rsqj < p.cutsq) {
-        const double r = sqrt(rsqj);
+        const double r = rj;
         const double tmp_fc = t_fc(r, p), tmp_fc_d = t_fc_d(r, p);
         const double tmp_exp = exp(-p.lam1 * r);
         const double fpair = -p.biga * tmp_exp * (tmp_fc_d - tmp_fc * p.lam1) / r;
@@ -1194,15 +1214,15 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
       continue;
     }
     RT zeta = 0;
-    const RT rij = M::sq(rsqj);
+    const RT rij = rj;
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      double dk[3], rsqk;
-      int tk;
-      nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
+      const double dk[3] = {nbr[kk][0], nbr[kk][1], nbr[kk][2]};
+      const double rsqk = nbr[kk][3];
+      const int tk = (int)nbr[kk][5];
       const TersP<RT> &q = tp[t3[(ti * MAXT + tj) * MAXT + tk]];
       if (rsqk >= q.cutsq) continue;
-      const RT rik = M::sq(rsqk);
+      const RT rik = nbr[kk][4];
       const RT costheta = (dj[0] * dk[0] + dj[1] * dk[1] + dj[2] * dk[2]) / (rij * rik);
       RT dd;
       const RT ex = t_exdelr(rij, rik, q, dd);
@@ -1241,9 +1261,9 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     }
     for (int kk = 0; kk < ns; kk++) {
       if (kk == jj) continue;
-      double dk[3], rsqk;
-      int tk;
-      nb_del(P.x, row[kk], prd, xi, dk, rsqk, tk);
+      const double dk[3] = {nbr[kk][0], nbr[kk][1], nbr[kk][2]};
+      const double rsqk = nbr[kk][3];
+      const int tk = (int)nbr[kk][5];
       const TersP<double> &q = tp[t3[(ti * MAXT + tj) * MAXT + tk]];
       double *fv = P.fb + bond_slot(i, kk, jj, ns) * 3;
       if (rsqk >= q.cutsq) {
@@ -1253,7 +1273,7 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
         continue;
       }
       double fi[3], fj[3], fk[3];
-      t_attractive(q, pref, rsqj, rsqk, dj, dk, fi, fj, fk);
+      t_attractive(q, pref, rj, nbr[kk][4], dj, dk, fi, fj, fk);
       F.x += fi[0];
       F.y += fi[1];
       F.z += fi[2];
@@ -1302,12 +1322,14 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     for (int c = 0; c < 6; c++) P.vatom[c * n + i] += vo[c];
 }
 
-// Per step 4: the force on atom a.  Lane 0 of a quad: the ILP pair sum, a's own Tersoff bonds
-// (both from the kernels before), and the normal-derivative forces (own d n_a / d x_a term and
-// a as a normal neighbour of m).  Lanes 1-3 take one reverse Tersoff partner i each (lane q
-// takes candidates q-1, q+2, ...) and add what i's bonds left for a: fb[i][jj][slot of a] over
-// jj (a as j when jj is a's own slot, as k otherwise).  The quad is summed with a fixed-order
-// shuffle butterfly.
+// Per step 4: the force on atom a, by a quad of lanes.  Lane 0 adds the ILP pair sum and a's
+// own Tersoff bonds (both from the kernels before) and a's own d n_a / d x_a term.  The quad
+// then walks one list of items, lane q taking items q, q+4, ...: first a's reverse Tersoff
+// partners i (what i's bonds left for a: fb[i][slot of a][jj] over jj, a as j when jj is a's
+// own slot, as k otherwise), then a's reverse normal candidates m (the normal-derivative force
+// on a as a normal neighbour of m).  Every item is a short chain of dependent loads; spreading
+// them over the quad keeps several chains in flight per atom.  The quad is summed with a
+// fixed-order shuffle butterfly.
 template <typename R, bool EF, bool VA>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   const int gt = xcd_block() * blockDim.x + threadIdx.x;
@@ -1319,6 +1341,7 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   double et = 0.0;
   double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff shares of i's bonds (per-atom only)
   double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
+  const int rt = P.rtcnt[a], rn = P.rncnt[a];
   if (quad == 0) {
     const double4 f0 = P.fil[a], f1 = P.ft[a];
     F = {f0.x + f1.x, f0.y + f1.y, f0.z + f1.z, 0.0};
@@ -1332,11 +1355,29 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
         (&F.x)[c] -= v;
       }
     }
-    // ---- ILP: a as a normal neighbour of m (m among a's reverse normal candidates) ----
-    const int rn = P.rncnt[a];
-    const uint32_t *rnrow = P.rnc + (size_t)a * MAXSH;
-    for (int q = 0; q < rn; q++) {
-      const int m = (int)rnrow[q];
+  }
+  for (int q = quad; q < rt + rn; q += 4) {
+    if (q < rt) {
+      // ---- Tersoff: what the bonds of i left for a ----
+      const int i = (int)P.rtc[(size_t)a * MAXSH + q];
+      const int ni = P.tcnt[i];
+      const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
+      int slot_a = -1;
+      for (int b = 0; b < ni; b++)
+        if (e_idx(irow[b]) == a) { slot_a = b; break; }
+      if (slot_a < 0) continue;
+      const double *fv = P.fb + bond_slot(i, slot_a, 0, ni) * 3;
+      for (int jj = 0; jj < ni; jj++) {
+        F.x += fv[3 * jj + 0];
+        F.y += fv[3 * jj + 1];
+        F.z += fv[3 * jj + 2];
+        if (VA)
+          for (int c = 0; c < 6; c++) va[c] += P.wb[bond_slot(i, slot_a, jj, ni) * 6 + c];
+      }
+      if (EF) et += 0.5 * P.eb[(size_t)i * MAXSH + slot_a];
+    } else {
+      // ---- ILP: a as a normal neighbour of m ----
+      const int m = (int)P.rnc[(size_t)a * MAXSH + (q - rt)];
       const uint32_t *nm = P.nnb + (size_t)m * 4;
       const int cm = (int)nm[0];
       for (int kk = 0; kk < cm; kk++) {
@@ -1366,28 +1407,6 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
         }
         break;
       }
-    }
-  } else {
-    // ---- Tersoff: what the bonds of i left for a ----
-    const int rt = P.rtcnt[a];
-    const uint32_t *rtrow = P.rtc + (size_t)a * MAXSH;
-    for (int q = quad - 1; q < rt; q += 3) {
-      const int i = (int)rtrow[q];
-      const int ni = P.tcnt[i];
-      const uint32_t *irow = P.tsh + (size_t)i * MAXSH;
-      int slot_a = -1;
-      for (int b = 0; b < ni; b++)
-        if (e_idx(irow[b]) == a) { slot_a = b; break; }
-      if (slot_a < 0) continue;
-      const double *fv = P.fb + bond_slot(i, slot_a, 0, ni) * 3;
-      for (int jj = 0; jj < ni; jj++) {
-        F.x += fv[3 * jj + 0];
-        F.y += fv[3 * jj + 1];
-        F.z += fv[3 * jj + 2];
-        if (VA)
-          for (int c = 0; c < 6; c++) va[c] += P.wb[bond_slot(i, slot_a, jj, ni) * 6 + c];
-      }
-      if (EF) et += 0.5 * P.eb[(size_t)i * MAXSH + slot_a];
     }
   }
   // quad reduction, the same order in every lane
