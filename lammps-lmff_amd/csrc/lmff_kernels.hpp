@@ -433,6 +433,9 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // pair in range and skip the pair arithmetic as a whole.
   int ns = 0, nsb = 0, ni = 0, no = 0;
   uint32_t *irow = P.ilist + (size_t)i * P.icap;
+  // box lengths in registers (the list stores below could alias the system table as far as
+  // the compiler knows, which would reload them for every chunk)
+  const double prd0 = s.prd[0], prd1 = s.prd[1], prd2 = s.prd[2];
   int r = 0;
   for (int v0 = 0; v0 < total; v0 += 64) {
     const int v = v0 + lane;
@@ -446,9 +449,9 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
       ent = e_make(j, sx, sy, sz);
       if (!(sh == 0x15 && j == i)) {
         const double4 xj = P.x[j];
-        const double gx = __dadd_rn(xj.x, (double)sx * s.prd[0]);
-        const double gy = __dadd_rn(xj.y, (double)sy * s.prd[1]);
-        const double gz = __dadd_rn(xj.z, (double)sz * s.prd[2]);
+        const double gx = __dadd_rn(xj.x, (double)sx * prd0);
+        const double gy = __dadd_rn(xj.y, (double)sy * prd1);
+        const double gz = __dadd_rn(xj.z, (double)sz * prd2);
         const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
         const double rsq = rsq_ref(delx, dely, delz);
         const int tj = (int)xj.w - 1;
