@@ -439,3 +439,22 @@ def test_patch_with_a_24_A_rim_reproduces_the_full_system(oracle_lib):
     # a rim shorter than the interaction range does not: the check can fail
     _, short = bench.patch_baseline(s, full, seed=5, frac=0.04, halo=6.0)
     assert short["rel_max_over_rms"] > 1e-4
+
+
+# ---------------------------------------------------------------- per-atom virial split
+def test_peratom_virial_matches_an_autograd_restatement(oracle_lib):
+    """The oracle's per-atom virial (every tally, incl. the split of the normal-derivative
+    forces between k and j) against tests/refimpl_virial.py: forces by automatic
+    differentiation of the published energy forms, the reference's tally calls on top.  An
+    open jittered bilayer flake, so edge atoms with two-neighbour normals take part."""
+    import refimpl_virial
+
+    s = S.layered_stack(6, 4, ["G", "G"], seed=21, jitter=0.08)
+    s.periodic = (0, 0, 0)
+    r = make_oracle(s, threads=os.cpu_count() or 1).compute(0, 4)
+    w = refimpl_virial.peratom_virial(s, CT, "C(O)", ILP, "C")
+    scale = np.abs(w).max()
+    assert np.abs(r["vatom"] - w).max() <= 1e-10 * scale
+    # the split is what is pinned: sharing the k-forces' virial with i instead of j differs
+    wi = refimpl_virial.peratom_virial(s, CT, "C(O)", ILP, "C", k_partner="i")
+    assert np.abs(r["vatom"] - wi).max() > 1e-6 * scale
