@@ -538,6 +538,7 @@ struct Ctx {
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
+    P.rmass = rmass.as<double>();
     {
       const double rc = P.rcut, rc4 = rc * rc * rc * rc;
       P.tap_poly[0] = -35.0 / rc4;
@@ -670,8 +671,10 @@ struct Ctx {
     else hipLaunchKernelGGL((k_tersoff_bonds<R, false, false, false, LDSP>), dim3(nb4), dim3(BS), 0, stream, P);
   }
 
+  // integ: the Verlet loop's final integration (and the step advance) runs inside the force
+  // gather; want_ke also fills the kinetic-energy column
   template <typename R>
-  void force_impl(int ef, int vf, int va) {
+  void force_impl(int ef, int vf, int va, int integ, int want_ke) {
     auto P = ptrs<R>();
     const int nb1 = (int)((n + BS - 1) / BS);
     const int nbw = (int)((n + 3) / 4);
@@ -692,21 +695,27 @@ struct Ctx {
     else launch_bonds<R, false>(P, ef, vf, va, nb4);
     mark_end(3, mk);
     mark_begin(4, mk);
-    if (va) hipLaunchKernelGGL((k_gather<R, true, true>), dim3(nb4), dim3(BS), 0, stream, P);
-    else if (ef) hipLaunchKernelGGL((k_gather<R, true, false>), dim3(nb4), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_gather<R, false, false>), dim3(nb4), dim3(BS), 0, stream, P);
+    if (integ) {
+      if (va) hipLaunchKernelGGL((k_gather<R, true, true, true>), dim3(nb4), dim3(BS), 0, stream, P, want_ke);
+      else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, true>), dim3(nb4), dim3(BS), 0, stream, P, want_ke);
+      else hipLaunchKernelGGL((k_gather<R, false, false, true>), dim3(nb4), dim3(BS), 0, stream, P, want_ke);
+    } else {
+      if (va) hipLaunchKernelGGL((k_gather<R, true, true, false>), dim3(nb4), dim3(BS), 0, stream, P, 0);
+      else if (ef) hipLaunchKernelGGL((k_gather<R, true, false, false>), dim3(nb4), dim3(BS), 0, stream, P, 0);
+      else hipLaunchKernelGGL((k_gather<R, false, false, false>), dim3(nb4), dim3(BS), 0, stream, P, 0);
+    }
     mark_end(4, mk);
   }
 
-  void force(int eflag, int vflag) {
+  void force(int eflag, int vflag, int integ = 0, int want_ke = 0) {
     if (n == 0) return;
     const int va = (vflag & 4) ? 1 : 0;
     // per-atom virial shares of the Tersoff triplets: only allocated once asked for
     if (va) wb.ensure((size_t)n * MAXSH * MAXSH * 6 * sizeof(double));
     const int vf = ((vflag % 4) != 0 || va) ? 1 : 0;
     const int ef = (eflag != 0 || vf) ? 1 : 0;
-    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va);
-    else force_impl<double>(ef, vf, va);
+    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va, integ, want_ke);
+    else force_impl<double>(ef, vf, va, integ, want_ke);
     last_eflag = ef ? (eflag | 1) : 0;
     last_vflag = va ? 4 | 2 : (vf ? 2 : 0);
     forces_valid = true;
@@ -1043,11 +1052,14 @@ struct Ctx {
   void zero_ke() { HIPCHK(hipMemsetAsync(th.as<double>() + 9 * (size_t)n, 0, n * sizeof(double), stream)); }
 
   // ---------------------------------------------------------------- stepping
-  void initial_integrate() {
+  // advance: this step's final integration runs fused into the force gather, so the step
+  // counter moves here instead of in k_advance_step
+  void initial_integrate(bool advance) {
     const double h = 0.5 * ilp_margin();
     hipLaunchKernelGGL(k_initial_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, x.as<double4>(),
                        v.as<double4>(), (const double4 *)f.p, sysid.as<int>(), systab.as<SysDev>(),
-                       rmass.as<double>(), (const double4 *)xhold.p, h * h, farflag.as<int>());
+                       rmass.as<double>(), (const double4 *)xhold.p, h * h, farflag.as<int>(),
+                       advance ? (int)sys.size() : 0);
   }
   void final_integrate(int want_ke) {
     hipLaunchKernelGGL(k_final_integrate, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n,
@@ -1056,15 +1068,21 @@ struct Ctx {
     hipLaunchKernelGGL(k_advance_step, dim3(((int)sys.size() + 63) / 64), dim3(64), 0, stream, (int)sys.size(),
                        systab.as<SysDev>());
   }
+  // forces, then post_force + final_integrate and the step advance, fused into the force
+  // gather (with no atoms only the step advance remains)
+  void force_and_final(int eflag, int vflag, int want_ke) {
+    if (n == 0) {
+      final_integrate(want_ke);
+      return;
+    }
+    force(eflag, vflag, 1, want_ke);
+  }
   void plain_step_body() {
     size_t mk = 0;
     mark_begin(5, mk);
-    initial_integrate();
+    initial_integrate(n > 0);
     mark_end(5, mk);
-    force(0, 0);
-    mark_begin(5, mk);
-    final_integrate(0);
-    mark_end(5, mk);
+    force_and_final(0, 0, 0);
   }
 
   bool rebuild_due() {
@@ -1168,25 +1186,19 @@ struct Ctx {
       }
       size_t mk = 0;
       mark_begin(5, mk);
-      initial_integrate();
+      initial_integrate(n > 0);
       mark_end(5, mk);
       ago++;
       if (rebuild_due()) rebuild();
       if (thermo) {
-        if (peratom_last && s == nsteps) force(3, 6);
-        else force(1, 2);
-        mark_begin(5, mk);
-        final_integrate(1);
-        mark_end(5, mk);
+        if (peratom_last && s == nsteps) force_and_final(3, 6, 1);
+        else force_and_final(1, 2, 1);
         mark_begin(6, mk);
         reduce(10, recs.as<double>() + (size_t)ri * nsys * 16);
         mark_end(6, mk);
         ri++;
       } else {
-        force(0, 0);
-        mark_begin(5, mk);
-        final_integrate(0);
-        mark_end(5, mk);
+        force_and_final(0, 0, 0);
       }
       step++;
       global_step++;

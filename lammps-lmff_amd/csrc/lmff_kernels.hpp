@@ -121,6 +121,8 @@ struct DevPtrs {
   // Tap(r) = 1 + r^4 (c4 + r (c5 + r (c6 + r c7))) and dTap/dr = d7 r^3 (r - rcut)^3:
   // {c4, c5, c6, c7, d7}
   double tap_poly[5];
+  // final integration fused into the force gather (the Verlet loop's plain and thermo steps)
+  const double *rmass;    // [MAXT] mass per type
 };
 
 // ---------------------------------------------------------------------------------------
@@ -1332,13 +1334,15 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
 // own slot, as k otherwise), then a's reverse normal candidates m (the normal-derivative force
 // on a as a normal neighbour of m).  Every item is a short chain of dependent loads; spreading
 // them over the quad keeps several chains in flight per atom.  The quad is summed with a
-// fixed-order shuffle butterfly.
-template <typename R, bool EF, bool VA>
-__global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
-  const int gt = xcd_block() * blockDim.x + threadIdx.x;
-  const int a = gt >> 2;
-  const int quad = gt & 3;
-  if (a >= P.n) return;  // the whole quad leaves together
+// fixed-order shuffle butterfly.  With INT (the Verlet loop), lane 0 then applies post_force
+// (Langevin) and final_integrate to a with the same arithmetic as k_final_integrate, which
+// saves a launch and a re-read of f and v per step.  In that loop k_initial_integrate has
+// already advanced the step counter, so the noise is keyed by step - 1, the step
+// k_final_integrate sees before k_advance_step.
+__device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]);
+
+template <typename R, bool EF, bool VA, bool INT>
+__device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad, int want_ke) {
   const size_t n = (size_t)P.n;
   double4 F = {0.0, 0.0, 0.0, 0.0};
   double et = 0.0;
@@ -1427,10 +1431,44 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P) {
   }
   if (quad != 0) return;
   F.w = 0.0;
+  if (INT) {
+    // post_force (Langevin) + final_integrate, as k_final_integrate does them
+    const SysDev &s = P.sys_tab[P.sys[a]];
+    const double m = P.rmass[(int)P.x[a].w - 1];
+    double4 vp = P.v[a];
+    if (s.style == 1) {
+      const double gamma1 = -m / s.damp / FTM2V;
+      const double gamma2 = sqrt(m) * sqrt(24.0 * BOLTZ / s.damp / s.dtv / MVV2E) / FTM2V * sqrt(s.t_target);
+      const long long step = s.step - 1;
+      uint32_t ctr[4] = {(uint32_t)step, (uint32_t)((unsigned long long)step >> 32), (uint32_t)P.tag[a], 0x4c4d4646u};
+      uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
+      philox4x32(ctr, key);
+      const double u0 = (ctr[0] + 0.5) * 2.3283064365386963e-10 - 0.5;
+      const double u1 = (ctr[1] + 0.5) * 2.3283064365386963e-10 - 0.5;
+      const double u2 = (ctr[2] + 0.5) * 2.3283064365386963e-10 - 0.5;
+      F.x += gamma1 * vp.x + gamma2 * u0;
+      F.y += gamma1 * vp.y + gamma2 * u1;
+      F.z += gamma1 * vp.z + gamma2 * u2;
+    }
+    const double dtfm = s.dtf / m;
+    vp.x += dtfm * F.x;
+    vp.y += dtfm * F.y;
+    vp.z += dtfm * F.z;
+    P.v[a] = vp;
+    if (want_ke) P.th[9 * n + a] = 0.5 * MVV2E * m * (vp.x * vp.x + vp.y * vp.y + vp.z * vp.z);
+  }
   P.f[a] = F;
   if (EF) P.th[2 * n + a] += et;
   if (VA)
     for (int c = 0; c < 6; c++) P.vatom[c * n + a] += va[c] + vk[c];
+}
+
+template <typename R, bool EF, bool VA, bool INT>
+__global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P, int want_ke) {
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int a = gt >> 2;
+  const int quad = gt & 3;
+  if (a < P.n) gather_atom<R, EF, VA, INT>(P, a, quad, want_ke);  // the whole quad or none of it
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1455,9 +1493,14 @@ __device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]) {
 // initial_integrate; also raises the far flag (sticky until the next build) when an atom is now
 // more than half the ILP split margin away from its position at the last build: only then can a
 // pair from the back segment of an ILP row be within the cutoff
+// advance_nsys > 0: block 0 also advances every system's step counter (the fused
+// force-gather integration of this step then keys its noise by step - 1); nothing in this
+// kernel reads the counter
 __global__ void k_initial_integrate(int n, double4 *x, double4 *v, const double4 *f, const int *sys,
-                                    const SysDev *st, const double *rmass_type, const double4 *xhold,
-                                    double far_sq, int *farflag) {
+                                    SysDev *st, const double *rmass_type, const double4 *xhold,
+                                    double far_sq, int *farflag, int advance_nsys) {
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < advance_nsys; k += blockDim.x) st[k].step++;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const SysDev &s = st[sys[p]];
