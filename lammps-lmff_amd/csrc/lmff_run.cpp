@@ -6,7 +6,8 @@
 // executes with the force field, the neighbor build and the integrator on the GPU.
 // Output follows LAMMPS' log format (thermo table, "Loop time", "Performance:" as in
 // LMFF/finish.cpp:124,162).  Also: compute pe/atom and stress/atom, dump custom (text,
-// DumpCustom::header_item, LMFF/dump_custom.cpp:422-440; atoms sorted by id), write_data
+// DumpCustom::header_item, LMFF/dump_custom.cpp:422-440; atoms sorted by id), dump atom
+// (DumpAtom, LMFF/dump_atom.cpp; scaled or, with dump_modify scale no, plain), write_data
 // (WriteData, LMFF/write_data.cpp:236-420, atom_style full with velocities) and
 // reset_timestep.  Extra command: "lmff_precision fp64|mixed".
 //
@@ -38,6 +39,8 @@ constexpr double BOLTZ = 8.617343e-5, MVV2E = 1.0364269e-4, NKTV2P = 1.6021765e6
 
 struct Dump {
   std::string id, file;
+  std::string style = "custom";  // custom | atom
+  bool scale = true;             // atom style: scaled coordinates (dump_modify scale)
   int64_t every = 1;
   std::vector<std::string> fields;
   std::string fmt_float = "%g";
@@ -320,6 +323,24 @@ void write_dump(Dump &dp, const Data &d, int64_t step, const std::string &bounds
   std::vector<int64_t> order(d.n);
   for (int64_t i = 0; i < d.n; i++) order[i] = i;
   std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return d.tag[a] < d.tag[b]; });
+  // dump atom (DumpAtom, LMFF/dump_atom.cpp:210-247,504-513): id type xs ys zs (scaled by the
+  // box, "%d %d %g %g %g") or id type x y z with dump_modify scale no
+  if (dp.style == "atom") {
+    FILE *fp = dp.fp;
+    std::fprintf(fp, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ATOMS\n%lld\n", (long long)step, (long long)d.n);
+    std::fprintf(fp, "ITEM: BOX BOUNDS %s\n", boundstr.c_str());
+    for (int k = 0; k < 3; k++) std::fprintf(fp, "%-1.16e %-1.16e\n", d.lo[k], d.hi[k]);
+    std::fprintf(fp, "ITEM: ATOMS %s\n", dp.scale ? "id type xs ys zs" : "id type x y z");
+    double inv[3];
+    for (int k = 0; k < 3; k++) inv[k] = 1.0 / (d.hi[k] - d.lo[k]);
+    for (int64_t i : order) {
+      double c[3];
+      for (int k = 0; k < 3; k++) c[k] = dp.scale ? (x[3 * i + k] - d.lo[k]) * inv[k] : x[3 * i + k];
+      std::fprintf(fp, "%lld %d %g %g %g\n", (long long)d.tag[i], d.type[i], c[0], c[1], c[2]);
+    }
+    std::fflush(fp);
+    return;
+  }
   // expand c_ID[*] into its six columns
   std::vector<std::string> cols;
   for (const auto &c : dp.fields) {
@@ -642,12 +663,16 @@ int main(int argc, char **argv) {
       if (w.size() >= 4 && (w[3] == "pe/atom" || w[3] == "stress/atom")) computes[w[1]] = w[3];
       else std::printf("WARNING: compute %s ignored by lmff_run\n", w.size() > 3 ? w[3].c_str() : "?");
     } else if (cmd == "dump") {
-      if (w.size() < 6 || w[3] != "custom") die("lmff_run supports dump ID group custom N file fields...");
+      if (w.size() < 6 || (w[3] != "custom" && w[3] != "atom"))
+        die("lmff_run supports dump ID all custom N file fields... and dump ID all atom N file");
+      if (w[2] != "all") die("lmff_run dumps group all only");
       Dump dp;
       dp.id = w[1];
+      dp.style = w[3];
       dp.every = std::stoll(w[4]);
       if (dp.every <= 0) die("Illegal dump command");
       dp.file = w[5];
+      if (dp.style == "atom" && w.size() > 6) die("Illegal dump atom command");
       for (size_t k = 6; k < w.size(); k++) {
         const std::string &c = w[k];
         static const char *plain[] = {"id", "mol", "type", "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"};
@@ -663,11 +688,23 @@ int main(int argc, char **argv) {
       }
       dumps.push_back(dp);
     } else if (cmd == "dump_modify") {
-      for (auto &dp : dumps)
-        if (dp.id == w.at(1))
-          for (size_t k = 2; k + 1 < w.size(); k++)
-            if (w[k] == "format" && k + 2 < w.size() && w[k + 1] == "float") dp.fmt_float = w[k + 2];
-      // atoms are always written sorted by id (dump_modify sort id)
+      // atoms are always written sorted by id: "sort id" / "sort 1" (the id column) and
+      // "sort off" (any order) are all satisfied by that
+      bool found = false;
+      for (auto &dp : dumps) {
+        if (dp.id != w.at(1)) continue;
+        found = true;
+        for (size_t k = 2; k < w.size(); k++) {
+          const std::string &key = w[k];
+          const std::string val = k + 1 < w.size() ? w[k + 1] : "";
+          if (key == "format" && val == "float" && k + 2 < w.size()) { dp.fmt_float = w[k + 2]; k += 2; }
+          else if (key == "scale") { dp.scale = val == "yes"; k++; }
+          else if (key == "sort" && (val == "off" || val == "id" || val == "1")) k++;
+          else if (key == "first" && val == "no") k++;
+          else die("lmff_run does not support dump_modify " + key + " " + val);
+        }
+      }
+      if (!found) die("Could not find dump_modify ID " + w.at(1));
     } else if (cmd == "undump") {
       for (auto it = dumps.begin(); it != dumps.end(); ++it)
         if (it->id == w.at(1)) {

@@ -200,3 +200,32 @@ def test_lmff_deck_form_and_replicate(tmp_path):
     o.set_system(rep)
     o.build_list()
     assert printed_close(step0_epair(r), o.compute(1, 0)["energy"])
+
+
+def test_dump_atom_style(tmp_path):
+    """dump ID all atom N file (the form in the reference's Cases decks, with dump_modify sort 1
+    first no): LAMMPS' header and 'id type xs ys zs' columns, scaled by the box; with
+    dump_modify scale no the unscaled x y z."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=91)
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    run_deck(tmp_path, "d.in", deck(data, "", "\n".join([
+        "dump 1 all atom 10 fric.lammpstrj", "dump_modify 1 sort 1 first no",
+        "dump 2 all atom 10 plain.lammpstrj", "dump_modify 2 scale no",
+        "dump 3 all custom 10 c.dump id type x y z", "dump_modify 3 format float %.17g",
+        "thermo 10", "run 20"])))
+    text = open(tmp_path / "fric.lammpstrj").read().splitlines()
+    assert text[:4] == ["ITEM: TIMESTEP", "0", "ITEM: NUMBER OF ATOMS", str(s.n)]
+    assert text[4] == "ITEM: BOX BOUNDS pp pp ff" and text[8] == "ITEM: ATOMS id type xs ys zs"
+    scaled = S.read_dump_custom(str(tmp_path / "fric.lammpstrj"))
+    plain = S.read_dump_custom(str(tmp_path / "plain.lammpstrj"))
+    custom = S.read_dump_custom(str(tmp_path / "c.dump"))
+    assert [f[0] for f in scaled] == [f[0] for f in custom] == [0, 10, 20]
+    lo, L = s.boxlo, s.boxhi - s.boxlo
+    for a, b, c in zip(scaled, plain, custom):
+        assert a[1] == ["id", "type", "xs", "ys", "zs"] and b[1] == ["id", "type", "x", "y", "z"]
+        assert np.array_equal(a[2][:, :2], c[2][:, :2])
+        np.testing.assert_allclose(lo + a[2][:, 2:5] * L, c[2][:, 2:5], rtol=0, atol=1e-5 * L.max())
+        np.testing.assert_allclose(b[2][:, 2:5], c[2][:, 2:5], rtol=1e-5, atol=1e-5)
