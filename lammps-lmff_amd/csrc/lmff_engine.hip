@@ -143,6 +143,7 @@ struct Ctx {
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
   int ntp = 0;  // Tersoff parameter records
+  bool ilp_sym = false;  // every mapped ILP type pair symmetric (k_ilp's SYM variant applies)
   // conjugate-gradient vectors (x0, g, h), carried through rebuilds while a CG minimisation runs
   DBuf cg_x0, cg_g, cg_h;
   bool cg_active = false;
@@ -220,6 +221,7 @@ struct Ctx {
         q.mapped = 1;
         q.vsym = (p.C6 == r.C6 && p.d == r.d && p.seff == r.seff) ? 1 : 0;
         q.esym = (p.lambda == r.lambda && p.z0 == r.z0) ? 1 : 0;
+        if (!q.vsym || !q.esym) ilp_sym = false;
       }
     tpv.clear();
     for (const auto &p : ters.params) {
@@ -252,6 +254,7 @@ struct Ctx {
     std::vector<TersP<double>> tpd;
     std::vector<TersP<float>> tpf;
     std::vector<int> t3v;
+    ilp_sym = true;
     fill_params(ipd, tpd, t3v);
     fill_params(ipf, tpf, t3v);
     ip_d.ensure(sizeof(IlpPair<double>) * ipd.size());
@@ -673,6 +676,14 @@ struct Ctx {
 
   // integ: the Verlet loop's final integration (and the step advance) runs inside the force
   // gather; want_ke also fills the kinetic-energy column
+  template <typename R, bool SYM>
+  void launch_ilp(const DevPtrs<R> &P, int ef, int vf, int va, int nbw) {
+    if (va) hipLaunchKernelGGL((k_ilp<R, true, true, true, SYM>), dim3(nbw), dim3(BS), 0, stream, P);
+    else if (vf) hipLaunchKernelGGL((k_ilp<R, true, true, false, SYM>), dim3(nbw), dim3(BS), 0, stream, P);
+    else if (ef) hipLaunchKernelGGL((k_ilp<R, true, false, false, SYM>), dim3(nbw), dim3(BS), 0, stream, P);
+    else hipLaunchKernelGGL((k_ilp<R, false, false, false, SYM>), dim3(nbw), dim3(BS), 0, stream, P);
+  }
+
   template <typename R>
   void force_impl(int ef, int vf, int va, int integ, int want_ke) {
     auto P = ptrs<R>();
@@ -684,10 +695,8 @@ struct Ctx {
     else hipLaunchKernelGGL((k_normals<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
     mark_end(1, mk);
     mark_begin(2, mk);
-    if (va) hipLaunchKernelGGL((k_ilp<R, true, true, true>), dim3(nbw), dim3(BS), 0, stream, P);
-    else if (vf) hipLaunchKernelGGL((k_ilp<R, true, true, false>), dim3(nbw), dim3(BS), 0, stream, P);
-    else if (ef) hipLaunchKernelGGL((k_ilp<R, true, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
-    else hipLaunchKernelGGL((k_ilp<R, false, false, false>), dim3(nbw), dim3(BS), 0, stream, P);
+    if (ilp_sym) launch_ilp<R, true>(P, ef, vf, va, nbw);
+    else launch_ilp<R, false>(P, ef, vf, va, nbw);
     mark_end(2, mk);
     mark_begin(3, mk);
     const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom

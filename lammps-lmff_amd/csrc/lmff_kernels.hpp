@@ -755,7 +755,9 @@ __device__ __forceinline__ Tap<R> calc_tap(R r, const double *poly, double rcut,
 #define LMFF_ILP_MIN_WAVES 4
 #endif
 
-template <typename R, bool EF, bool VF, bool VA>
+// SYM: every mapped type pair has the same vdW and exp(-lambda (r - z0)) parameters in both
+// orders (true of both shipped parameter files), so neither per-lane choice below is needed
+template <typename R, bool EF, bool VF, bool VA, bool SYM>
 __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
@@ -815,7 +817,7 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
     const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
     // ---- van der Waals, one half per ordered pair ----
     const IlpPair<R> *pv = &pij;
-    if (!pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
+    if (!SYM && !pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
     // E = -C6 / (r^6 (1 + e)), e = exp(-d (r/seff - 1)) (ilp.cpp:510-530); fpv = -dE/dr / r
     const R r2inv = rinv * rinv;
     const R edamp = M::ex(-pv->d * (r * pv->inv_seff - (R)1));
@@ -833,7 +835,7 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
     const R a1 = pn1 * fq1;
     // ---- repulsion (j,i) with n_j; del_ji = -del ----
     const auto nj4 = P.nrm[j];
-    const R exp0b = pij.esym ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
+    const R exp0b = (SYM || pij.esym) ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
     const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dz);
     const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
     const R frho2 = M::ex(-rd2) * pji.C;

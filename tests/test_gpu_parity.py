@@ -116,6 +116,47 @@ def test_fp64_reference_six_layer_deck(tap, maps):
     assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
 
 
+def test_fp64_asymmetric_pair_parameters(tmp_path):
+    """Both shipped ILP files give (ti,tj) and (tj,ti) the same parameters, for which the pair
+    kernel has a specialised variant.  Here the B-C and N-C rows differ from C-B and C-N (z0,
+    lambda, C6, d, s_R), so the general variant runs: the vdW parameters follow the tag-parity
+    half selection and exp(-lambda (r - z0)) is taken separately for (i,j) and (j,i)."""
+    rows = open(ILP).read().splitlines()
+    out = []
+    for line in rows:
+        w = line.split()
+        if len(w) == 13 and (w[0], w[1]) in (("B", "C"), ("N", "C")):
+            v = [float(t) for t in w[2:]]
+            v[0] *= 1.03   # beta (z0)
+            v[1] *= 0.97   # alpha
+            v[8] *= 1.1    # C6
+            v[5] *= 1.02   # d
+            v[6] *= 0.98   # sR
+            line = f"{w[0]} {w[1]} " + " ".join(repr(t) for t in v)
+        out.append(line)
+    asym = str(tmp_path / "asym.ILP")
+    open(asym, "w").write("\n".join(out) + "\n")
+    import oracle
+
+    s = S.layered_stack(30, 20, ["G", "BN"], seed=17)
+    o = oracle.Oracle(3, BNC, s.elements, asym, s.elements, tap_flag=1, threads=8)
+    o.set_system(s)
+    o.build_list()
+    r = o.compute(3, 6)
+    with capi.Context(0) as ctx:
+        ctx.settings(3, 16.0, 1)
+        ctx.coeff_tersoff(BNC, list(s.elements))
+        ctx.coeff_ilp(asym, list(s.elements))
+        ctx.neighbor(2.0, 10, 0, 0)
+        ctx.masses(list(s.mass_per_type))
+        g = ctx.pair_compute(s, 3, 6)
+    _check_fp64(g, r)
+    assert np.abs(g["eatom"] - r["eatom"]).max() <= 1e-10 * np.abs(r["eatom"]).max()
+    assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
+    # the asymmetry matters: the symmetric file gives a different energy
+    assert abs(r["energy"] - oracle_for(s).compute(1, 0)["energy"]) > 1e-6
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_mixed_force_error(name):
     s = CASES[name]()
