@@ -181,8 +181,9 @@ def patch_baseline(s, f_gpu, seed, frac=0.01, halo=24.0):
 
 
 def profiled_traffic(kernel, nx, ny):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (calibrated against a plain copy), or None when no summary covers this workload."""
+    """HBM bytes per launch of the first kernel whose name starts with `kernel` in the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy), or None when no
+    summary covers this workload."""
     path = os.path.join(REPO, "profiles", f"hbm_traffic_bilayer_{nx * ny * 8}.md")
     try:
         lines = open(path).read().splitlines()
@@ -190,7 +191,7 @@ def profiled_traffic(kernel, nx, ny):
         return None
     for line in lines:
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == f"`{kernel}`":
+        if len(cells) == 6 and cells[0].startswith(f"`{kernel}"):
             return (float(cells[4]) + float(cells[5])) * 1e6
     return None
 
@@ -318,7 +319,7 @@ def main():
         "kernel_ms_per_launch": dict(zip(FAMILIES, res64["fam_ms"])),
         "roofline": {"bound": "valu", "kernel": "k_ilp<double>", "achieved": achieved, "peak": FP64_VALU_PEAK,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK,
-                     "traffic": profiled_traffic("k_ilp<double, false, false, false>", nx, ny),
+                     "traffic": profiled_traffic("k_ilp<double, false, false, false", nx, ny),
                      "algorithmic_bytes_per_launch": traffic_alg,
                      "flop_per_launch": flop, "ilp_pairs": stats["ilp_pairs"],
                      "ilp_list_entries": stats["ilp_entries"]},
