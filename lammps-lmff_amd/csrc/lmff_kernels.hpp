@@ -9,8 +9,9 @@
 //   ilist          [n][icap] uint32: other-molecule atoms within ILP cut + skin (the
 //                  layer-filtered ILP list)
 //   list entries   bits 0-25 atom index, bits 26-27/28-29/30-31 image shift+1 in x/y/z
-//   nrm, dn, dndri normal (R4), d n / d x_k [3 nbrs][3 comps][3 coords] (27 R),
-//                  d n / d x_i (9 R) -- R = double (FP64 mode) or float (mixed mode)
+//   nrm, dn, dndri normal (R4; w = |n1| before normalisation), d n / d x_k [3 nbrs][3 comps]
+//                  [3 coords] (27 R, virial runs only), d n / d x_i (9 R) -- R = double (FP64
+//                  mode) or float (mixed mode)
 //   nnb            [n][4] uint32: count + up to 3 normal neighbours
 //   tsh            [n][MAXSH] uint32: Tersoff neighbours within cutmax
 //   fz, pref, eb   [n][MAXSH] double: per-bond force_zeta scalar, prefactor, energy
@@ -502,6 +503,72 @@ __device__ __forceinline__ int ilp_slot(int k, int nfront, int back_off) { retur
 // Per step 1: Tersoff neighbours within cutmax, ILP normal neighbours and the normal with its
 // derivatives (ILP_neigh + calc_normal, pair_ilp_graphene_hbn.cpp:681-1075).  One thread/atom.
 
+// The unnormalised normal n1 of an atom from its normal-neighbour vectors vet[k] = x_k - x_i
+// (calc_normal, ilp.cpp:835-1067): nn = 2, n1 = v1 x v2 (it depends on x_i); nn = 3,
+// n1 = (v1 x v2 + v3 x v1 + v2 x v3) / 3 (it does not).  Also |n1|^2 and |n1|, both set to 1
+// when n1 = 0 (the return value is then true).
+template <typename R>
+__device__ __forceinline__ bool normal_raw(const R vet[3][3], int nn, R n1[3], R &nn2, R &len) {
+  using M = MathOps<R>;
+  if (nn == 2) {
+    n1[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
+    n1[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
+    n1[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
+  } else {
+    R pv12[3], pv31[3], pv23[3];
+    pv12[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
+    pv12[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
+    pv12[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
+    pv31[0] = vet[2][1] * vet[0][2] - vet[0][1] * vet[2][2];
+    pv31[1] = vet[2][2] * vet[0][0] - vet[0][2] * vet[2][0];
+    pv31[2] = vet[2][0] * vet[0][1] - vet[0][0] * vet[2][1];
+    pv23[0] = vet[1][1] * vet[2][2] - vet[2][1] * vet[1][2];
+    pv23[1] = vet[1][2] * vet[2][0] - vet[2][2] * vet[1][0];
+    pv23[2] = vet[1][0] * vet[2][1] - vet[2][0] * vet[1][1];
+    for (int a = 0; a < 3; a++) n1[a] = (pv12[a] + pv31[a] + pv23[a]) / (R)3;
+  }
+  nn2 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+  len = M::sq(nn2);
+  if (len == (R)0) {
+    len = 1;
+    nn2 = 1;
+    return true;
+  }
+  return false;
+}
+
+// d n1 / d x_m for neighbour slot m is the cross-product derivative [w]x =
+// [[0, w2, -w1], [-w2, 0, w0], [w1, -w0, 0]] with w = (v_{m+1} - v_{m+2}) / 3 (cyclic) for three
+// neighbours and w = v2 (m = 0), -v1 (m = 1) or 0 (m = 2) for two.  Reads only the rows of vet
+// other than m.
+template <typename R>
+__device__ __forceinline__ void normal_dslot_w(const R vet[3][3], int nn, int m, R w[3]) {
+  if (nn == 2) {
+    for (int d = 0; d < 3; d++) w[d] = m == 0 ? vet[1][d] : m == 1 ? -vet[0][d] : (R)0;
+  } else {
+    const R third = (R)1 / (R)3;
+    const int p = m == 0 ? 1 : m == 1 ? 2 : 0, q = m == 0 ? 2 : m == 1 ? 0 : 1;
+    for (int d = 0; d < 3; d++) w[d] = (vet[p][d] - vet[q][d]) * third;
+  }
+}
+
+// d n / d x_m of the normal n = n1 / |n1| for neighbour slot m (calc_normal's dnormal):
+// D[a][b] = d n[a] / d x_m[b] = dn1[a][b] / |n1| - n1[a] dnn[b] / |n1|^2 with
+// dnn[b] = n1 . dn1[:, b] / |n1| and dn1 = [w]x.  Stored by the normals kernel for the virial
+// tallies only; the force gather uses the contracted form -(d n / d x_m)^T S = -w x P / |n1|,
+// P = S - (n . S) n.
+template <typename R>
+__device__ __forceinline__ void normal_dslot(const R vet[3][3], int nn, int m, const R n1[3], R nn2, R len,
+                                             R D[3][3]) {
+  R w[3];
+  normal_dslot_w<R>(vet, nn, m, w);
+  const R dn1[3][3] = {{0, w[2], -w[1]}, {-w[2], 0, w[0]}, {w[1], -w[0], 0}};
+  for (int b = 0; b < 3; b++) {
+    const R dnn = (n1[0] * dn1[0][b] + n1[1] * dn1[1][b] + n1[2] * dn1[2][b]) / len;
+    for (int a = 0; a < 3; a++) D[a][b] = dn1[a][b] / len - n1[a] * dnn / nn2;
+  }
+}
+
 // short-list entries fetched together per thread in the normals kernel
 #ifndef LMFF_NORMALS_BATCH
 #define LMFF_NORMALS_BATCH 8
@@ -509,7 +576,6 @@ __device__ __forceinline__ int ilp_slot(int k, int nfront, int back_off) { retur
 
 template <typename R, bool WANT_VET>
 __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
-  using M = MathOps<R>;
   // per type pair (ti, tj): Tersoff-mapped, ILP-mapped, cutILPsq and the widened reverse cutoff,
   // built once per block in LDS (read per short-list entry)
   struct NPair {
@@ -625,90 +691,39 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   R vet[3][3];
   for (int k = 0; k < 3; k++)
     for (int d = 0; d < 3; d++) vet[k][d] = (R)vetd[k][d];
-  R n1[3], normal[3], dn1[3][3][3];
   R *dno = P.dn + (size_t)i * 27;
   if (nn <= 1 || !s_np[ti * MAXT + ti].imapped) {
     P.nrm[i] = {0, 0, 1, 0};
-    for (int q = 0; q < 27; q++) dno[q] = 0;
+    if (WANT_VET)
+      for (int q = 0; q < 27; q++) dno[q] = 0;
     nrow[0] = (uint32_t)(nn <= 1 ? nn : 0);
     return;
   }
-  R nn2, nrm_len;
+  R n1[3], nn2, nrm_len;
+  if (normal_raw<R>(vet, nn, n1, nn2, nrm_len)) atomicOr(P.err, EB_NORMAL_ZERO);
   if (nn == 2) {
-    // pv12 = v1 x v2; n depends on x_i (ilp.cpp:835-927)
-    n1[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
-    n1[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
-    n1[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
+    // d n / d x_i of the two-neighbour normal n1 = v1 x v2 (ilp.cpp:835-927)
     R dpvdri[3][3];
     dpvdri[0][0] = 0; dpvdri[0][1] = vet[0][2] - vet[1][2]; dpvdri[0][2] = vet[1][1] - vet[0][1];
     dpvdri[1][0] = vet[1][2] - vet[0][2]; dpvdri[1][1] = 0; dpvdri[1][2] = vet[0][0] - vet[1][0];
     dpvdri[2][0] = vet[0][1] - vet[1][1]; dpvdri[2][1] = vet[1][0] - vet[0][0]; dpvdri[2][2] = 0;
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) dn1[a][b][2] = 0;
-    dn1[0][0][0] = 0;          dn1[0][1][0] = vet[1][2];  dn1[0][2][0] = -vet[1][1];
-    dn1[1][0][0] = -vet[1][2]; dn1[1][1][0] = 0;          dn1[1][2][0] = vet[1][0];
-    dn1[2][0][0] = vet[1][1];  dn1[2][1][0] = -vet[1][0]; dn1[2][2][0] = 0;
-    dn1[0][0][1] = 0;          dn1[0][1][1] = -vet[0][2]; dn1[0][2][1] = vet[0][1];
-    dn1[1][0][1] = vet[0][2];  dn1[1][1][1] = 0;          dn1[1][2][1] = -vet[0][0];
-    dn1[2][0][1] = -vet[0][1]; dn1[2][1][1] = vet[0][0];  dn1[2][2][1] = 0;
-    nn2 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
-    nrm_len = M::sq(nn2);
-    if (nrm_len == (R)0) {
-      atomicOr(P.err, EB_NORMAL_ZERO);
-      nrm_len = 1;
-      nn2 = 1;
-    }
     R dni[3];
     for (int b = 0; b < 3; b++) dni[b] = (n1[0] * dpvdri[0][b] + n1[1] * dpvdri[1][b] + n1[2] * dpvdri[2][b]) / nrm_len;
     R *dro = P.dndri + (size_t)i * 9;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) dro[a * 3 + b] = dpvdri[a][b] / nrm_len - n1[a] * dni[b] / nn2;
-  } else {
-    // n1 = (v1xv2 + v3xv1 + v2xv3)/3; independent of x_i (ilp.cpp:929-1067)
-    R pv12[3], pv31[3], pv23[3];
-    pv12[0] = vet[0][1] * vet[1][2] - vet[1][1] * vet[0][2];
-    pv12[1] = vet[0][2] * vet[1][0] - vet[1][2] * vet[0][0];
-    pv12[2] = vet[0][0] * vet[1][1] - vet[1][0] * vet[0][1];
-    pv31[0] = vet[2][1] * vet[0][2] - vet[0][1] * vet[2][2];
-    pv31[1] = vet[2][2] * vet[0][0] - vet[0][2] * vet[2][0];
-    pv31[2] = vet[2][0] * vet[0][1] - vet[0][0] * vet[2][1];
-    pv23[0] = vet[1][1] * vet[2][2] - vet[2][1] * vet[1][2];
-    pv23[1] = vet[1][2] * vet[2][0] - vet[2][2] * vet[1][0];
-    pv23[2] = vet[1][0] * vet[2][1] - vet[2][0] * vet[1][1];
-    for (int a = 0; a < 3; a++) n1[a] = (pv12[a] + pv31[a] + pv23[a]) / (R)3;
-    // d n1 / d x_m = (dpv12 + dpv23 + dpv31)/3 for neighbour m; the cross-product derivative
-    // of u x w with respect to u is [[0, w2, -w1], [-w2, 0, w0], [w1, -w0, 0]] (rows: n comps)
-    // m = 0: pv12 (u = v1, w = v2) and pv31 (v3 x v1, derivative wrt v1 = -[v3]x) ...
-    const R third = (R)1 / (R)3;
-    // difference vectors: derivative of n1 wrt x_m equals (1/3) * cross-derivative with
-    // w_m = v_{m+1} - v_{m+2} (cyclic), i.e. d/dx_m [(x2-x1)x(x3-x1)]
-    R wv[3][3];
-    for (int d = 0; d < 3; d++) {
-      wv[0][d] = vet[1][d] - vet[2][d];
-      wv[1][d] = vet[2][d] - vet[0][d];
-      wv[2][d] = vet[0][d] - vet[1][d];
-    }
-    for (int m = 0; m < 3; m++) {
-      dn1[0][0][m] = 0;                  dn1[0][1][m] = wv[m][2] * third;  dn1[0][2][m] = -wv[m][1] * third;
-      dn1[1][0][m] = -wv[m][2] * third;  dn1[1][1][m] = 0;                 dn1[1][2][m] = wv[m][0] * third;
-      dn1[2][0][m] = wv[m][1] * third;   dn1[2][1][m] = -wv[m][0] * third; dn1[2][2][m] = 0;
-    }
-    nn2 = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
-    nrm_len = M::sq(nn2);
-    if (nrm_len == (R)0) {
-      atomicOr(P.err, EB_NORMAL_ZERO);
-      nrm_len = 1;
-      nn2 = 1;
-    }
   }
   const R rl = (R)1 / nrm_len;
-  for (int a = 0; a < 3; a++) normal[a] = n1[a] * rl;
-  P.nrm[i] = {normal[0], normal[1], normal[2], (R)0};
-  // dnormal[id][ip][m] = dn1/nn - n1 * dnn/nn2, dnn[ip][m] = n1 . dn1[:, ip, m] / nn
-  for (int m = 0; m < 3; m++)
-    for (int b = 0; b < 3; b++) {
-      const R dnn = (n1[0] * dn1[0][b][m] + n1[1] * dn1[1][b][m] + n1[2] * dn1[2][b][m]) / nrm_len;
-      for (int a = 0; a < 3; a++) dno[m * 9 + a * 3 + b] = dn1[a][b][m] / nrm_len - n1[a] * dnn / nn2;
+  P.nrm[i] = {n1[0] * rl, n1[1] * rl, n1[2] * rl, nrm_len};
+  // d n / d x_k of every neighbour slot, for the virial tallies of the ILP kernel; the force
+  // gather evaluates the one slot it needs itself, from positions, so plain force steps
+  // neither store nor load these 27 values per atom
+  if (WANT_VET)
+    for (int m = 0; m < 3; m++) {
+      R D[3][3];
+      normal_dslot<R>(vet, nn, m, n1, nn2, nrm_len, D);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) dno[m * 9 + a * 3 + b] = D[a][b];
     }
 }
 
@@ -1351,6 +1366,7 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
   double va[6] = {0, 0, 0, 0, 0, 0};   // Tersoff shares of i's bonds (per-atom only)
   double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
   const int rt = P.rtcnt[a], rn = P.rncnt[a];
+  const double *prd = P.sys_tab[P.sys[a]].prd;  // a's reverse candidates share a's system
   if (quad == 0) {
     const double4 f0 = P.fil[a], f1 = P.ft[a];
     F = {f0.x + f1.x, f0.y + f1.y, f0.z + f1.z, 0.0};
@@ -1387,19 +1403,53 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
     } else {
       // ---- ILP: a as a normal neighbour of m ----
       const int m = (int)P.rnc[(size_t)a * MAXSH + (q - rt)];
-      const uint32_t *nm = P.nnb + (size_t)m * 4;
-      const int cm = (int)nm[0];
-      for (int kk = 0; kk < cm; kk++) {
-        if (e_idx(nm[1 + kk]) != a) continue;
+      const uint4 nm = *(const uint4 *)(P.nnb + (size_t)m * 4);
+      const int cm = (int)nm.x;
+      if (cm < 2) continue;  // the normal is z and does not move with its neighbour
+      const int kk = e_idx(nm.y) == a ? 0 : e_idx(nm.z) == a ? 1 : (cm == 3 && e_idx(nm.w) == a) ? 2 : -1;
+      if (kk < 0) continue;
+      {
+        // F_a = -(d n_m / d x_a)^T S_m = -w x P / |n1_m| with P = S_m - (n_m . S_m) n_m and w
+        // the vector of normal_dslot_w from m's other normal neighbours (nrm.w holds |n1_m|)
+        const auto n4 = P.nrm[m];
         const double4 Sm = P.S[m];
-        const R *D = P.dn + (size_t)m * 27 + kk * 9;
+        double w0, w1, w2;
+        if (cm == 3) {
+          // w = (x_p - x_q) / 3 over m's two other neighbours (x_m cancels)
+          const uint32_t ep = kk == 0 ? nm.z : kk == 1 ? nm.w : nm.y;
+          const uint32_t eq = kk == 0 ? nm.w : kk == 1 ? nm.y : nm.z;
+          double px_, py_, pz_, qx_, qy_, qz_;
+          ghost(P.x[e_idx(ep)], ep, prd, px_, py_, pz_);
+          ghost(P.x[e_idx(eq)], eq, prd, qx_, qy_, qz_);
+          const R third = (R)1 / (R)3;
+          w0 = (double)((R)(px_ - qx_) * third);
+          w1 = (double)((R)(py_ - qy_) * third);
+          w2 = (double)((R)(pz_ - qz_) * third);
+        } else {
+          // w = v_other (kk = 0) or -v_other (kk = 1), v = x_other - x_m
+          const uint32_t eo = kk == 0 ? nm.z : nm.y;
+          const double4 xm = P.x[m];
+          double ox, oy, oz;
+          ghost(P.x[e_idx(eo)], eo, prd, ox, oy, oz);
+          const double sg = kk == 0 ? 1.0 : -1.0;
+          w0 = sg * (double)(R)(ox - xm.x);
+          w1 = sg * (double)(R)(oy - xm.y);
+          w2 = sg * (double)(R)(oz - xm.z);
+        }
+        const double nx = n4.x, ny = n4.y, nz = n4.z;
+        const double ns = nx * Sm.x + ny * Sm.y + nz * Sm.z;
+        const double px = Sm.x - ns * nx, py = Sm.y - ns * ny, pz = Sm.z - ns * nz;
+        const double rl = 1.0 / (double)n4.w;
         double Fk[3];
-        for (int c = 0; c < 3; c++) Fk[c] = -((double)D[0 * 3 + c] * Sm.x + (double)D[1 * 3 + c] * Sm.y + (double)D[2 * 3 + c] * Sm.z);
+        Fk[0] = -(w1 * pz - w2 * py) * rl;
+        Fk[1] = -(w2 * px - w0 * pz) * rl;
+        Fk[2] = -(w0 * py - w1 * px) * rl;
         F.x += Fk[0];
         F.y += Fk[1];
         F.z += Fk[2];
         if (VA) {
-          // 0.5 * (vet_k(m) (x) F_k - T_m D_k)
+          // 0.5 * (vet_k(m) (x) F_k - T_m D_k), D_k as the normals kernel stored it
+          const R *D = P.dn + (size_t)m * 27 + kk * 9;
           const double *vt = P.vet + (size_t)m * 9 + kk * 3;
           const double *Tm = P.T + (size_t)m * 6;
           const double Tf[3][3] = {{Tm[0], Tm[3], Tm[4]}, {Tm[3], Tm[1], Tm[5]}, {Tm[4], Tm[5], Tm[2]}};
@@ -1414,7 +1464,6 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
           vk[4] += 0.5 * (vt[0] * Fk[2] - TD[0][2]);
           vk[5] += 0.5 * (vt[1] * Fk[2] - TD[1][2]);
         }
-        break;
       }
     }
   }
