@@ -100,6 +100,12 @@ int lmff_system_natoms(lmff_ctx *ctx, int sys_id, int64_t *natoms);
 /* copies the system's state back in the caller's original atom order (any may be NULL) */
 int lmff_system_get(lmff_ctx *ctx, int sys_id, double *x, double *v, double *f);
 int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *x, const double *v);
+/* Image flags [n][3] in the caller's atom order (atom->image unpacked, LMFF/atom.h:57): the
+ * device counts box crossings when it wraps positions at a neighbor-list build, as
+ * Domain::pbc does (LMFF/domain.cpp:501); unwrapped x = x + image * box length (Domain::unmap,
+ * LMFF/domain.cpp:1534).  A new system starts with image 0; set accepts -512..511. */
+int lmff_system_get_image(lmff_ctx *ctx, int sys_id, int32_t *image);
+int lmff_system_set_image(lmff_ctx *ctx, int sys_id, const int32_t *image);
 /* per-atom energy [n] and virial [n][6] of the last evaluation with those flags */
 int lmff_system_peratom(lmff_ctx *ctx, int sys_id, double *eatom, double *vatom);
 /* Step counter of a system (the Langevin noise is keyed by it); the reset_timestep command

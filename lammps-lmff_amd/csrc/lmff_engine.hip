@@ -133,11 +133,11 @@ struct Ctx {
   struct Staged {
     int id;
     std::vector<double> x, v;
-    std::vector<int> type, tag, mol;
+    std::vector<int> type, tag, mol, img;
   };
   std::vector<Staged> staged;
   // device state
-  DBuf x, v, f, xhold, mol, tag, sysid, orig, key, key2, val, perm, tmp4, tmpi;
+  DBuf x, v, f, xhold, mol, tag, img, sysid, orig, key, key2, val, perm, tmp4, tmpi;
   DBuf slist, scnt, ilist, icnt, ocnt;
   DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
@@ -176,7 +176,7 @@ struct Ctx {
   int64_t pair_n = -1;
 
   Ctx() {
-    DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
+    DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &img, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
                    &slist, &scnt, &ilist, &icnt, &ocnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
@@ -321,8 +321,8 @@ struct Ctx {
     }
     if (nn >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)nn);
     const size_t m1 = std::max<size_t>((size_t)nn, 1);
-    DBuf nx, nv, nmol, ntag, norig, nsys, nf;
-    for (DBuf *b : {&nx, &nv, &nmol, &ntag, &norig, &nsys, &nf}) b->tally = &bytes_in_use;
+    DBuf nx, nv, nmol, ntag, nimg, norig, nsys, nf;
+    for (DBuf *b : {&nx, &nv, &nmol, &ntag, &nimg, &norig, &nsys, &nf}) b->tally = &bytes_in_use;
     nf.ensure(m1 * sizeof(double4));
     HIPCHK(hipMemsetAsync(nf.p, 0, m1 * sizeof(double4), stream));
     std::vector<std::pair<int, int>> keep_f;
@@ -330,6 +330,7 @@ struct Ctx {
     nv.ensure(m1 * sizeof(double4));
     nmol.ensure(m1 * sizeof(int));
     ntag.ensure(m1 * sizeof(int));
+    nimg.ensure(m1 * sizeof(int));
     norig.ensure(m1 * sizeof(int));
     nsys.ensure(m1 * sizeof(int));
     for (size_t k = 0; k < next.size(); k++) {
@@ -342,6 +343,7 @@ struct Ctx {
         HIPCHK(hipMemcpyAsync(nv.as<double4>() + s.begin, v.as<double4>() + o, m * sizeof(double4), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(nmol.as<int>() + s.begin, mol.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(ntag.as<int>() + s.begin, tag.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nimg.as<int>() + s.begin, img.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(norig.as<int>() + s.begin, orig.as<int>() + o, m * sizeof(int), hipMemcpyDeviceToDevice, stream));
         if (forces_valid) {
           // the force of the last step (pair + post_force terms such as Langevin) stays with the system
@@ -361,6 +363,7 @@ struct Ctx {
         HIPCHK(hipMemcpy(nv.as<double4>() + s.begin, hv.data(), m * sizeof(double4), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(nmol.as<int>() + s.begin, st.mol.data(), m * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ntag.as<int>() + s.begin, st.tag.data(), m * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(nimg.as<int>() + s.begin, st.img.data(), m * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(norig.as<int>() + s.begin, ho.data(), m * sizeof(int), hipMemcpyHostToDevice));
       }
       hipLaunchKernelGGL(k_fill_int, dim3((m + BS - 1) / BS), dim3(BS), 0, stream, (int)m, nsys.as<int>() + s.begin,
@@ -371,6 +374,7 @@ struct Ctx {
     adopt(v, nv);
     adopt(mol, nmol);
     adopt(tag, ntag);
+    adopt(img, nimg);
     adopt(orig, norig);
     adopt(sysid, nsys);
     adopt(fsave, nf);
@@ -421,8 +425,8 @@ struct Ctx {
   // alternate key/value buffers and list rows of scap + icap entries.
   static double atom_bytes(int scap_, int icap_) {
     const double d4 = 9 * 32.0;                                     // x v f xhold tmp4 nrm ft fil S
-    const double i32 = 15 * 4.0;                                    // mol tag sysid orig tmpi key key2 val
-                                                                    // perm scnt icnt ocnt tcnt rtcnt rncnt
+    const double i32 = 16 * 4.0;                                    // mol tag img sysid orig tmpi key key2
+                                                                    // val perm scnt icnt ocnt tcnt rtcnt rncnt
     const double f64 = 8.0 * (27 + 9 + 9 + 6 + NTH + 6 + MAXSH + 3 * MAXSH * MAXSH);  // dn dndri vet T th
                                                                     // vatom eb fb
     const double u32 = 4.0 * (4 + 3 * MAXSH);                       // nnb tsh rtc rnc
@@ -450,7 +454,8 @@ struct Ctx {
     m = std::max<size_t>(m, 1);
     x.ensure(m * sizeof(double4)); v.ensure(m * sizeof(double4)); f.ensure(m * sizeof(double4));
     xhold.ensure(m * sizeof(double4)); tmp4.ensure(m * sizeof(double4));
-    mol.ensure(m * sizeof(int)); tag.ensure(m * sizeof(int)); sysid.ensure(m * sizeof(int));
+    mol.ensure(m * sizeof(int)); tag.ensure(m * sizeof(int)); img.ensure(m * sizeof(int));
+    sysid.ensure(m * sizeof(int));
     orig.ensure(m * sizeof(int)); tmpi.ensure(m * sizeof(int));
     key.ensure(m * sizeof(uint32_t)); key2.ensure(m * sizeof(uint32_t));
     val.ensure(m * sizeof(int)); perm.ensure(m * sizeof(int));
@@ -596,8 +601,8 @@ struct Ctx {
     size_t mk = 0;
     mark_begin(0, mk);
     const int nb = (int)((n + BS - 1) / BS);
-    hipLaunchKernelGGL(k_wrap_key, dim3(nb), dim3(BS), 0, stream, (int)n, x.as<double4>(), sysid.as<int>(),
-                       systab.as<SysDev>(), key.as<uint32_t>(), val.as<int>(), errw.as<int>());
+    hipLaunchKernelGGL(k_wrap_key, dim3(nb), dim3(BS), 0, stream, (int)n, x.as<double4>(), img.as<int>(),
+                       sysid.as<int>(), systab.as<SysDev>(), key.as<uint32_t>(), val.as<int>(), errw.as<int>());
     size_t tb = sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(), val.as<int>(),
                                               perm.as<int>(), (int)n, 0, end_bit, stream));
@@ -607,6 +612,7 @@ struct Ctx {
     if (!restore_f.empty()) permute4(fsave);
     permutei(mol);
     permutei(tag);
+    permutei(img);
     permutei(orig);
     if (cg_active) {
       permute4(cg_x0);
@@ -1468,6 +1474,7 @@ int lmff_system_add(lmff_ctx *ctx, int64_t natoms, const double boxlo[3], const 
     st.type.resize(natoms);
     st.tag.resize(natoms);
     st.mol.resize(natoms);
+    st.img.assign(natoms, IMG_ZERO);
     for (int64_t i = 0; i < natoms; i++) {
       if (type[i] < 1 || type[i] > c.ntypes) fail(LMFF_ERR_ARG, "Invalid atom type %d", (int)type[i]);
       if (tag[i] <= 0 || tag[i] > INT32_MAX) fail(LMFF_ERR_ARG, "atom IDs must be in 1..2^31-1");
@@ -1647,6 +1654,47 @@ int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *xi, const double *v
       c.need_rebuild = true;
       c.forces_valid = false;
     }
+  });
+}
+
+int lmff_system_get_image(lmff_ctx *ctx, int sys_id, int32_t *image) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!image) fail(LMFF_ERR_ARG, "bad image array");
+    c.relayout();
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    auto h = c.download<int>(c.img, s->begin, m);
+    for (size_t q = 0; q < m; q++)
+      for (int d = 0; d < 3; d++)
+        image[3 * (size_t)org[q] + d] = (int32_t)(((uint32_t)h[q] >> (IMG_BITS * d)) & IMG_MASK) - IMG_MAX;
+  });
+}
+
+int lmff_system_set_image(lmff_ctx *ctx, int sys_id, const int32_t *image) {
+  return guard(ctx, [&] {
+    auto &c = ctx->c;
+    if (!image) fail(LMFF_ERR_ARG, "bad image array");
+    c.relayout();
+    SysHost *s = c.find(sys_id);
+    if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const size_t m = (size_t)(s->end - s->begin);
+    auto org = c.download<int>(c.orig, s->begin, m);
+    std::vector<int> h(m);
+    for (size_t q = 0; q < m; q++) {
+      uint32_t packed = 0;
+      for (int d = 0; d < 3; d++) {
+        const int32_t v = image[3 * (size_t)org[q] + d];
+        if (v < -IMG_MAX || v >= IMG_MAX) fail(LMFF_ERR_ARG, "image flag %d out of range", (int)v);
+        packed |= (uint32_t)(v + IMG_MAX) << (IMG_BITS * d);
+      }
+      h[q] = (int)packed;
+    }
+    if (m) HIPCHK(hipMemcpy(c.img.as<int>() + s->begin, h.data(), m * sizeof(int), hipMemcpyHostToDevice));
   });
 }
 

@@ -34,6 +34,11 @@ constexpr int MAXSH = 8;       // Tersoff neighbours within cutmax per atom
 constexpr int IDX_BITS = 26;
 constexpr uint32_t IDX_MASK = (1u << IDX_BITS) - 1u;
 constexpr int NTH = 10;        // thermo columns
+// image flags, as LAMMPS' default 32-bit imageint (LMFF/lmptype.h:98-101)
+constexpr int IMG_BITS = 10;
+constexpr uint32_t IMG_MASK = (1u << IMG_BITS) - 1u;
+constexpr int IMG_MAX = 1 << (IMG_BITS - 1);
+constexpr int IMG_ZERO = IMG_MAX | (IMG_MAX << IMG_BITS) | (IMG_MAX << (2 * IMG_BITS));
 constexpr double FTM2V = 1.0 / 1.0364269e-4;  // LMFF/update.cpp:170
 constexpr double MVV2E = 1.0364269e-4;
 constexpr double BOLTZ = 8.617343e-5;
@@ -254,7 +259,11 @@ __device__ __forceinline__ bool tag_skip(int itag, int jtag, double xi, double y
 // ---------------------------------------------------------------------------------------
 // Rebuild: wrap (Domain::pbc, domain.cpp:501-560), cell key, sort, permute, cell ranges
 
-__global__ void k_wrap_key(int n, double4 *x, const int *sys, const SysDev *st, uint32_t *key, int *val, int *err) {
+// Domain::pbc (LMFF/domain.cpp:501): wraps into the periodic box and counts the crossing in
+// the atom's image flags, packed as LAMMPS' 32-bit imageint (10 bits per dimension, offset
+// IMG_MAX; x in bits 0-9, y in 10-19, z in 20-29)
+__global__ void k_wrap_key(int n, double4 *x, int *img, const int *sys, const SysDev *st, uint32_t *key, int *val,
+                           int *err) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const SysDev &s = st[sys[p]];
@@ -262,15 +271,23 @@ __global__ void k_wrap_key(int n, double4 *x, const int *sys, const SysDev *st, 
   double c[3] = {xp.x, xp.y, xp.z};
   if (!isfinite(c[0]) || !isfinite(c[1]) || !isfinite(c[2])) atomicOr(err, EB_NONFINITE);
   int cell[3];
+  uint32_t im = (uint32_t)img[p];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     if (s.per[d]) {
+      const int sh = IMG_BITS * d;
+      uint32_t idim = (im >> sh) & IMG_MASK;
       double hi = s.lo[d] + s.prd[d];
-      if (c[d] < s.lo[d]) c[d] += s.prd[d];
+      if (c[d] < s.lo[d]) {
+        c[d] += s.prd[d];
+        idim = (idim - 1) & IMG_MASK;
+      }
       if (c[d] >= hi) {
         c[d] -= s.prd[d];
         c[d] = c[d] > s.lo[d] ? c[d] : s.lo[d];
+        idim = (idim + 1) & IMG_MASK;
       }
+      im = (im & ~(IMG_MASK << sh)) | (idim << sh);
     }
     double f = floor((c[d] - s.glo[d]) * s.inv_bs[d]);
     int ci = isfinite(f) ? (int)fmax(fmin(f, (double)(s.nc[d] - 1)), 0.0) : 0;
@@ -278,6 +295,7 @@ __global__ void k_wrap_key(int n, double4 *x, const int *sys, const SysDev *st, 
   }
   xp.x = c[0]; xp.y = c[1]; xp.z = c[2];
   x[p] = xp;
+  img[p] = (int)im;
   key[p] = (uint32_t)(s.cell_off + (cell[2] * s.nc[1] + cell[1]) * s.nc[0] + cell[0]);
   val[p] = p;
 }

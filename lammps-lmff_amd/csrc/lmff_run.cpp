@@ -8,8 +8,10 @@
 // LMFF/finish.cpp:124,162).  Also: compute pe/atom and stress/atom, dump custom (text,
 // DumpCustom::header_item, LMFF/dump_custom.cpp:422-440; atoms sorted by id), dump atom
 // (DumpAtom, LMFF/dump_atom.cpp; scaled or, with dump_modify scale no, plain), write_data
-// (WriteData, LMFF/write_data.cpp:236-420, atom_style full with velocities) and
-// reset_timestep.  Extra command: "lmff_precision fp64|mixed".
+// (WriteData, LMFF/write_data.cpp:236-420, atom_style full with image flags and velocities),
+// write_restart / read_restart / restart N file [file2] (a binary file of this runner's own
+// layout) and reset_timestep.  Image flags count box crossings as Domain::pbc does.  Extra
+// command: "lmff_precision fp64|mixed".
 //
 // usage: lmff_run -in deck [-var NAME VALUE ...]
 #include <algorithm>
@@ -285,10 +287,10 @@ struct Formula {
   }
 };
 
-// write_data: header (WriteData::header), Masses, Atoms # full (AtomVecFull::write_data; image
-// flags 0: positions are kept wrapped), Velocities (AtomVec::write_vel)
+// write_data: header (WriteData::header), Masses, Atoms # full (AtomVecFull::write_data: wrapped
+// positions and image flags), Velocities (AtomVec::write_vel)
 void write_data(const std::string &path, const Data &d, int64_t step, const std::vector<double> &x,
-                const std::vector<double> &v) {
+                const std::vector<double> &v, const std::vector<int> &img) {
   FILE *fp = std::fopen(path.c_str(), "w");
   if (!fp) die("Cannot open data file " + path);
   std::fprintf(fp, "LAMMPS data file via write_data, lmff_run, timestep = %lld\n\n", (long long)step);
@@ -302,20 +304,147 @@ void write_data(const std::string &path, const Data &d, int64_t step, const std:
   std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return d.tag[a] < d.tag[b]; });
   std::fprintf(fp, "\nAtoms # full\n\n");
   for (int64_t i : order)
-    std::fprintf(fp, "%lld %lld %d %-1.16e %-1.16e %-1.16e %-1.16e 0 0 0\n", (long long)d.tag[i], (long long)d.mol[i],
-                 d.type[i], 0.0, x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    std::fprintf(fp, "%lld %lld %d %-1.16e %-1.16e %-1.16e %-1.16e %d %d %d\n", (long long)d.tag[i],
+                 (long long)d.mol[i], d.type[i], 0.0, x[3 * i], x[3 * i + 1], x[3 * i + 2], img[3 * i], img[3 * i + 1],
+                 img[3 * i + 2]);
   std::fprintf(fp, "\nVelocities\n\n");
   for (int64_t i : order)
     std::fprintf(fp, "%lld %-1.16e %-1.16e %-1.16e\n", (long long)d.tag[i], v[3 * i], v[3 * i + 1], v[3 * i + 2]);
   std::fclose(fp);
 }
 
+// Binary restart file of the device-resident state (the role of WriteRestart::write /
+// ReadRestart::command, LMFF/write_restart.cpp:238, LMFF/read_restart.cpp:76; not LAMMPS' own
+// binary layout): the timestep, dt, boundary, box, per-type masses and, for every atom in id
+// order, id, molecule, type, image flags, wrapped position and velocity.  Little-endian, as
+// written by this host; pair settings are not stored (tersoff and ilp/graphene/hbn keep none in
+// LAMMPS restarts either, so a deck repeats pair_style and pair_coeff after read_restart).
+constexpr char RESTART_MAGIC[8] = {'L', 'M', 'F', 'F', 'R', 'S', 'T', '1'};
+constexpr char RESTART_END[8] = {'L', 'M', 'F', 'F', 'E', 'N', 'D', '1'};
+
+struct Restart {
+  Data data;
+  int64_t step = 0;
+  double dt = 0.001;
+  int periodic[3] = {1, 1, 1};
+  std::string bstr[3];
+};
+
+void write_restart(const std::string &path, const Data &d, int64_t step, double dt, const int periodic[3],
+                   const std::string bstr[3], const std::vector<double> &x, const std::vector<double> &v,
+                   const std::vector<int> &img) {
+  FILE *fp = std::fopen(path.c_str(), "wb");
+  if (!fp) die("Cannot open restart file " + path);
+  auto put = [&](const void *p, size_t nb) {
+    if (std::fwrite(p, 1, nb, fp) != nb) die("Cannot write restart file " + path);
+  };
+  const int32_t version = 1;
+  put(RESTART_MAGIC, 8);
+  put(&version, 4);
+  put(&step, 8);
+  put(&dt, 8);
+  for (int k = 0; k < 3; k++) {
+    const int32_t pk = periodic[k];
+    char b[4] = {0, 0, 0, 0};
+    std::strncpy(b, bstr[k].c_str(), 3);
+    put(&pk, 4);
+    put(b, 4);
+  }
+  put(d.lo, sizeof d.lo);
+  put(d.hi, sizeof d.hi);
+  const int32_t nt = d.ntypes;
+  put(&nt, 4);
+  for (int t = 1; t <= d.ntypes; t++) {
+    const double m = d.mass.count(t) ? d.mass.at(t) : std::nan("");
+    put(&m, 8);
+  }
+  put(&d.n, 8);
+  std::vector<int64_t> order(d.n);
+  for (int64_t i = 0; i < d.n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return d.tag[a] < d.tag[b]; });
+  for (int64_t i : order) {
+    const int32_t ty = d.type[i];
+    const int32_t im[3] = {img[3 * i], img[3 * i + 1], img[3 * i + 2]};
+    put(&d.tag[i], 8);
+    put(&d.mol[i], 8);
+    put(&ty, 4);
+    put(im, 12);
+    put(&x[3 * i], 24);
+    put(&v[3 * i], 24);
+  }
+  put(RESTART_END, 8);
+  if (std::fclose(fp) != 0) die("Cannot write restart file " + path);
+}
+
+Restart read_restart(const std::string &path) {
+  FILE *fp = std::fopen(path.c_str(), "rb");
+  if (!fp) die("Cannot open restart file " + path);
+  auto get = [&](void *p, size_t nb) {
+    if (std::fread(p, 1, nb, fp) != nb) die("Unexpected end of restart file " + path);
+  };
+  char magic[8];
+  get(magic, 8);
+  if (std::memcmp(magic, RESTART_MAGIC, 8) != 0) die("Invalid restart file " + path + " (not written by lmff_run)");
+  int32_t version = 0;
+  get(&version, 4);
+  if (version != 1) die("Restart file " + path + " has an unsupported version");
+  Restart r;
+  get(&r.step, 8);
+  get(&r.dt, 8);
+  for (int k = 0; k < 3; k++) {
+    int32_t pk = 0;
+    char b[4];
+    get(&pk, 4);
+    get(b, 4);
+    b[3] = 0;
+    r.periodic[k] = pk ? 1 : 0;
+    r.bstr[k] = b;
+  }
+  Data &d = r.data;
+  get(d.lo, sizeof d.lo);
+  get(d.hi, sizeof d.hi);
+  int32_t nt = 0;
+  get(&nt, 4);
+  if (nt < 1 || nt > 64) die("Invalid number of atom types in restart file " + path);
+  d.ntypes = nt;
+  for (int t = 1; t <= nt; t++) {
+    double m = 0;
+    get(&m, 8);
+    if (!std::isnan(m)) d.mass[t] = m;
+  }
+  get(&d.n, 8);
+  if (d.n < 0) die("Invalid atom count in restart file " + path);
+  d.x.resize(3 * d.n);
+  d.v.resize(3 * d.n);
+  d.img.resize(3 * d.n);
+  d.type.resize(d.n);
+  d.tag.resize(d.n);
+  d.mol.resize(d.n);
+  for (int64_t i = 0; i < d.n; i++) {
+    int32_t ty = 0, im[3];
+    get(&d.tag[i], 8);
+    get(&d.mol[i], 8);
+    get(&ty, 4);
+    get(im, 12);
+    get(&d.x[3 * i], 24);
+    get(&d.v[3 * i], 24);
+    if (ty < 1 || ty > nt) die("Invalid atom type in restart file " + path);
+    d.type[i] = ty;
+    for (int k = 0; k < 3; k++) d.img[3 * i + k] = im[k];
+  }
+  get(magic, 8);
+  if (std::memcmp(magic, RESTART_END, 8) != 0) die("Restart file " + path + " is corrupted");
+  std::fclose(fp);
+  return r;
+}
+
 // one snapshot of dump custom: columns id mol type x y z vx vy vz fx fy fz and c_ID / c_ID[k] /
 // c_ID[*] of compute pe/atom (eV) and stress/atom (compute_stress_atom.cpp:254-340: minus
 // (m v_a v_b + per-atom virial) in pressure*volume units, order xx yy zz xy xz yz)
 void write_dump(Dump &dp, const Data &d, int64_t step, const std::string &boundstr, const std::vector<double> &x,
-                const std::vector<double> &v, const std::vector<double> &f, const std::vector<double> &eatom,
-                const std::vector<double> &vatom, const std::map<std::string, std::string> &computes) {
+                const std::vector<double> &v, const std::vector<double> &f, const std::vector<int> &img,
+                const std::vector<double> &eatom, const std::vector<double> &vatom,
+                const std::map<std::string, std::string> &computes) {
   if (!dp.fp) {
     dp.fp = std::fopen(dp.file.c_str(), "w");
     if (!dp.fp) die("Cannot open dump file " + dp.file);
@@ -365,8 +494,14 @@ void write_dump(Dump &dp, const Data &d, int64_t step, const std::string &bounds
       if (c == "id") { std::fprintf(fp, "%lld", (long long)d.tag[i]); continue; }
       if (c == "mol") { std::fprintf(fp, "%lld", (long long)d.mol[i]); continue; }
       if (c == "type") { std::fprintf(fp, "%d", d.type[i]); continue; }
+      if (c == "ix" || c == "iy" || c == "iz") { std::fprintf(fp, "%d", img[3 * i + (c[1] - 'x')]); continue; }
       double val = 0.0;
       if (c == "x" || c == "y" || c == "z") val = x[3 * i + (c[0] - 'x')];
+      else if (c == "xu" || c == "yu" || c == "zu") {
+        // unwrapped (DumpCustom::pack_xu, Domain::unmap): x + image * box length
+        const int k = c[0] - 'x';
+        val = x[3 * i + k] + img[3 * i + k] * (d.hi[k] - d.lo[k]);
+      }
       else if (c == "vx" || c == "vy" || c == "vz") val = v[3 * i + (c[1] - 'x')];
       else if (c == "fx" || c == "fy" || c == "fz") val = f[3 * i + (c[1] - 'x')];
       else {
@@ -420,6 +555,9 @@ int main(int argc, char **argv) {
   std::map<std::string, std::string> computes;  // id -> pe/atom | stress/atom
   std::vector<Dump> dumps;
   std::string bstr[3] = {"p", "p", "p"};
+  int64_t restart_every = 0;  // restart N file [file2] (Output::restart): written at multiples of N
+  std::vector<std::string> restart_files;
+  int restart_toggle = 0;
   std::string min_style = "cg";
   lmff_fire_params fire;
   lmff_fire_defaults(&fire);
@@ -479,6 +617,29 @@ int main(int argc, char **argv) {
                         data.x.data(), data.v.data(), &sys))
       die(lmff_last_error(ctx));
     if (step > 0 && lmff_system_set_step(ctx, sys, step)) die(lmff_last_error(ctx));
+    if (data.img.size() == 3 * (size_t)data.n && data.n > 0 && lmff_system_set_image(ctx, sys, data.img.data()))
+      die(lmff_last_error(ctx));
+  };
+  // the current positions, velocities and image flags: from the device once a system is
+  // resident, from the data read otherwise
+  auto current = [&](std::vector<double> &x, std::vector<double> &v, std::vector<int> &im) {
+    x = data.x;
+    v = data.v;
+    im = data.img;
+    im.resize(3 * data.n, 0);
+    if (!pair) return;
+    lmff_ctx *ctx = pair->context();
+    if (lmff_system_get(ctx, sys, x.data(), v.data(), nullptr)) die(lmff_last_error(ctx));
+    if (data.n > 0 && lmff_system_get_image(ctx, sys, im.data())) die(lmff_last_error(ctx));
+  };
+  auto restart_now = [&](const std::string &file) {
+    std::vector<double> x, v;
+    std::vector<int> im;
+    current(x, v, im);
+    std::string f = file;
+    const size_t star = f.find('*');
+    if (star != std::string::npos) f = f.substr(0, star) + std::to_string(step) + f.substr(star + 1);
+    write_restart(f, data, step, dt, periodic, bstr, x, v, im);
   };
 
   std::string line;
@@ -605,11 +766,13 @@ int main(int argc, char **argv) {
       std::string boundstr;
       for (int k = 0; k < 3; k++) boundstr += (k ? " " : "") + (bstr[k].size() == 1 ? bstr[k] + bstr[k] : bstr[k]);
       std::vector<double> hx(3 * data.n), hv(3 * data.n), hf(3 * data.n), he, hw;
+      std::vector<int> him(3 * data.n, 0);
       auto snapshot = [&](int64_t at) {
         bool any = false;
         for (const auto &dp : dumps) any = any || (at % dp.every == 0 && dp.last != at);
         if (!any) return;
         if (lmff_system_get(ctx, sys, hx.data(), hv.data(), hf.data())) die(lmff_last_error(ctx));
+        if (data.n > 0 && lmff_system_get_image(ctx, sys, him.data())) die(lmff_last_error(ctx));
         if (want_peratom) {
           he.assign(data.n, 0.0);
           hw.assign(6 * data.n, 0.0);
@@ -617,7 +780,7 @@ int main(int argc, char **argv) {
         }
         for (auto &dp : dumps)
           if (at % dp.every == 0 && dp.last != at) {
-            write_dump(dp, data, at, boundstr, hx, hv, hf, he, hw, computes);
+            write_dump(dp, data, at, boundstr, hx, hv, hf, him, he, hw, computes);
             dp.last = at;
           }
       };
@@ -641,6 +804,7 @@ int main(int argc, char **argv) {
       while (at < end) {
         int64_t next = std::min(end, (at / te + 1) * te);
         for (const auto &dp : dumps) next = std::min(next, (at / dp.every + 1) * dp.every);
+        if (restart_every > 0) next = std::min(next, (at / restart_every + 1) * restart_every);
         const int64_t len = next - at;
         lmff_thermo t;
         auto c0 = std::chrono::steady_clock::now();
@@ -650,6 +814,11 @@ int main(int argc, char **argv) {
         if (at % te == 0 || at == end)
           std::printf("%8lld %12.6g %16.10g %16.10g %12.6g\n", (long long)at, t.temp, t.pe, t.pe + t.ke, press(t));
         snapshot(at);
+        if (restart_every > 0 && at % restart_every == 0) {
+          step = at;
+          restart_now(restart_files[restart_files.size() == 2 ? restart_toggle : 0]);
+          restart_toggle ^= 1;
+        }
       }
       if (lmff_peratom_on_last_step(ctx, 0)) die(lmff_last_error(ctx));
       step = end;
@@ -675,7 +844,8 @@ int main(int argc, char **argv) {
       if (dp.style == "atom" && w.size() > 6) die("Illegal dump atom command");
       for (size_t k = 6; k < w.size(); k++) {
         const std::string &c = w[k];
-        static const char *plain[] = {"id", "mol", "type", "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"};
+        static const char *plain[] = {"id", "mol", "type", "x", "y", "z", "xu", "yu", "zu", "ix", "iy", "iz",
+                                      "vx", "vy", "vz", "fx", "fy", "fz"};
         bool ok = std::find_if(std::begin(plain), std::end(plain), [&](const char *p) { return c == p; }) != std::end(plain);
         if (!ok && c.rfind("c_", 0) == 0) {
           const size_t br = c.find('[');
@@ -718,9 +888,32 @@ int main(int argc, char **argv) {
       if (pair && lmff_system_set_step(pair->context(), sys, step)) die(lmff_last_error(pair->context()));
     } else if (cmd == "write_data") {
       if (!have_data) die("Write_data command before simulation box is defined");
-      std::vector<double> x = data.x, v = data.v;
-      if (pair && lmff_system_get(pair->context(), sys, x.data(), v.data(), nullptr)) die(lmff_last_error(pair->context()));
-      write_data(w.at(1), data, step, x, v);
+      std::vector<double> x, v;
+      std::vector<int> im;
+      current(x, v, im);
+      write_data(w.at(1), data, step, x, v, im);
+    } else if (cmd == "write_restart") {
+      if (!have_data) die("Write_restart command before simulation box is defined");
+      restart_now(w.at(1));
+    } else if (cmd == "restart") {
+      restart_every = std::stoll(w.at(1));
+      if (restart_every < 0) die("Illegal restart command");
+      restart_files.assign(w.begin() + 2, w.end());
+      if (restart_every > 0 && (restart_files.empty() || restart_files.size() > 2)) die("Illegal restart command");
+      if (restart_files.size() == 2 && restart_files[0].find('*') != std::string::npos) die("Illegal restart command");
+      restart_toggle = 0;
+    } else if (cmd == "read_restart") {
+      if (have_data) die("Cannot read_restart after simulation box is defined");
+      Restart r = read_restart(w.at(1));
+      data = std::move(r.data);
+      step = r.step;
+      dt = r.dt;
+      for (int k = 0; k < 3; k++) {
+        periodic[k] = r.periodic[k];
+        bstr[k] = r.bstr[k];
+      }
+      have_data = true;
+      std::printf("  %lld atoms\n", (long long)data.n);
     } else if (cmd == "min_style") {
       min_style = w.at(1);
     } else if (cmd == "min_modify") {

@@ -30,7 +30,7 @@ SYMBOLS = [
     "lmff_system_peratom", "lmff_evaluate", "lmff_system_thermo", "lmff_run", "lmff_profile",
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
     "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
-    "lmff_minimize", "lmff_minimize_cg",
+    "lmff_minimize", "lmff_minimize_cg", "lmff_system_get_image", "lmff_system_set_image",
 ]
 
 STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
@@ -109,6 +109,8 @@ def load(path: str = LIB_PATH):
     L.lmff_system_natoms.argtypes = [vp, C.c_int, _i64]
     L.lmff_system_get.argtypes = [vp, C.c_int, _d, _d, _d]
     L.lmff_system_set.argtypes = [vp, C.c_int, _d, _d]
+    L.lmff_system_get_image.argtypes = [vp, C.c_int, _i32]
+    L.lmff_system_set_image.argtypes = [vp, C.c_int, _i32]
     L.lmff_system_peratom.argtypes = [vp, C.c_int, _d, _d]
     L.lmff_evaluate.argtypes = [vp, C.c_int, C.c_int]
     L.lmff_system_thermo.argtypes = [vp, C.c_int, C.POINTER(Thermo)]
@@ -264,6 +266,16 @@ class Context:
         x = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
         v = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
         self._chk(self.L.lmff_system_set(self.h, int(sid), _p(x, C.c_double), _p(v, C.c_double)))
+
+    def get_image(self, sid: int):
+        """Image flags [n][3] (box crossings counted when positions are wrapped)."""
+        im = np.zeros((self.natoms(sid), 3), dtype=np.int32)
+        self._chk(self.L.lmff_system_get_image(self.h, int(sid), _p(im, C.c_int32)))
+        return im
+
+    def set_image(self, sid: int, image):
+        im = np.ascontiguousarray(image, dtype=np.int32)
+        self._chk(self.L.lmff_system_set_image(self.h, int(sid), _p(im, C.c_int32)))
 
     def peratom(self, sid: int, energy=True, virial=True):
         n = self.natoms(sid)

@@ -269,7 +269,8 @@ def read_lammps_data(path: str) -> System:
 
     Follows the section layout LAMMPS' ``read_data`` accepts (LMFF/read_data.cpp:116):
     header counts and box bounds, ``Masses``, ``Atoms # full`` rows
-    ``id mol type q x y z [ix iy iz]`` and ``Velocities`` rows ``id vx vy vz``.
+    ``id mol type q x y z [ix iy iz]`` and ``Velocities`` rows ``id vx vy vz``.  The image flags
+    (0 where a row has none) are returned as ``meta["image"]``, [n][3] in id order.
     """
     with open(path) as fh:
         lines = fh.read().splitlines()
@@ -310,9 +311,12 @@ def read_lammps_data(path: str) -> System:
     mol = np.array([int(a[1]) for a in atoms], dtype=np.int64)
     typ = np.array([int(a[2]) for a in atoms], dtype=np.int32)
     x = np.array([[float(a[4]), float(a[5]), float(a[6])] for a in atoms])
+    img = np.array([[int(w) for w in a[7:10]] if len(a) >= 10 else [0, 0, 0] for a in atoms],
+                   dtype=np.int32).reshape(-1, 3)
     order = np.argsort(ids, kind="stable")
     s = System(x=x[order], type=typ[order], tag=ids[order], mol=mol[order], boxlo=lo, boxhi=hi,
-               periodic=(1, 1, 1), elements=["C"] * max(ntypes, 1), meta={"masses": masses})
+               periodic=(1, 1, 1), elements=["C"] * max(ntypes, 1),
+               meta={"masses": masses, "image": img[order]})
     if vel:
         s.v = np.array([vel[int(t)] for t in s.tag])
     return s
@@ -321,9 +325,13 @@ def read_lammps_data(path: str) -> System:
 def write_lammps_data(s: System, path: str, step: int = 0) -> None:
     """Writes ``s`` as an ``atom_style full`` data file in the layout of LAMMPS' ``write_data``
     (LMFF/write_data.cpp:236-420, AtomVecFull::write_data): header with the timestep, counts,
-    box, ``Masses``, ``Atoms # full`` rows ``id mol type q x y z ix iy iz`` (q and image flags
-    0) and ``Velocities``, numbers in ``%-1.16e`` so that a read gives the same doubles back."""
+    box, ``Masses``, ``Atoms # full`` rows ``id mol type q x y z ix iy iz`` (q 0; image flags from
+    ``meta["image"]`` when present, else 0) and ``Velocities``, numbers in ``%-1.16e`` so that a
+    read gives the same doubles back."""
     order = np.argsort(s.tag, kind="stable")
+    img = np.asarray(s.meta.get("image", ()), dtype=np.int64).reshape(-1, 3)
+    if img.shape[0] != s.n:  # none, or left over from a system of another size
+        img = np.zeros((s.n, 3), dtype=np.int64)
     with open(path, "w") as fh:
         fh.write(f"LAMMPS data file via write_data, lmff, timestep = {int(step)}\n\n")
         fh.write(f"{s.n} atoms\n{len(s.elements)} atom types\n\n")
@@ -336,7 +344,7 @@ def write_lammps_data(s: System, path: str, step: int = 0) -> None:
         for i in order:
             x = s.x[i]
             fh.write(f"{int(s.tag[i])} {int(s.mol[i])} {int(s.type[i])} {0.0:<1.16e} "
-                     f"{x[0]:<1.16e} {x[1]:<1.16e} {x[2]:<1.16e} 0 0 0\n")
+                     f"{x[0]:<1.16e} {x[1]:<1.16e} {x[2]:<1.16e} {img[i, 0]} {img[i, 1]} {img[i, 2]}\n")
         if s.v is not None:
             fh.write("\nVelocities\n\n")
             for i in order:

@@ -99,3 +99,60 @@ def test_replicate_numbering_and_positions(tmp_path):
         assert out.mol[k] == m and out.type[k] == t
         np.testing.assert_allclose(out.x[k], p, rtol=0, atol=1e-12)
         np.testing.assert_array_equal(out.v[k], v)
+
+
+def _small_data(path, images):
+    """Four atoms in a 10 x 12 x 20 box with velocities and the given image flags."""
+    rows = [(3, 1, 1, 9.5, 1.0, 0.0), (7, 1, 2, 0.5, 11.9, 0.2), (5, 2, 1, 4.0, 6.0, 3.3), (1, 0, 2, 8.0, 2.0, 3.4)]
+    with open(path, "w") as fh:
+        fh.write("test\n\n4 atoms\n2 atom types\n\n0 10 xlo xhi\n0 12 ylo yhi\n-5 15 zlo zhi\n")
+        fh.write("\nMasses\n\n1 12.0107\n2 10.811\n\nAtoms # full\n\n")
+        for (i, m, t, x, y, z), im in zip(rows, images):
+            fh.write(f"{i} {m} {t} 0.0 {x!r} {y!r} {z!r} {im[0]} {im[1]} {im[2]}\n")
+        fh.write("\nVelocities\n\n")
+        for k, (i, *_rest) in enumerate(rows):
+            fh.write(f"{i} {0.1 * k + 0.013!r} {-0.2 * k!r} {0.7 / (k + 1)!r}\n")
+
+
+def test_write_restart_read_restart_round_trip(tmp_path):
+    """write_restart -> read_restart keeps the timestep, dt, box, masses, ids, molecules,
+    types, image flags, positions and velocities exactly (read back through write_data, whose
+    Atoms rows now carry the image flags read_data was given)."""
+    images = [(0, 0, 0), (-1, 2, 0), (3, 0, -1), (0, -511, 511)]
+    _small_data(tmp_path / "in.data", images)
+    r = run(tmp_path, "\n".join([
+        "units metal", "atom_style full", "boundary p p f", "read_data in.data", "mass 2 10.5",
+        "reset_timestep 1234", "timestep 0.002", "write_data direct.data", "write_restart r.bin", ""]))
+    assert r.returncode == 0, r.stdout + r.stderr
+    (tmp_path / "in2.deck").write_text("read_restart r.bin\nwrite_data back.data\n")
+    r2 = subprocess.run([RUNNER, "-in", "in2.deck"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    a = S.read_lammps_data(str(tmp_path / "in.data"))
+    for name in ("direct.data", "back.data"):
+        b = S.read_lammps_data(str(tmp_path / name))
+        assert (tmp_path / name).read_text().splitlines()[0].endswith("timestep = 1234")
+        np.testing.assert_array_equal(b.tag, a.tag)
+        np.testing.assert_array_equal(b.mol, a.mol)
+        np.testing.assert_array_equal(b.type, a.type)
+        np.testing.assert_array_equal(b.x, a.x)
+        np.testing.assert_array_equal(b.v, a.v)
+        np.testing.assert_array_equal(b.meta["image"], a.meta["image"])
+        np.testing.assert_array_equal(b.boxhi, a.boxhi)
+        assert b.meta["masses"] == {1: 12.0107, 2: 10.5}
+
+
+def test_restart_errors(tmp_path):
+    _small_data(tmp_path / "in.data", [(0, 0, 0)] * 4)
+    (tmp_path / "junk.bin").write_bytes(b"not a restart file at all")
+    r = run(tmp_path, "read_restart junk.bin\n")
+    assert r.returncode != 0 and "Invalid restart file" in r.stderr
+    r = run(tmp_path, "read_data in.data\nread_restart junk.bin\n")
+    assert r.returncode != 0 and "Cannot read_restart after simulation box is defined" in r.stderr
+    r = run(tmp_path, "read_data in.data\nwrite_restart r.bin\n")
+    assert r.returncode == 0, r.stderr
+    data = (tmp_path / "r.bin").read_bytes()
+    (tmp_path / "cut.bin").write_bytes(data[: len(data) - 20])
+    r = run(tmp_path, "read_restart cut.bin\n")
+    assert r.returncode != 0 and "Unexpected end of restart file" in r.stderr
+    r = run(tmp_path, "read_data in.data\nrestart 10\n")
+    assert r.returncode != 0 and "Illegal restart command" in r.stderr

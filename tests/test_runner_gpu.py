@@ -229,3 +229,44 @@ def test_dump_atom_style(tmp_path):
         assert np.array_equal(a[2][:, :2], c[2][:, :2])
         np.testing.assert_allclose(lo + a[2][:, 2:5] * L, c[2][:, 2:5], rtol=0, atol=1e-5 * L.max())
         np.testing.assert_allclose(b[2][:, 2:5], c[2][:, 2:5], rtol=1e-5, atol=1e-5)
+
+
+def test_image_flags_and_binary_restart(tmp_path):
+    """Atoms drifting across the periodic x and y boundaries: the dumped image flags satisfy
+    xu = x + ix * Lx and the unwrapped positions follow the drift (a missed or doubled crossing
+    would be off by a box length); write_restart at step 100 + read_restart continues to the same
+    state and image flags as the uninterrupted 200-step run."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=81)
+    drift = np.array([25.0, -15.0, 0.0])  # A/ps: 5 A and -3 A over 0.2 ps
+    s.v = s.v + drift
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    run_deck(tmp_path, "a.in", deck(data, "", "\n".join([
+        "dump d all custom 200 a.dump id x y z xu yu zu ix iy iz", "dump_modify d format float %.17g",
+        "thermo 100", "run 200", "write_data a.data"])))
+    run_deck(tmp_path, "b.in", deck(data, "", "\n".join(["thermo 100", "run 100", "write_restart r.bin"])))
+    run_deck(tmp_path, "c.in", "\n".join([
+        "units metal", "atom_style full", "read_restart r.bin",
+        "pair_style hybrid/overlay tersoff ilp/graphene/hbn 16.0 1",
+        f"pair_coeff * * tersoff {os.path.join(POT, 'BNC.tersoff')} C B N",
+        f"pair_coeff * * ilp/graphene/hbn {os.path.join(POT, 'BNCH.ILP')} C B N",
+        "neighbor 2.0 bin", "neigh_modify every 10 delay 0 check no",
+        "fix 1 all nve", "thermo 100", "run 100", "write_data c.data", ""]))
+    frames = S.read_dump_custom(str(tmp_path / "a.dump"))
+    assert [f[0] for f in frames] == [0, 200]
+    a = frames[1][2]
+    L = s.boxhi - s.boxlo
+    img = a[:, 7:10].astype(np.int64)
+    np.testing.assert_allclose(a[:, 4:7], a[:, 1:4] + img * L, rtol=0, atol=1e-9)
+    order = np.argsort(s.tag)
+    expect = s.x[order] + drift * 0.2
+    assert np.abs(a[:, 4:6] - expect[:, :2]).max() < 1.0
+    assert (img[:, 0] != 0).sum() >= 20 and (img[:, 1] != 0).sum() >= 20 and not img[:, 2].any()
+    wa = S.read_lammps_data(str(tmp_path / "a.data"))
+    np.testing.assert_array_equal(wa.meta["image"], img)
+    wc = S.read_lammps_data(str(tmp_path / "c.data"))
+    assert open(tmp_path / "c.data").readline().strip().endswith("timestep = 200")
+    np.testing.assert_array_equal(wc.meta["image"], wa.meta["image"])
+    assert np.abs(wc.x - wa.x).max() < 1e-8 and np.abs(wc.v - wa.v).max() < 1e-8
