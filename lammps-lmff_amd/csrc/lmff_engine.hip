@@ -81,6 +81,24 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// page-locked host memory (hipHostMalloc), grown on demand, for device-to-host copies at
+// full PCIe rate
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    HIPCHK(hipHostMalloc(&p, need, hipHostMallocDefault));
+    bytes = need;
+  }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct SysHost {
   int id;
   bool live;
@@ -142,6 +160,8 @@ struct Ctx {
   DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
   DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
+  DBuf hb_in, hb_out;  // host-buffer Pair::compute: caller-order arrays on the device
+  PinnedBuf hb_host;   // and the forces on their way back
   int ntp = 0;  // Tersoff parameter records
   bool ilp_sym = false;  // every mapped ILP type pair symmetric (k_ilp's SYM variant applies)
   // conjugate-gradient vectors (x0, g, h), carried through rebuilds while a CG minimisation runs
@@ -181,7 +201,7 @@ struct Ctx {
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
-                   &cg_x0, &cg_g, &cg_h};
+                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out};
     for (DBuf *b : all) b->tally = &bytes_in_use;
   }
   ~Ctx() {
@@ -1775,10 +1795,55 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
   return guard(ctx, [&] {
     auto &c = ctx->c;
     if (nlocal < 0 || (nlocal > 0 && !x)) fail(LMFF_ERR_ARG, "bad atom arrays");
-    const bool fresh = new_list || c.pair_sys < 0 || c.pair_n != nlocal || !c.find(c.pair_sys);
-    if (fresh) {
-      if (c.pair_sys >= 0 && c.find(c.pair_sys)) {
-        c.find(c.pair_sys)->live = false;
+    SysHost *ps = c.pair_sys >= 0 ? c.find(c.pair_sys) : nullptr;
+    const bool fresh = new_list || !ps || c.pair_n != nlocal;
+    const int nb = (int)((nlocal + BS - 1) / BS);
+    // a fresh list for the same number of atoms in the same box reloads the system where it
+    // is (no re-pack of the context); the next evaluation sorts and lists it
+    bool in_place = false;
+    if (fresh && ps && !c.layout_dirty && c.pair_n == nlocal && nlocal > 0 && type && tag && mol) {
+      in_place = true;
+      for (int d = 0; d < 3; d++)
+        if (ps->lo[d] != boxlo[d] || ps->hi[d] != boxhi[d] || ps->per[d] != (periodic[d] ? 1 : 0)) in_place = false;
+      if (in_place) {
+        SysHost g = *ps;
+        setup_grid(g, x, nlocal, c.cutneigh);
+        for (int d = 0; d < 3; d++)
+          if (g.nc[d] != ps->nc[d]) in_place = false;
+        if (in_place) {
+          for (int64_t i = 0; i < nlocal; i++) {
+            if (type[i] < 1 || type[i] > c.ntypes) fail(LMFF_ERR_ARG, "Invalid atom type %d", (int)type[i]);
+            if (tag[i] <= 0 || tag[i] > INT32_MAX) fail(LMFF_ERR_ARG, "atom IDs must be in 1..2^31-1");
+            if (mol[i] < INT32_MIN || mol[i] > INT32_MAX) fail(LMFF_ERR_ARG, "molecule IDs must fit in 32 bits");
+            for (int d = 0; d < 3; d++)
+              if (!std::isfinite(x[3 * i + d])) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords");
+          }
+          for (int d = 0; d < 3; d++) {
+            ps->glo[d] = g.glo[d];
+            ps->bs[d] = g.bs[d];
+          }
+          c.upload_systab();
+          const size_t m = (size_t)nlocal;
+          c.hb_in.ensure(m * (3 * sizeof(double) + 2 * sizeof(int64_t) + sizeof(int32_t)));
+          double *dx = c.hb_in.as<double>();
+          int64_t *dtag = (int64_t *)(dx + 3 * m), *dmol = dtag + m;
+          int32_t *dtype = (int32_t *)(dmol + m);
+          HIPCHK(hipMemcpyAsync(dx, x, m * 3 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+          HIPCHK(hipMemcpyAsync(dtag, tag, m * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+          HIPCHK(hipMemcpyAsync(dmol, mol, m * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+          HIPCHK(hipMemcpyAsync(dtype, type, m * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+          const int b = ps->begin;
+          hipLaunchKernelGGL(k_hb_load, dim3(nb), dim3(BS), 0, c.stream, (int)m, dx, dtype, (const long long *)dtag,
+                             (const long long *)dmol, c.x.as<double4>() + b, c.v.as<double4>() + b, c.tag.as<int>() + b,
+                             c.mol.as<int>() + b, c.orig.as<int>() + b, c.img.as<int>() + b);
+          c.need_rebuild = true;
+          c.forces_valid = false;
+        }
+      }
+    }
+    if (fresh && !in_place) {
+      if (ps) {
+        ps->live = false;
         c.layout_dirty = true;
       }
       int id = -1;
@@ -1787,18 +1852,14 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
       c.pair_sys = id;
       c.pair_n = nlocal;
       c.relayout();
-    } else {
-      SysHost *s = c.find(c.pair_sys);
-      const size_t m = (size_t)(s->end - s->begin);
-      auto org = c.download<int>(c.orig, s->begin, m);
-      auto h = c.download<double4>(c.x, s->begin, m);
-      for (size_t q = 0; q < m; q++) {
-        const size_t i = (size_t)org[q];
-        h[q].x = x[3 * i];
-        h[q].y = x[3 * i + 1];
-        h[q].z = x[3 * i + 2];
-      }
-      HIPCHK(hipMemcpy(c.x.as<double4>() + s->begin, h.data(), m * sizeof(double4), hipMemcpyHostToDevice));
+    } else if (!fresh && nlocal > 0) {
+      // positions only: caller order -> the system's sorted slots, on the device
+      const size_t m = (size_t)nlocal;
+      c.hb_in.ensure(m * 3 * sizeof(double));
+      HIPCHK(hipMemcpyAsync(c.hb_in.p, x, m * 3 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      hipLaunchKernelGGL(k_hb_scatter_x, dim3(nb), dim3(BS), 0, c.stream, (int)m, c.orig.as<int>() + ps->begin,
+                         c.img.as<int>() + ps->begin, ps->hi[0] - ps->lo[0], ps->hi[1] - ps->lo[1],
+                         ps->hi[2] - ps->lo[2], c.hb_in.as<double>(), c.x.as<double4>() + ps->begin);
       c.far_check();
     }
     c.evaluate(eflag, vflag);
@@ -1808,16 +1869,19 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
     for (size_t q = 0; q < c.sys.size(); q++)
       if (c.sys[q].id == c.pair_sys) k = (int)q;
     const size_t m = (size_t)(s->end - s->begin);
-    auto org = c.download<int>(c.orig, s->begin, m);
-    if (f) {
-      auto h = c.download<double4>(c.f, s->begin, m);
-      for (size_t q = 0; q < m; q++) {
-        const size_t i = (size_t)org[q];
-        f[3 * i] += h[q].x;
-        f[3 * i + 1] += h[q].y;
-        f[3 * i + 2] += h[q].z;
-      }
+    if (f && m > 0) {
+      // forces gathered into caller order on the device, one copy into page-locked memory
+      c.hb_out.ensure(m * 3 * sizeof(double));
+      c.hb_host.ensure(m * 3 * sizeof(double));
+      hipLaunchKernelGGL(k_hb_gather_f, dim3((int)((m + BS - 1) / BS)), dim3(BS), 0, c.stream, (int)m,
+                         c.orig.as<int>() + s->begin, c.f.as<double4>() + s->begin, c.hb_out.as<double>());
+      HIPCHK(hipMemcpyAsync(c.hb_host.p, c.hb_out.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      const double *h = (const double *)c.hb_host.p;
+      for (size_t q = 0; q < 3 * m; q++) f[q] += h[q];
     }
+    std::vector<int> org;
+    if ((eatom && (eflag & 2)) || (vatom && (vflag & 4))) org = c.download<int>(c.orig, s->begin, m);
     double h9[9] = {0};
     if (eflag || (vflag % 4)) HIPCHK(hipMemcpy(h9, c.sums.as<double>() + (size_t)k * 9, sizeof h9, hipMemcpyDeviceToHost));
     if (eng_vdwl) *eng_vdwl = (eflag & 1) ? h9[0] + h9[1] + h9[2] : 0.0;

@@ -311,6 +311,50 @@ __global__ void k_permute(int n, const int *perm, const T *src, T *dst) {
   if (q < n) dst[q] = src[perm[q]];
 }
 
+// Host-buffer Pair::compute: orig[q] is the caller's index of the q-th atom of the system in
+// the device's cell-sorted order.  Positions in caller order are scattered into place (type in
+// w kept), forces gathered back into caller order, and a fresh set of atoms is loaded in caller
+// order (the next list build sorts it).  The image flags of a loaded system count the wraps of
+// its last list build, so a caller position that the build moved by a box length is shifted
+// the same way here and stays consistent with the list's image shifts (a LAMMPS caller wraps
+// before it asks for a new list and never needs it).
+__global__ void k_hb_scatter_x(int m, const int *orig, const int *img, double prd0, double prd1, double prd2,
+                               const double *xin, double4 *x) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t i = (size_t)orig[q];
+  const uint32_t im = (uint32_t)img[q];
+  const int ix = (int)(im & IMG_MASK) - IMG_MAX, iy = (int)((im >> IMG_BITS) & IMG_MASK) - IMG_MAX,
+            iz = (int)(im >> (2 * IMG_BITS)) - IMG_MAX;
+  double4 p = x[q];
+  p.x = xin[3 * i] - ix * prd0;
+  p.y = xin[3 * i + 1] - iy * prd1;
+  p.z = xin[3 * i + 2] - iz * prd2;
+  x[q] = p;
+}
+
+__global__ void k_hb_gather_f(int m, const int *orig, const double4 *f, double *fout) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t i = (size_t)orig[q];
+  const double4 a = f[q];
+  fout[3 * i] = a.x;
+  fout[3 * i + 1] = a.y;
+  fout[3 * i + 2] = a.z;
+}
+
+__global__ void k_hb_load(int m, const double *xin, const int *type, const long long *tag, const long long *mol,
+                          double4 *x, double4 *v, int *tagd, int *mold, int *orig, int *img) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  x[q] = {xin[3 * (size_t)q], xin[3 * (size_t)q + 1], xin[3 * (size_t)q + 2], (double)type[q]};
+  v[q] = {0.0, 0.0, 0.0, 0.0};
+  tagd[q] = (int)tag[q];
+  mold[q] = (int)mol[q];
+  orig[q] = q;
+  img[q] = IMG_ZERO;
+}
+
 // cell_start[c] = first sorted position with key >= c, for c in [0, ncells]
 __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;

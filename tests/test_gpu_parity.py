@@ -421,3 +421,41 @@ def test_pair_compute_list_reuse_follows_the_oracle():
     assert np.sqrt((disp ** 2).sum(1)).max() > 0.5
     assert np.abs(d).max() < 1e-8
     np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
+
+
+def test_pair_compute_fresh_lists_with_reordered_atoms():
+    """LAMMPS reneighbouring through the Pair boundary: a fresh list every 10 steps, each time
+    with the atoms handed over in a new order (as atom sorting and exchange reorder them), the
+    other steps reusing the list.  Fresh calls for the same atom count and box reload the
+    system in place; positions and energies follow the oracle."""
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=82, temperature=3000.0)
+    o = oracle_for(s)
+    x_ref, v_ref, th = o.run_nve(s.x, s.v, s.mass_per_type, 0.001, 30, rebuild_every=10, thermo_every=10)
+    dt, ftm2v = 0.001, 1.0 / 1.0364269e-4
+    rng = np.random.default_rng(5)
+    x, v = s.x.copy(), s.v.copy()
+    t = s.copy()
+    pe = []
+    with ctx_for(s) as ctx:
+        f = None
+        for step in range(0, 31):
+            if step > 0:
+                v += 0.5 * dt * ftm2v / t.masses()[:, None] * f
+                x += dt * v
+            fresh = step % 10 == 0
+            if fresh:
+                perm = rng.permutation(s.n)
+                x, v = x[perm], v[perm]
+                t.type, t.tag, t.mol = t.type[perm], t.tag[perm], t.mol[perm]
+            r = ctx.pair_compute(t, 1, 0, new_list=int(fresh), x=x)
+            f = r["f"]
+            if step > 0:
+                v += 0.5 * dt * ftm2v / t.masses()[:, None] * f
+            if step % 10 == 0 and step > 0:
+                pe.append(r["energy"])
+    order = np.argsort(t.tag)
+    L = s.boxhi - s.boxlo
+    d = x[order] - x_ref
+    d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
+    assert np.abs(d).max() < 1e-8
+    np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
