@@ -1818,11 +1818,13 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
             for (int d = 0; d < 3; d++)
               if (!std::isfinite(x[3 * i + d])) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords");
           }
+          c.sync_steps_from_device();  // other systems' step counters live on the device
           for (int d = 0; d < 3; d++) {
             ps->glo[d] = g.glo[d];
             ps->bs[d] = g.bs[d];
           }
           c.upload_systab();
+          c.gen++;
           const size_t m = (size_t)nlocal;
           c.hb_in.ensure(m * (3 * sizeof(double) + 2 * sizeof(int64_t) + sizeof(int32_t)));
           double *dx = c.hb_in.as<double>();
