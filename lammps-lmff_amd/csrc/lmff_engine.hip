@@ -164,6 +164,9 @@ struct Ctx {
   PinnedBuf hb_host;   // and the forces on their way back
   int ntp = 0;  // Tersoff parameter records
   bool ilp_sym = false;  // every mapped ILP type pair symmetric (k_ilp's SYM variant applies)
+  // every Tersoff-mapped type pair has the same repulsive parameters (A, lambda1, R, D) in the
+  // (i, j, j) and (j, i, i) entries, so the tag-parity choice between them is moot
+  bool ters_rep_sym = false;
   // conjugate-gradient vectors (x0, g, h), carried through rebuilds while a CG minimisation runs
   DBuf cg_x0, cg_g, cg_h;
   bool cg_active = false;
@@ -263,6 +266,16 @@ struct Ctx {
           if (ea < 0 || eb_ < 0 || ec < 0) continue;
           t3v[((a - 1) * MAXT + (b - 1)) * MAXT + (c - 1)] = ters.index(ea, eb_, ec);
         }
+    if (sizeof(R) != 8) return;  // decided on the FP64 values (float rounding could merge them)
+    ters_rep_sym = true;
+    for (int a = 0; a < ntypes; a++)
+      for (int b = 0; b < ntypes; b++) {
+        const int pab = t3v[(a * MAXT + b) * MAXT + b], pba = t3v[(b * MAXT + a) * MAXT + a];
+        if (pab < 0 || pba < 0) continue;
+        const TersP<R> &u = tpv[pab], &w = tpv[pba];
+        if (u.biga != w.biga || u.lam1 != w.lam1 || u.bigr != w.bigr || u.bigd != w.bigd || u.cutsq != w.cutsq)
+          ters_rep_sym = false;
+      }
   }
 
   void upload_params() {
@@ -566,6 +579,7 @@ struct Ctx {
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
+    P.ters_rep_sym = ters_rep_sym ? 1 : 0;
     P.rmass = rmass.as<double>();
     {
       const double rc = P.rcut, rc4 = rc * rc * rc * rc;

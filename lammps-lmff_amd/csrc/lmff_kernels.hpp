@@ -124,6 +124,7 @@ struct DevPtrs {
   double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
   int tap_flag;
+  int ters_rep_sym;       // repulsive Tersoff parameters equal in both orders of every type pair
   // Tap(r) = 1 + r^4 (c4 + r (c5 + r (c6 + r c7))) and dTap/dr = d7 r^3 (r - rcut)^3:
   // {c4, c5, c6, c7, d7}
   double tap_poly[5];
@@ -1260,11 +1261,15 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     // ---- repulsion, i's half; the parameter is the one of whichever atom computes the pair
     //      in the reference (tag parity), so both halves agree ----
     {
-      // the coordinates only decide between two images of the same atom
-      const int jtag = P.tag[e_idx(ej)];
-      double gx = 0.0, gy = 0.0, gz = 0.0;
-      if (jtag == itag) ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
-      const bool skip = tag_skip(itag, jtag, xi.x, xi.y, xi.z, gx, gy, gz);
+      // the coordinates only decide between two images of the same atom; with equal
+      // parameters in both orders the choice is moot and the tag is not loaded
+      bool skip = false;
+      if (!P.ters_rep_sym) {
+        const int jtag = P.tag[e_idx(ej)];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        if (jtag == itag) ghost(P.x[e_idx(ej)], ej, prd, gx, gy, gz);
+        skip = tag_skip(itag, jtag, xi.x, xi.y, xi.z, gx, gy, gz);
+      }
       const int pr = skip ? t3[(tj * MAXT + ti) * MAXT + ti] : t3[(ti * MAXT + tj) * MAXT + tj];
       const TersP<double> &p = tp[pr];
       if (rsqj < p.cutsq) {

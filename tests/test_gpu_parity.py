@@ -120,7 +120,9 @@ def test_fp64_asymmetric_pair_parameters(tmp_path):
     """Both shipped ILP files give (ti,tj) and (tj,ti) the same parameters, for which the pair
     kernel has a specialised variant.  Here the B-C and N-C rows differ from C-B and C-N (z0,
     lambda, C6, d, s_R), so the general variant runs: the vdW parameters follow the tag-parity
-    half selection and exp(-lambda (r - z0)) is taken separately for (i,j) and (j,i)."""
+    half selection and exp(-lambda (r - z0)) is taken separately for (i,j) and (j,i).  Likewise
+    the Tersoff B N N entry's repulsion (A, lambda1) differs from N B B's, so the bonds kernel
+    picks each B-N pair's repulsive parameters by tag parity as the reference does."""
     rows = open(ILP).read().splitlines()
     out = []
     for line in rows:
@@ -136,21 +138,37 @@ def test_fp64_asymmetric_pair_parameters(tmp_path):
         out.append(line)
     asym = str(tmp_path / "asym.ILP")
     open(asym, "w").write("\n".join(out) + "\n")
+    tout = []
+    for line in open(BNC).read().splitlines():
+        w = line.split()
+        if len(w) == 17 and w[:3] == ["B", "N", "N"]:
+            v = [float(t) for t in w[3:]]
+            v[12] *= 1.01   # lambda1
+            v[13] *= 0.98   # A
+            line = "B N N " + " ".join(repr(t) for t in v)
+        tout.append(line)
+    tasym = str(tmp_path / "asym.tersoff")
+    open(tasym, "w").write("\n".join(tout) + "\n")
     import oracle
 
     s = S.layered_stack(30, 20, ["G", "BN"], seed=17)
-    o = oracle.Oracle(3, BNC, s.elements, asym, s.elements, tap_flag=1, threads=8)
+    o = oracle.Oracle(3, tasym, s.elements, asym, s.elements, tap_flag=1, threads=8)
     o.set_system(s)
     o.build_list()
     r = o.compute(3, 6)
     with capi.Context(0) as ctx:
         ctx.settings(3, 16.0, 1)
-        ctx.coeff_tersoff(BNC, list(s.elements))
+        ctx.coeff_tersoff(tasym, list(s.elements))
         ctx.coeff_ilp(asym, list(s.elements))
         ctx.neighbor(2.0, 10, 0, 0)
         ctx.masses(list(s.mass_per_type))
         g = ctx.pair_compute(s, 3, 6)
     _check_fp64(g, r)
+    # the Tersoff asymmetry matters too
+    o2 = oracle.Oracle(3, BNC, s.elements, asym, s.elements, tap_flag=1, threads=8)
+    o2.set_system(s)
+    o2.build_list()
+    assert abs(r["energy"] - o2.compute(1, 0)["energy"]) > 1e-6
     assert np.abs(g["eatom"] - r["eatom"]).max() <= 1e-10 * np.abs(r["eatom"]).max()
     assert np.abs(g["vatom"] - r["vatom"]).max() <= 1e-10 * np.abs(r["vatom"]).max()
     # the asymmetry matters: the symmetric file gives a different energy
