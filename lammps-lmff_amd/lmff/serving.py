@@ -104,11 +104,11 @@ class Server:
         self._cancel.add(job_id)
 
     def snapshot(self, job_id: int):
-        """Positions and velocities of a running job now (caller's atom order)."""
+        """Positions, velocities and image flags of a running job now (caller's atom order)."""
         for job in self.resident.values():
             if job.job_id == job_id:
                 x, v, _ = self.ctx.get(job.sys_id)
-                return {"step": job.done_steps, "x": x, "v": v}
+                return {"step": job.done_steps, "x": x, "v": v, "image": self.ctx.get_image(job.sys_id)}
         raise KeyError(f"job {job_id} is not running")
 
     def close(self):
@@ -128,9 +128,10 @@ class Server:
 
     def _retire(self, job: Job, now: float):
         x, v, _ = self.ctx.get(job.sys_id)
+        img = self.ctx.get_image(job.sys_id)
         last = job.trace[-1] if job.trace else {}
         job.result = {"e_interlayer": last.get("e_vdw", float("nan")) + last.get("e_rep", float("nan")),
-                      "pe": last.get("pe"), "temp": last.get("temp"), "x": x, "v": v}
+                      "pe": last.get("pe"), "temp": last.get("temp"), "x": x, "v": v, "image": img}
         self.ctx.remove_system(job.sys_id)
         self.reserved -= self._job_bytes(job)
         del self.resident[job.sys_id]
@@ -231,6 +232,6 @@ def run_alone(job: Job, precision=capi.LMFF_MIXED, tick: int = 10, dt: float = 0
         x, v, _ = ctx.get(sid)
         last = trace[-1]
         return {"e_interlayer": last["e_vdw"] + last["e_rep"], "pe": last["pe"], "temp": last["temp"], "x": x,
-                "v": v, "trace": trace}
+                "v": v, "image": ctx.get_image(sid), "trace": trace}
     finally:
         ctx.close()

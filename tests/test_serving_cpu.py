@@ -33,6 +33,9 @@ class FakeCtx:
         s = self.systems[sid]
         return s.x, np.zeros_like(s.x), np.zeros_like(s.x)
 
+    def get_image(self, sid):
+        return np.zeros((self.systems[sid].n, 3), dtype=np.int32)
+
     def run(self, nsteps, thermo_every, use_graph=True, nsys=1):
         assert nsys == len(self.systems) and nsteps == thermo_every
         self.runs.append(sorted(self.systems))

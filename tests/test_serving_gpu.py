@@ -26,6 +26,7 @@ def test_stream_results_equal_alone_bitwise():
         a = alone[j.job_id]
         assert np.array_equal(j.result["x"], a["x"])
         assert np.array_equal(j.result["v"], a["v"])
+        assert np.array_equal(j.result["image"], a["image"])
         assert j.result["e_interlayer"] == a["e_interlayer"]
         assert j.result["e_interlayer"] < 0
     assert st["latency_p99_s"] >= st["latency_p50_s"] > 0
