@@ -515,13 +515,14 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
       ent = e_make(j, sx, sy, sz);
       if (!(sh == 0x15 && j == i)) {
         const double4 xj = P.x[j];
+        const int mj = P.mol[j];  // issued with the position, not after the distance test
         const double gx = __dadd_rn(xj.x, (double)sx * prd0);
         const double gy = __dadd_rn(xj.y, (double)sy * prd1);
         const double gz = __dadd_rn(xj.z, (double)sz * prd2);
         const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
         const double rsq = rsq_ref(delx, dely, delz);
         const int tj = (int)xj.w - 1;
-        const bool ilp = (rsq <= P.cutneighsq) && (P.mol[j] != mi) && ((ilp_mask >> tj) & 1u);
+        const bool ilp = (rsq <= P.cutneighsq) && (mj != mi) && ((ilp_mask >> tj) & 1u);
         in_i = ilp && rsq < P.ilp_splitsq;
         in_o = ilp && !(rsq < P.ilp_splitsq);
         in_s = (rsq <= P.shortsq) && rsq < P.short_splitsq;
@@ -682,6 +683,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
   for (int k0 = 0; k0 < cnt; k0 += NB) {
     uint32_t eb_[NB];
     double4 xb_[NB];
+    int mb_[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int k = k0 + b;
@@ -689,7 +691,10 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
     }
 #pragma unroll
     for (int b = 0; b < NB; b++)
-      if (k0 + b < cnt) xb_[b] = P.x[e_idx(eb_[b])];
+      if (k0 + b < cnt) {
+        xb_[b] = P.x[e_idx(eb_[b])];
+        mb_[b] = P.mol[e_idx(eb_[b])];
+      }
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       if (k0 + b >= cnt) break;
@@ -711,7 +716,7 @@ __global__ __launch_bounds__(256) void k_normals(DevPtrs<R> P) {
         if (nrt < MAXSH) rt_row[nrt] = (uint32_t)j;
         nrt++;
       }
-      if (pp.imapped && P.mol[j] == mi && rsq != 0.0) {
+      if (pp.imapped && mb_[b] == mi && rsq != 0.0) {
         if (rsq < pp.rc2w) {
           if (nrn < MAXSH) rn_row[nrn] = (uint32_t)j;
           nrn++;
