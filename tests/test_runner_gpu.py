@@ -235,7 +235,8 @@ def test_image_flags_and_binary_restart(tmp_path):
     """Atoms drifting across the periodic x and y boundaries: the dumped image flags satisfy
     xu = x + ix * Lx and the unwrapped positions follow the drift (a missed or doubled crossing
     would be off by a box length); write_restart at step 100 + read_restart continues to the same
-    state and image flags as the uninterrupted 200-step run."""
+    state and image flags as the uninterrupted 200-step run, whose restart 100 file of step 100
+    is byte for byte the write_restart file."""
     if not os.path.exists(RUNNER):
         pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
     s = S.layered_stack(15, 9, ["G", "BN"], seed=81)
@@ -245,7 +246,7 @@ def test_image_flags_and_binary_restart(tmp_path):
     S.write_lammps_data(s, data)
     run_deck(tmp_path, "a.in", deck(data, "", "\n".join([
         "dump d all custom 200 a.dump id x y z xu yu zu ix iy iz", "dump_modify d format float %.17g",
-        "thermo 100", "run 200", "write_data a.data"])))
+        "restart 100 ra.*.bin", "thermo 100", "run 200", "write_data a.data"])))
     run_deck(tmp_path, "b.in", deck(data, "", "\n".join(["thermo 100", "run 100", "write_restart r.bin"])))
     run_deck(tmp_path, "c.in", "\n".join([
         "units metal", "atom_style full", "read_restart r.bin",
@@ -254,6 +255,10 @@ def test_image_flags_and_binary_restart(tmp_path):
         f"pair_coeff * * ilp/graphene/hbn {os.path.join(POT, 'BNCH.ILP')} C B N",
         "neighbor 2.0 bin", "neigh_modify every 10 delay 0 check no",
         "fix 1 all nve", "thermo 100", "run 100", "write_data c.data", ""]))
+    # restart N file: written at steps 100 and 200 of the uninterrupted run; the one of step 100
+    # holds exactly the state write_restart saved after the 100-step run
+    assert (tmp_path / "ra.100.bin").read_bytes() == (tmp_path / "r.bin").read_bytes()
+    assert (tmp_path / "ra.200.bin").exists() and not (tmp_path / "ra.0.bin").exists()
     frames = S.read_dump_custom(str(tmp_path / "a.dump"))
     assert [f[0] for f in frames] == [0, 200]
     a = frames[1][2]
