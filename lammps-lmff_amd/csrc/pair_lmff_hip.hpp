@@ -7,8 +7,9 @@
 // init_style(), init_one(i, j), compute(eflag, vflag), with the same outputs (eng_vdwl,
 // virial[6], eatom, vatom, pvector[2]) and error behaviour (an error aborts the command; here
 // it throws lmff::PairError carrying the reference's message).  Inside LAMMPS the same class
-// body sits in a `class PairLMFFHip : public Pair`; outside LAMMPS the
-// caller hands in the atom arrays through AtomView.
+// body sits in a `class PairLMFFHip : public Pair` (compiled against the LAMMPS headers and
+// linked with the library); outside LAMMPS the caller hands in the atom arrays through
+// AtomView.
 #pragma once
 
 #include <stdexcept>
