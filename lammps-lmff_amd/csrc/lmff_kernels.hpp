@@ -529,9 +529,12 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
         in_sb = (rsq <= P.shortsq) && !(rsq < P.short_splitsq);
       }
     }
-    const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long bi = __ballot(in_i), bo = __ballot(in_o), bs = __ballot(in_s), bsb = __ballot(in_sb);
-    const int pi = __popcll(bi & lt), po = __popcll(bo & lt), ps = __popcll(bs & lt), psb = __popcll(bsb & lt);
+    // set bits of each mask below this lane (v_mbcnt)
+    auto below = [](unsigned long long m) {
+      return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    };
+    const int pi = below(bi), po = below(bo), ps = below(bs), psb = below(bsb);
     if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
     if (in_o && no + po < P.icap) irow[P.icap - 1 - (no + po)] = ent;
     // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
