@@ -78,7 +78,9 @@ int lmff_masses(lmff_ctx *ctx, const double *mass_per_type);
  * vflag&4 = per-atom virial.  Outputs ACCUMULATE (+=) into f[n][3], eatom[n], vatom[n][6]
  * as LAMMPS pair styles do; eng_vdwl and virial[6] (xx yy zz xy xz yz) and pvector[2]
  * (ILP vdW, ILP repulsion energies) are overwritten.  new_list != 0 forces a neighbor
- * rebuild (the caller's neighbor->ago == 0); the first call always builds.
+ * rebuild (the caller's neighbor->ago == 0); the first call always builds.  Between builds
+ * the atoms must come in the same order (LAMMPS reorders them only when it reneighbours);
+ * positions need not be wrapped: the library applies the wrap of its last build to them.
  * Any output pointer may be NULL. */
 int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int32_t *type,
                       const int64_t *tag, const int64_t *mol, const double boxlo[3],
