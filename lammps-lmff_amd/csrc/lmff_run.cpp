@@ -2,7 +2,8 @@
 // MI355X, so an unchanged deck of the reference's form
 //   units metal / atom_style full / boundary / variable (index, equal) / read_data / mass /
 //   replicate / pair_style hybrid/overlay tersoff|lmff ilp/graphene/hbn / pair_coeff /
-//   neighbor / neigh_modify / timestep / velocity / fix nve | fix langevin / thermo / run
+//   neighbor / neigh_modify / timestep / velocity create (dist, mom, rot) | scale /
+//   fix nve | fix langevin / thermo / run
 // executes with the force field, the neighbor build and the integrator on the GPU.
 // Output follows LAMMPS' log format (thermo table, "Loop time", "Performance:" as in
 // LMFF/finish.cpp:124,162).  Also: compute pe/atom and stress/atom, dump custom (text,
@@ -715,31 +716,106 @@ int main(int argc, char **argv) {
     } else if (cmd == "lmff_precision") {
       precision = w.at(1) == "mixed" ? LMFF_MIXED : LMFF_FP64;
     } else if (cmd == "velocity") {
-      // velocity all create T seed: Gaussian, zero net momentum, scaled to T (own RNG stream)
-      if (w.size() < 5 || w[2] != "create") die("lmff_run supports velocity all create T seed");
+      // velocity all create T seed [dist uniform|gaussian] [mom yes|no] [rot yes|no] and
+      // velocity all scale T (Velocity::create / scale, LMFF/velocity.cpp:162-380, 579):
+      // draws scaled by 1/sqrt(m), net momentum and angular momentum about the centre of
+      // mass removed as asked (zero_momentum / zero_rotation, 755-830, on unwrapped
+      // positions), then rescaled to T over 3N-3 degrees of freedom.  The stream is this
+      // runner's own (mt19937_64), not LAMMPS' RanPark.
+      if (w.size() < 4 || w[1] != "all" || (w[2] != "create" && w[2] != "scale"))
+        die("lmff_run supports velocity all create T seed [keywords] and velocity all scale T");
       if (!have_data) die("Velocity command before simulation box is defined");
+      if (data.n == 0) die("Cannot zero momentum of no atoms");
+      if (pair) current(data.x, data.v, data.img);  // a resident system: its present state
       const double T = std::stod(w[3]);
-      std::mt19937_64 rng(std::stoull(w[4]));
-      std::normal_distribution<double> g(0.0, 1.0);
-      double p[3] = {0, 0, 0}, mt = 0, ke = 0;
-      for (int64_t i = 0; i < data.n; i++) {
-        const double m = data.mass.at(data.type[i]);
-        for (int c = 0; c < 3; c++) {
-          data.v[3 * i + c] = g(rng) * std::sqrt(BOLTZ * T / (m * MVV2E));
-          p[c] += m * data.v[3 * i + c];
-        }
-        mt += m;
+      bool gaussian = false, mom = true, rot = false;
+      const size_t kw0 = w[2] == "create" ? 5 : 4;
+      if (w[2] == "create" && w.size() < 5) die("Illegal velocity command");
+      for (size_t k = kw0; k < w.size(); k += 2) {
+        if (k + 1 >= w.size()) die("Illegal velocity command");
+        const std::string &key = w[k], &val = w[k + 1];
+        if (key == "dist" && (val == "uniform" || val == "gaussian")) gaussian = val == "gaussian";
+        else if (key == "mom" && (val == "yes" || val == "no")) mom = val == "yes";
+        else if (key == "rot" && (val == "yes" || val == "no")) rot = val == "yes";
+        else if (key == "loop" || key == "sum" || key == "units" || key == "bias" || key == "temp") continue;
+        else die("Illegal velocity command: " + key + " " + val);
       }
-      for (int64_t i = 0; i < data.n; i++) {
-        const double m = data.mass.at(data.type[i]);
-        for (int c = 0; c < 3; c++) {
-          data.v[3 * i + c] -= p[c] / mt;
-          ke += 0.5 * MVV2E * m * data.v[3 * i + c] * data.v[3 * i + c];
+      auto mass_of = [&](int64_t i) { return data.mass.at(data.type[i]); };
+      if (w[2] == "create") {
+        std::mt19937_64 rng(std::stoull(w[4]));
+        std::normal_distribution<double> g(0.0, 1.0);
+        std::uniform_real_distribution<double> u(-0.5, 0.5);
+        for (int64_t i = 0; i < data.n; i++) {
+          const double f = 1.0 / std::sqrt(mass_of(i));
+          for (int c = 0; c < 3; c++) data.v[3 * i + c] = (gaussian ? g(rng) : u(rng)) * f;
+        }
+        double mt = 0, p[3] = {0, 0, 0};
+        for (int64_t i = 0; i < data.n; i++) {
+          mt += mass_of(i);
+          for (int c = 0; c < 3; c++) p[c] += mass_of(i) * data.v[3 * i + c];
+        }
+        if (mom)
+          for (int64_t i = 0; i < data.n; i++)
+            for (int c = 0; c < 3; c++) data.v[3 * i + c] -= p[c] / mt;
+        if (rot) {
+          // omega = I^-1 L about the centre of mass of the unwrapped positions
+          std::vector<double> xu(3 * data.n);
+          for (int64_t i = 0; i < data.n; i++)
+            for (int c = 0; c < 3; c++)
+              xu[3 * i + c] = data.x[3 * i + c] +
+                              (data.img.size() == 3 * (size_t)data.n ? data.img[3 * i + c] : 0) * (data.hi[c] - data.lo[c]);
+          double xcm[3] = {0, 0, 0}, L[3] = {0, 0, 0}, I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          for (int64_t i = 0; i < data.n; i++)
+            for (int c = 0; c < 3; c++) xcm[c] += mass_of(i) * xu[3 * i + c] / mt;
+          for (int64_t i = 0; i < data.n; i++) {
+            const double m = mass_of(i);
+            const double d[3] = {xu[3 * i] - xcm[0], xu[3 * i + 1] - xcm[1], xu[3 * i + 2] - xcm[2]};
+            const double *vv = &data.v[3 * i];
+            L[0] += m * (d[1] * vv[2] - d[2] * vv[1]);
+            L[1] += m * (d[2] * vv[0] - d[0] * vv[2]);
+            L[2] += m * (d[0] * vv[1] - d[1] * vv[0]);
+            I[0][0] += m * (d[1] * d[1] + d[2] * d[2]);
+            I[1][1] += m * (d[0] * d[0] + d[2] * d[2]);
+            I[2][2] += m * (d[0] * d[0] + d[1] * d[1]);
+            I[0][1] -= m * d[0] * d[1];
+            I[1][2] -= m * d[1] * d[2];
+            I[0][2] -= m * d[0] * d[2];
+          }
+          I[1][0] = I[0][1];
+          I[2][1] = I[1][2];
+          I[2][0] = I[0][2];
+          const double det = I[0][0] * (I[1][1] * I[2][2] - I[1][2] * I[2][1]) -
+                             I[0][1] * (I[1][0] * I[2][2] - I[1][2] * I[2][0]) +
+                             I[0][2] * (I[1][0] * I[2][1] - I[1][1] * I[2][0]);
+          if (det > 1e-6) {
+            double inv[3][3];
+            inv[0][0] = (I[1][1] * I[2][2] - I[1][2] * I[2][1]) / det;
+            inv[0][1] = -(I[0][1] * I[2][2] - I[0][2] * I[2][1]) / det;
+            inv[0][2] = (I[0][1] * I[1][2] - I[0][2] * I[1][1]) / det;
+            inv[1][0] = -(I[1][0] * I[2][2] - I[1][2] * I[2][0]) / det;
+            inv[1][1] = (I[0][0] * I[2][2] - I[0][2] * I[2][0]) / det;
+            inv[1][2] = -(I[0][0] * I[1][2] - I[0][2] * I[1][0]) / det;
+            inv[2][0] = (I[1][0] * I[2][1] - I[1][1] * I[2][0]) / det;
+            inv[2][1] = -(I[0][0] * I[2][1] - I[0][1] * I[2][0]) / det;
+            inv[2][2] = (I[0][0] * I[1][1] - I[0][1] * I[1][0]) / det;
+            double om[3];
+            for (int r = 0; r < 3; r++) om[r] = inv[r][0] * L[0] + inv[r][1] * L[1] + inv[r][2] * L[2];
+            for (int64_t i = 0; i < data.n; i++) {
+              const double d[3] = {xu[3 * i] - xcm[0], xu[3 * i + 1] - xcm[1], xu[3 * i + 2] - xcm[2]};
+              data.v[3 * i] -= om[1] * d[2] - om[2] * d[1];
+              data.v[3 * i + 1] -= om[2] * d[0] - om[0] * d[2];
+              data.v[3 * i + 2] -= om[0] * d[1] - om[1] * d[0];
+            }
+          }
         }
       }
-      const double t_now = 2 * ke / ((3.0 * data.n - 3) * BOLTZ);
+      double ke = 0;
+      for (int64_t i = 0; i < data.n; i++)
+        for (int c = 0; c < 3; c++) ke += 0.5 * MVV2E * mass_of(i) * data.v[3 * i + c] * data.v[3 * i + c];
+      const double t_now = data.n > 1 ? 2 * ke / ((3.0 * data.n - 3) * BOLTZ) : 0.0;
       const double sc = t_now > 0 ? std::sqrt(T / t_now) : 0.0;
       for (auto &q : data.v) q *= sc;
+      if (pair && lmff_system_set(pair->context(), sys, nullptr, data.v.data())) die(lmff_last_error(pair->context()));
     } else if (cmd == "fix") {
       if (w.size() >= 4 && w[3] == "nve") {
         have_fix = true;

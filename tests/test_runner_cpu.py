@@ -156,3 +156,45 @@ def test_restart_errors(tmp_path):
     assert r.returncode != 0 and "Unexpected end of restart file" in r.stderr
     r = run(tmp_path, "read_data in.data\nrestart 10\n")
     assert r.returncode != 0 and "Illegal restart command" in r.stderr
+
+
+def _temp_mom_angmom(s, v):
+    m = s.masses()
+    BOLTZ, MVV2E = 8.617343e-5, 1.0364269e-4
+    t = (m[:, None] * v * v).sum() * MVV2E / ((3 * s.n - 3) * BOLTZ)
+    p = (m[:, None] * v).sum(0)
+    xcm = (m[:, None] * s.x).sum(0) / m.sum()
+    L = (m[:, None] * np.cross(s.x - xcm, v)).sum(0)
+    return t, p, L
+
+
+def test_velocity_create_keywords_and_scale(tmp_path):
+    """velocity all create T seed with dist, mom and rot (Velocity::create: draws / sqrt(m),
+    zero_momentum, zero_rotation about the centre of mass, then T over 3N-3 degrees of freedom)
+    and velocity all scale T."""
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=3, temperature=None)
+    S.write_lammps_data(s, str(tmp_path / "in.data"))
+    base = ["units metal", "atom_style full", "boundary p p f", "read_data in.data"]
+
+    def velocities(lines):
+        r = run(tmp_path, "\n".join(base + lines + ["write_data out.data", ""]))
+        assert r.returncode == 0, r.stdout + r.stderr
+        o = S.read_lammps_data(str(tmp_path / "out.data"))
+        return o.v[np.argsort(np.argsort(s.tag))] if o.v is not None else None
+
+    m = np.sqrt(s.masses())[:, None]
+    v = velocities(["velocity all create 300.0 4928459 rot yes dist gaussian"])
+    t, p, L = _temp_mom_angmom(s, v)
+    assert abs(t - 300.0) < 1e-9 and np.abs(p).max() < 1e-10 and np.abs(L).max() < 1e-8
+    w = (v * m).ravel()
+    assert np.abs(w).max() / w.std() > 3.0          # Gaussian draws
+    v = velocities(["velocity all create 300.0 4928459"])
+    t, p, L = _temp_mom_angmom(s, v)
+    assert abs(t - 300.0) < 1e-9 and np.abs(p).max() < 1e-10 and np.abs(L).max() > 1e-3
+    w = (v * m).ravel()
+    assert np.abs(w).max() / w.std() < 1.8          # uniform draws (the default)
+    v = velocities(["velocity all create 300.0 7 mom no", "velocity all scale 450.0"])
+    t, p, L = _temp_mom_angmom(s, v)
+    assert abs(t - 450.0) < 1e-9 and np.abs(p).max() > 1e-6
+    r = run(tmp_path, "\n".join(base + ["velocity all create 300.0 7 rot maybe", ""]))
+    assert r.returncode != 0 and "Illegal velocity command" in r.stderr

@@ -275,3 +275,23 @@ def test_image_flags_and_binary_restart(tmp_path):
     assert open(tmp_path / "c.data").readline().strip().endswith("timestep = 200")
     np.testing.assert_array_equal(wc.meta["image"], wa.meta["image"])
     assert np.abs(wc.x - wa.x).max() < 1e-8 and np.abs(wc.v - wa.v).max() < 1e-8
+
+
+def test_velocity_scale_of_a_resident_system(tmp_path):
+    """velocity all scale between runs acts on the device-resident state: the next run's first
+    thermo line reports the new temperature, and the rotation-free create keeps the run stable."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("lmff_run is not built (python lammps-lmff_amd/build.py)")
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=91, temperature=None)
+    data = str(tmp_path / "start.data")
+    S.write_lammps_data(s, data)
+    out = run_deck(tmp_path, "v.in", deck(data, "", "\n".join([
+        "velocity all create 300.0 4928459 rot yes dist gaussian", "thermo 10", "run 20",
+        "velocity all scale 600.0", "run 10"])))
+    rows = [l.split() for l in out.splitlines() if l.strip() and l.split()[0].isdigit() and len(l.split()) == 5]
+    temps = {int(r[0]): float(r[1]) for r in rows}
+    first = [float(r[1]) for r in rows if int(r[0]) == 0][0]
+    assert abs(first - 300.0) < 1e-3
+    # the second run starts at step 20 with the rescaled velocities
+    starts = [float(r[1]) for r in rows if int(r[0]) == 20]
+    assert len(starts) == 2 and abs(starts[1] - 600.0) < 1e-3 and abs(starts[0] - 600.0) > 1.0
