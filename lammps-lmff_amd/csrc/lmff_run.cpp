@@ -1039,7 +1039,8 @@ int main(int argc, char **argv) {
       std::printf("  Force two-norm initial, final = %g %g\n", ms.fnorm2_initial, ms.fnorm2_final);
       std::printf("  Force max component initial, final = %g %g\n", ms.fnorminf_initial, ms.fnorminf_final);
       std::printf("  Iterations, force evaluations = %lld %lld\n\n", (long long)ms.iterations, (long long)ms.evaluations);
-    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "unfix") {
+    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "unfix" || cmd == "fix_modify" ||
+               cmd == "thermo_modify") {
       std::printf("WARNING: %s ignored by lmff_run\n", cmd.c_str());
     } else {
       die("Unknown command: " + line);
