@@ -6,7 +6,9 @@ rebuild every 10 steps with a 2 A skin.  A "step" is one Velocity-Verlet step of
 system (force evaluation + integration + the amortised neighbor rebuild), with all state
 resident in HBM.  The FP64 path gives `value`; the mixed-precision path (FP32 ILP pair math,
 FP64 displacements, Tersoff and accumulation) is timed on the same input and reported beside
-it, with its force error against FP64.
+it, with its force error against FP64.  The same line carries `stack_check`: the
+mixed-vs-FP64 step time, force error and drift ratio of the 2,704,000-atom 10-layer stack over
+50 steps per precision (--no-stack skips it).
 
 Other workloads (not the default line):
   --workload multilayer   10-layer alternating G/hBN stack, 260x260 cells per layer
@@ -232,12 +234,40 @@ def md_compare(s, steps, warmup):
 FAMILIES = ["list_build", "normals", "ilp_pair", "tersoff_bonds", "gather", "integrate", "reduce"]
 
 
+def stack_check(steps=50, warmup=5):
+    """The mixed-vs-FP64 check on the 2,704,000-atom 10-layer alternating G/hBN stack (260x260
+    cells per layer): graph-replayed NVE runs of both precisions (step time, energy drift) and
+    the mixed force error against FP64 on the starting positions."""
+    kinds = ["G" if k % 2 == 0 else "BN" for k in range(10)]
+    s = S.layered_stack(260, 260, kinds, seed=12347)
+    n = s.n
+    log(f"10-layer stack, {n} atoms: single-shot forces and {steps}-step runs, FP64 and mixed")
+    r64 = one_shot(s, capi.LMFF_FP64)
+    rmx = one_shot(s, capi.LMFF_MIXED)
+    df = rmx["f"] - r64["f"]
+    res = {}
+    for name, prec in (("fp64", capi.LMFF_FP64), ("mixed", capi.LMFF_MIXED)):
+        ctx, sid = build_ctx(s, prec)
+        res[name] = timed_run(ctx, sid, s, steps, warmup, profile=False)
+        ctx.close()
+    ms = {k: v["gpu_ms"] / steps for k, v in res.items()}
+    drift = {k: (v["e1"] - v["e0"]) / n for k, v in res.items()}
+    return {"workload": f"10-layer alternating G/hBN stack 260x260 cells/layer, {n} atoms, NVE dt=1 fs, "
+                        f"rebuild every 10 steps, skin 2 A, ILP 16 A with Tap; {steps} steps per precision",
+            "natoms": n, "steps": steps, "fp64_ms_per_step": ms["fp64"], "mixed_ms_per_step": ms["mixed"],
+            "speedup_mixed_vs_fp64": ms["fp64"] / ms["mixed"],
+            "force_err_vs_fp64": {"max_abs": float(np.abs(df).max()), "rms": float(np.sqrt((df ** 2).sum(1).mean()))},
+            "drift_eV_per_atom": {"fp64": drift["fp64"], "mixed": drift["mixed"],
+                                  "mixed_over_fp64": abs(drift["mixed"]) / abs(drift["fp64"]) if drift["fp64"] else None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stack", action="store_true", help="bilayer: skip the 10-layer mixed-vs-FP64 check")
     ap.add_argument("--workload", choices=["bilayer", "multilayer", "serving"], default="bilayer")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
@@ -337,6 +367,7 @@ def main():
             "atom_evals_per_s": n / r64["wall_s"],
             "atom_steps_per_s_every10": n / ((r64["wall_s"] + 9 * r64["wall_reuse_s"]) / 10)},
         "wall_s": {"fp64": res64["wall_s"], "mixed": resmx["wall_s"]},
+        "stack_check": None if args.no_stack else stack_check(),
     }
     print(json.dumps(out))
 
