@@ -477,3 +477,28 @@ def test_pair_compute_fresh_lists_with_reordered_atoms():
     d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
     assert np.abs(d).max() < 1e-8
     np.testing.assert_allclose(pe, th[1:, 1], rtol=1e-10)
+
+
+def test_image_flags_set_get_and_range():
+    """lmff_system_set_image / get_image: a round trip in the caller's atom order, kept through
+    a short run in which no atom crosses the box, and -512..511 enforced."""
+    s = S.layered_stack(15, 9, ["G", "BN"], seed=83, temperature=10.0)
+    rng = np.random.default_rng(2)
+    img = rng.integers(-40, 40, size=(s.n, 3)).astype(np.int32)
+    img[:, 2] = 0
+    with ctx_for(s) as ctx:
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+        ctx.set_image(sid, img)
+        np.testing.assert_array_equal(ctx.get_image(sid), img)
+        x0, _, _ = ctx.get(sid)
+        ctx.run(20, 20)
+        x1, _, _ = ctx.get(sid)
+        lo, hi = s.boxlo[:2] + 0.2, s.boxhi[:2] - 0.2
+        inside = np.all((x0[:, :2] > lo) & (x0[:, :2] < hi), axis=1)
+        assert inside.sum() > s.n // 2
+        np.testing.assert_array_equal(ctx.get_image(sid)[inside], img[inside])
+        bad = img.copy()
+        bad[0, 0] = 512
+        with pytest.raises(capi.LmffError):
+            ctx.set_image(sid, bad)
