@@ -17,6 +17,16 @@ Written from the published functional forms, not from the oracle's code:
             two, z for fewer (the sign is irrelevant: n enters squared).
   Tap(x) = 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1, x = r/rc.
 Parameters are read by a minimal parser of the same LAMMPS potential files.
+
+Two conventions are taken from the reference rather than from the published forms, because
+they decide what a tally or an asymmetric parameter table gives:
+  * which atom of an unordered pair computes it (and so whose parameter row is used) when a
+    full list is halved: the tag-parity rule of pair_tersoff.cpp:150-158 and
+    pair_ilp_graphene_hbn.cpp:488-496 (owner_is_first below);
+  * per-atom energies: Pair::ev_tally / ev_tally_xyz give half of every tallied energy to each
+    of the two atoms (LMFF/pair.cpp:939-941, 1087-1089).  Tersoff tallies the repulsive pair
+    energy (pair_tersoff.cpp:164-174) and 1/2 b_ij f_A of each ordered pair (214-224), ILP the
+    van der Waals half pair (ilp.cpp:536) and the repulsion of each ordered pair (671).
 """
 from __future__ import annotations
 
@@ -80,7 +90,34 @@ def fc(r, R, D):
     return np.where(r < R - D, 1.0, np.where(r > R + D, 0.0, 0.5 * (1.0 - np.sin(0.5 * np.pi * (r - R) / D))))
 
 
-def tersoff_energy(s, tfile, elem_of_type):
+def owner_is_first(ti, tj):
+    """True where the pair (i, j) of a full list is computed by i: tags of opposite parity are
+    kept by the larger tag, tags of equal parity by the smaller one (pair_tersoff.cpp:150-158,
+    pair_ilp_graphene_hbn.cpp:488-496).  Equal tags (an atom and its own image) never occur
+    here: every periodic side exceeds twice the cutoff."""
+    ti = np.asarray(ti, dtype=np.int64)
+    tj = np.asarray(tj, dtype=np.int64)
+    assert not np.any(ti == tj)
+    odd = (ti + tj) % 2 == 1
+    return np.where(ti > tj, odd, ~odd)
+
+
+def _share(eatom, i, j, e, split):
+    """Per-atom tally of energy e of the pair (i, j): split "half" as Pair::ev_tally, "first"
+    (everything to i: a counter-example for the tests)."""
+    if split == "half":
+        eatom[i] += 0.5 * e
+        eatom[j] += 0.5 * e
+    elif split == "first":
+        eatom[i] += e
+    else:
+        raise ValueError(split)
+
+
+def tersoff_energy(s, tfile, elem_of_type, peratom=False, split="half", parity=True):
+    """Total Tersoff energy; with peratom, (E, eatom).  parity=False averages the repulsive
+    energy of a pair over both parameter rows (i,j,j) and (j,i,i) instead of taking the owner's
+    (a counter-example for asymmetric tables)."""
     elems = [e for e in elem_of_type]
     P = tersoff_params(tfile, set(elems))
     el = [elems[t - 1] for t in s.type]
@@ -89,9 +126,25 @@ def tersoff_energy(s, tfile, elem_of_type):
     n = s.n
     cutmax = max(p["R"] + p["D"] for p in P.values())
     E = 0.0
+    eatom = np.zeros(n)
     nb = [np.nonzero((r[i] < cutmax) & (np.arange(n) != i))[0] for i in range(n)]
     for i in range(n):
         for j in nb[i]:
+            # repulsive: once per unordered pair, with the parameters of the atom that computes it
+            if j > i:
+                rij = r[i, j]
+                if parity:
+                    o, q = (i, j) if owner_is_first(s.tag[i], s.tag[j]) else (j, i)
+                    rows = [P[(el[o], el[q], el[q])]]
+                else:
+                    rows = [P[(el[i], el[j], el[j])], P[(el[j], el[i], el[i])]]
+                er = 0.0
+                for pr in rows:
+                    if rij < pr["R"] + pr["D"]:
+                        er += fc(rij, pr["R"], pr["D"]) * pr["A"] * np.exp(-pr["lam1"] * rij) / len(rows)
+                if er != 0.0:
+                    E += er
+                    _share(eatom, i, j, er, split)
             pij = P[(el[i], el[j], el[j])]
             rij = r[i, j]
             if rij >= pij["R"] + pij["D"]:
@@ -109,8 +162,10 @@ def tersoff_energy(s, tfile, elem_of_type):
                 zeta += fc(rik, p["R"], p["D"]) * g * np.exp((p["lam3"] * (rij - rik)) ** int(p["m"]))
             b = (1.0 + (pij["beta"] * zeta) ** pij["n"]) ** (-0.5 / pij["n"])
             f = fc(rij, pij["R"], pij["D"])
-            E += 0.5 * f * (pij["A"] * np.exp(-pij["lam1"] * rij) - b * pij["B"] * np.exp(-pij["lam2"] * rij))
-    return float(E)
+            ea = -0.5 * f * b * pij["B"] * np.exp(-pij["lam2"] * rij)
+            E += ea
+            _share(eatom, i, j, ea, split)
+    return (float(E), eatom) if peratom else float(E)
 
 
 def normals(s, ifile, elem_of_type):
@@ -138,8 +193,11 @@ def normals(s, ifile, elem_of_type):
     return nv, cnt
 
 
-def ilp_energy(s, ifile, elem_of_type, rc=16.0, tap_flag=1):
-    """returns (E_vdw, E_rep)"""
+def ilp_energy(s, ifile, elem_of_type, rc=16.0, tap_flag=1, peratom=False, split="half", parity=True):
+    """returns (E_vdw, E_rep), with peratom (E_vdw, E_rep, eatom).  The van der Waals energy of
+    an unordered pair uses the parameter row of the atom that computes it (parity=False: the
+    average of both rows, a counter-example for asymmetric tables); the repulsion of the
+    ordered pair (i, j) uses row (i, j) and i's normal."""
     P = ilp_params(ifile, set(elem_of_type))
     el = [elem_of_type[t - 1] for t in s.type]
     nv, _ = normals(s, ifile, elem_of_type)
@@ -157,4 +215,22 @@ def ilp_energy(s, ifile, elem_of_type, rc=16.0, tap_flag=1):
     rho2 = rr * rr - pn * pn
     erep = T * np.exp(-get("lam") * (rr - get("z0"))) * (0.5 * get("eps") + get("C") * np.exp(-rho2 / get("delta") ** 2))
     evdw = -T * get("C6") / (rr**6 * (1.0 + np.exp(-get("d") * (rr / get("seff") - 1.0))))
-    return float(0.5 * evdw.sum()), float(erep.sum())
+    # every ordered pair evaluated with its own row (i, j); an unordered pair's vdW energy is
+    # the owner's row, i.e. the ordered pair (i, j) whose i owns it
+    wv = owner_is_first(s.tag[ii], s.tag[jj]).astype(np.float64) if parity else np.full(len(ii), 0.5)
+    ev = float((wv * evdw).sum())
+    er = float(erep.sum())
+    if not peratom:
+        return ev, er
+    eatom = np.zeros(s.n)
+    if split == "half":
+        # each ordered pair carries w*evdw (its share of the unordered vdW pair) + erep; both
+        # halves of each go to i and j
+        e = wv * evdw + erep
+        np.add.at(eatom, ii, 0.5 * e)
+        np.add.at(eatom, jj, 0.5 * e)
+    elif split == "first":
+        np.add.at(eatom, ii, wv * evdw + erep)
+    else:
+        raise ValueError(split)
+    return ev, er, eatom

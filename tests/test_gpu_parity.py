@@ -123,32 +123,9 @@ def test_fp64_asymmetric_pair_parameters(tmp_path):
     half selection and exp(-lambda (r - z0)) is taken separately for (i,j) and (j,i).  Likewise
     the Tersoff B N N entry's repulsion (A, lambda1) differs from N B B's, so the bonds kernel
     picks each B-N pair's repulsive parameters by tag parity as the reference does."""
-    rows = open(ILP).read().splitlines()
-    out = []
-    for line in rows:
-        w = line.split()
-        if len(w) == 13 and (w[0], w[1]) in (("B", "C"), ("N", "C")):
-            v = [float(t) for t in w[2:]]
-            v[0] *= 1.03   # beta (z0)
-            v[1] *= 0.97   # alpha
-            v[8] *= 1.1    # C6
-            v[5] *= 1.02   # d
-            v[6] *= 0.98   # sR
-            line = f"{w[0]} {w[1]} " + " ".join(repr(t) for t in v)
-        out.append(line)
-    asym = str(tmp_path / "asym.ILP")
-    open(asym, "w").write("\n".join(out) + "\n")
-    tout = []
-    for line in open(BNC).read().splitlines():
-        w = line.split()
-        if len(w) == 17 and w[:3] == ["B", "N", "N"]:
-            v = [float(t) for t in w[3:]]
-            v[12] *= 1.01   # lambda1
-            v[13] *= 0.98   # A
-            line = "B N N " + " ".join(repr(t) for t in v)
-        tout.append(line)
-    tasym = str(tmp_path / "asym.tersoff")
-    open(tasym, "w").write("\n".join(tout) + "\n")
+    import asym_tables
+
+    tasym, asym = asym_tables.write(tmp_path)
     import oracle
 
     s = S.layered_stack(30, 20, ["G", "BN"], seed=17)

@@ -160,6 +160,61 @@ def test_energy_matches_independent_numpy(oracle_lib, layers, seed):
     np.testing.assert_allclose(np.abs((nrm * nref).sum(1)), 1.0, atol=1e-12)
 
 
+@pytest.mark.parametrize("layers,seed", [(("G", "G"), 3), (("G", "BN"), 4)])
+def test_peratom_energy_halves_match_independent_numpy(oracle_lib, layers, seed):
+    """Every atom's energy against tests/refimpl.py, which gives half of each tallied pair
+    energy to each atom as Pair::ev_tally does (LMFF/pair.cpp:939-941): the Tersoff repulsive
+    pair, 1/2 b_ij f_A of each ordered pair, the ILP van der Waals half pair and the repulsion of
+    each ordered pair.  A split that gives each term to its first atom keeps every total and
+    must fail."""
+    s = small(layers, seed=seed, nx=15, ny=9)
+    o = make_oracle(s)
+    r = o.compute(3, 0)
+    el = s.elements
+    tfile, telem = (CT, ["C(O)"]) if s.elements == ["C"] else (BNC, s.elements)
+    et, eat = refimpl.tersoff_energy(s, tfile, telem, peratom=True)
+    ev, er, eai = refimpl.ilp_energy(s, ILP, el, peratom=True)
+    ref = eat + eai
+    scale = np.abs(ref).max()
+    assert np.abs(r["eatom"] - ref).max() <= 1e-11 * scale
+    assert r["eatom"].sum() == pytest.approx(r["energy"], rel=1e-12)
+    _, eat1 = refimpl.tersoff_energy(s, tfile, telem, peratom=True, split="first")
+    _, _, eai1 = refimpl.ilp_energy(s, ILP, el, peratom=True, split="first")
+    wrong = eat1 + eai1
+    assert wrong.sum() == pytest.approx(ref.sum(), rel=1e-12)
+    assert np.abs(r["eatom"] - wrong).max() > 1e-4 * scale
+
+
+def test_asymmetric_tables_follow_tag_parity(oracle_lib, tmp_path):
+    """With parameter rows that differ between (i, j) and (j, i) (tests/asym_tables.py), the
+    reference computes each unordered pair's van der Waals and Tersoff repulsive energy with the
+    row of the atom that the tag-parity rule selects (pair_tersoff.cpp:150-158,
+    pair_ilp_graphene_hbn.cpp:488-496).  The oracle against refimpl's restatement of that rule,
+    per term and per atom; averaging both rows instead must fail."""
+    import asym_tables
+    import oracle
+
+    tasym, asym = asym_tables.write(tmp_path)
+    s = small(("G", "BN"), seed=6, nx=15, ny=9)
+    o = oracle.Oracle(3, tasym, s.elements, asym, s.elements, tap_flag=1, threads=os.cpu_count() or 1)
+    o.set_system(s)
+    o.build_list()
+    r = o.compute(3, 0)
+    et, eat = refimpl.tersoff_energy(s, tasym, s.elements, peratom=True)
+    ev, er, eai = refimpl.ilp_energy(s, asym, s.elements, peratom=True)
+    assert r["e_tersoff"] == pytest.approx(et, rel=1e-12)
+    assert r["e_vdw"] == pytest.approx(ev, rel=1e-12)
+    assert r["e_rep"] == pytest.approx(er, rel=1e-12)
+    ref = eat + eai
+    assert np.abs(r["eatom"] - ref).max() <= 1e-11 * np.abs(ref).max()
+    # averaging the two rows is a different answer for both terms
+    et_avg = refimpl.tersoff_energy(s, tasym, s.elements, parity=False)
+    ev_avg, _ = refimpl.ilp_energy(s, asym, s.elements, parity=False)
+    assert abs(et_avg - et) > 1e3 * 1e-12 * abs(et)
+    assert abs(ev_avg - ev) > 1e3 * 1e-12 * abs(ev)
+    assert abs(r["e_tersoff"] - et_avg) > 1e-9 and abs(r["e_vdw"] - ev_avg) > 1e-9
+
+
 def _fd_force(o, x0, atom, comp, h=1e-3):
     e = []
     for k in (-2, -1, 1, 2):
