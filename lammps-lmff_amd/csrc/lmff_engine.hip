@@ -308,14 +308,28 @@ struct Ctx {
     HIPCHK(hipMemcpy(t3.p, t3v.data(), sizeof(int) * t3v.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(rmass.p, m.data(), sizeof(double) * MAXT, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(errw.p, 0, sizeof(int) * 4));
-    double rc_max = 0.0;
-    for (double c : ilp.cutILPsq) rc_max = std::max(rc_max, std::sqrt(c));
-    const double cutforce = std::max(ilp_cut, ters.cutmax);
-    cutneigh = cutforce + skin;
-    short_reach = std::max(ters.cutmax, rc_max);
+    cutoffs(cutneigh, short_reach);
     short_cut = short_reach + skin;
     params_dirty = false;
-    gen++;
+    gen++;  // captured graphs hold the old tables' launch arguments
+  }
+
+  // neighbor cutoff (pair cutoff + skin) and the short-list reach the current settings give
+  void cutoffs(double &cutn, double &reach) const {
+    double rc_max = 0.0;
+    for (double c : ilp.cutILPsq) rc_max = std::max(rc_max, std::sqrt(c));
+    cutn = std::max(ilp_cut, ters.cutmax) + skin;
+    reach = std::max(ters.cutmax, rc_max);
+  }
+
+  // Called by every setter: the device tables are refreshed as soon as the settings are complete
+  // (and after every later change), so evaluations never see stale masses or coefficients.  The
+  // setters refuse cutoff changes while systems are resident (they were binned for the old ones).
+  void params_changed() {
+    params_dirty = true;
+    if (!(have_settings && have_tersoff && have_ilp && (int)mass.size() == ntypes)) return;
+    upload_params();
+    forces_valid = false;
   }
 
   // ---------------------------------------------------------------- layout
@@ -561,12 +575,20 @@ struct Ctx {
     P.eb = eb.as<double>(); P.fb = fb.as<double>(); P.wb = wb.as<double>(); P.ft = ft.as<double4>();
     P.fil = fil.as<double4>(); P.S = S.as<double4>(); P.T = T.as<double>();
     P.th = th.as<double>(); P.vatom = vatom.as<double>();
+    P.err = errw.as<int>();
+    P.ovf = ovf.as<int>();
+    P.farflag = farflag.as<int>();
+    scalars(P);
+    return P;
+  }
+
+  // parameter tables, cutoffs and Tap coefficients (everything of DevPtrs but the per-atom arrays)
+  template <typename R>
+  void scalars(DevPtrs<R> &P) {
     P.ip = (const IlpPair<R> *)(sizeof(R) == 8 ? ip_d.p : ip_f.p);
     P.tp = (const TersP<double> *)tp_d.p;
     P.ntp = ntp;
     P.t3 = t3.as<int>();
-    P.err = errw.as<int>();
-    P.ovf = ovf.as<int>();
     P.cutneighsq = cutneigh * cutneigh;
     P.shortsq = short_cut * short_cut;
     P.tersoff_cutmaxsq = ters.cutmax * ters.cutmax;
@@ -575,7 +597,6 @@ struct Ctx {
     // the per-step tests reach to cutmax and rcut widened by 1e-12 in r^2 (reverse candidates)
     const double sr = short_reach * (1.0 + 1e-9) + ilp_margin();
     P.short_splitsq = sr * sr;
-    P.farflag = farflag.as<int>();
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
@@ -589,7 +610,6 @@ struct Ctx {
       P.tap_poly[3] = 20.0 / (rc4 * rc * rc * rc);
       P.tap_poly[4] = 140.0 / (rc4 * rc * rc * rc);
     }
-    return P;
   }
 
   // ILP rows keep pairs with r < rc + margin at the build in front (short lists: r < max(Tersoff
@@ -726,7 +746,13 @@ struct Ctx {
 
   template <typename R>
   void force_impl(int ef, int vf, int va, int integ, int want_ke) {
-    auto P = ptrs<R>();
+    force_kernels<R>(ptrs<R>(), ef, vf, va, integ, want_ke);
+  }
+
+  // normals -> ILP pair -> Tersoff bonds -> gather over the arrays P points at
+  template <typename R>
+  void force_kernels(const DevPtrs<R> &P, int ef, int vf, int va, int integ, int want_ke) {
+    const int n = P.n;
     const int nb1 = (int)((n + BS - 1) / BS);
     const int nbw = (int)((n + 3) / 4);
     size_t mk = 0;
@@ -1409,11 +1435,13 @@ int lmff_settings(lmff_ctx *ctx, int ntypes, double ilp_cut_global, int tap_flag
     if (!(ilp_cut_global > 0)) fail(LMFF_ERR_ARG, "Illegal pair_style command");
     auto &c = ctx->c;
     if (!c.sys.empty() && ntypes != c.ntypes) fail(LMFF_ERR_STATE, "cannot change ntypes with resident systems");
+    if (!c.sys.empty() && ilp_cut_global != c.ilp_cut)
+      fail(LMFF_ERR_STATE, "cannot change the ILP cutoff with resident systems");
     c.ntypes = ntypes;
     c.ilp_cut = ilp_cut_global;
     c.tap_flag = tap_flag ? 1 : 0;
     c.have_settings = true;
-    c.params_dirty = true;
+    c.params_changed();
   });
 }
 
@@ -1427,9 +1455,11 @@ int lmff_coeff_tersoff(lmff_ctx *ctx, const char *file, const char *const *eleme
     if (!map_elements(c.ntypes, elements, t.emap, err)) fail(LMFF_ERR_ARG, "%s", err.c_str());
     int rc = read_tersoff(file, t, err);
     if (rc) fail(rc, "%s", err.c_str());
+    if (!c.sys.empty() && c.have_tersoff && t.cutmax != c.ters.cutmax)
+      fail(LMFF_ERR_STATE, "cannot change the Tersoff cutoff with resident systems");
     c.ters = std::move(t);
     c.have_tersoff = true;
-    c.params_dirty = true;
+    c.params_changed();
   });
 }
 
@@ -1443,9 +1473,15 @@ int lmff_coeff_ilp(lmff_ctx *ctx, const char *file, const char *const *elements)
     if (!map_elements(c.ntypes, elements, t.emap, err)) fail(LMFF_ERR_ARG, "%s", err.c_str());
     int rc = read_ilp(file, t, err);
     if (rc) fail(rc, "%s", err.c_str());
+    if (!c.sys.empty() && c.have_ilp) {
+      double r0 = 0.0, r1 = 0.0;
+      for (double q : c.ilp.cutILPsq) r0 = std::max(r0, q);
+      for (double q : t.cutILPsq) r1 = std::max(r1, q);
+      if (r0 != r1) fail(LMFF_ERR_STATE, "cannot change the ILP normal cutoff with resident systems");
+    }
     c.ilp = std::move(t);
     c.have_ilp = true;
-    c.params_dirty = true;
+    c.params_changed();
   });
 }
 
@@ -1458,7 +1494,7 @@ int lmff_neighbor(lmff_ctx *ctx, double skin, int every, int delay, int check) {
     c.every = every;
     c.delay = delay;
     c.check = check ? 1 : 0;
-    c.params_dirty = true;
+    c.params_changed();
   });
 }
 
@@ -1474,10 +1510,10 @@ int lmff_masses(lmff_ctx *ctx, const double *m) {
   return guard(ctx, [&] {
     auto &c = ctx->c;
     if (!c.have_settings || !m) fail(LMFF_ERR_ARG, "masses need lmff_settings first");
+    for (int t = 0; t < c.ntypes; t++)
+      if (!(m[t] > 0)) fail(LMFF_ERR_ARG, "Invalid mass value");
     c.mass.assign(m, m + c.ntypes);
-    for (double q : c.mass)
-      if (!(q > 0)) fail(LMFF_ERR_ARG, "Invalid mass value");
-    c.params_dirty = true;
+    c.params_changed();
   });
 }
 
