@@ -88,6 +88,25 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
                       int vflag, double *f, double *eng_vdwl, double virial[6], double *eatom,
                       double *vatom, double pvector[2]);
 
+/* ---- Allocation-free Pair::compute on device arrays ------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same force
+ * evaluation as lmff_pair_compute, on DEVICE arrays: x[n][3], type[n] (1-based), tag[n], mol[n]
+ * in, f[n][3] accumulated (+=) in the caller's order, globals[9] overwritten with {eng_vdwl,
+ * virial[6] (xx yy zz xy xz yz), pvector[2]} (either output may be NULL).  Every call builds
+ * its neighbor lists afresh; per-atom energy/virial (eflag bit1, vflag&4) are refused.  All
+ * working arrays are carved out of one fixed device array of lmff_noalloc_scratch_bytes()
+ * (256 MiB) per GPU, held until the process ends: about 50,000 atoms of a bilayer, fewer for
+ * thicker stacks (their ILP rows are longer); larger inputs fail with LMFF_ERR_MEMORY.  The call
+ * makes no hipMalloc/hipFree or allocating library call; it synchronises the context's stream.
+ * Calls on one GPU share that scratch: calls that overlap in time on one GPU (two contexts, two
+ * host threads) race.  pair_style, pair_coeff and masses must be complete (LMFF_ERR_STATE). */
+int lmff_pair_compute_noalloc(lmff_ctx *ctx, int64_t nlocal, const double *x, const int32_t *type,
+                              const int64_t *tag, const int64_t *mol, const double boxlo[3],
+                              const double boxhi[3], const int periodic[3], int eflag, int vflag,
+                              double *f, double *globals);
+int64_t lmff_noalloc_scratch_bytes(void);
+
 /* ---- Device-resident systems (standalone runs and the batched job server) ------------
  * A context holds any number of independent systems packed into one atom array; every
  * kernel launch covers all of them.  x, v are [n][3]; v may be NULL (zero). */

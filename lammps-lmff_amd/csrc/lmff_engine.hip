@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,10 +27,23 @@
 
 using namespace lmff;
 
+// Scratch of the allocation-free entry point (lmff_pair_compute_noalloc): every per-atom array
+// of one force evaluation lives here instead of in library allocations.  Each GPU the library
+// runs on holds its own copy until the process ends; calls on one GPU share it (and race if they
+// overlap).  Host code never passes the array's name as an address: the device address of each
+// GPU's copy comes from hipGetSymbolAddress at context creation (scratch_address).
+constexpr size_t LMFF_SCRATCH_BYTES = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[LMFF_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int CHUNK = 4096;
 constexpr int BS = 256;
+constexpr int MAX_DEVICES = 64;
+
+// device address of `scratch` per GPU, looked up once per device with that device current
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
 
 struct HipError {
   hipError_t e;
@@ -117,6 +131,9 @@ struct SysHost {
   std::vector<int> chunk_ids;
 };
 
+void check_box(const double lo[3], const double hi[3], const int per[3], double cutneigh);
+void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh);
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -197,6 +214,10 @@ struct Ctx {
   // pair-plugin bookkeeping
   int pair_sys = -1;
   int64_t pair_n = -1;
+  // allocation-free entry point: this device's copy of `scratch` and a small page-locked word
+  // block for the status it reads back (both set up by lmff_create)
+  unsigned char *scratch_dev = nullptr;
+  int *note_host = nullptr;
 
   Ctx() {
     DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &img, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
@@ -208,6 +229,7 @@ struct Ctx {
     for (DBuf *b : all) b->tally = &bytes_in_use;
   }
   ~Ctx() {
+    if (note_host) (void)hipHostFree(note_host);
     if (graph) (void)hipGraphExecDestroy(graph);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1318,6 +1340,213 @@ struct Ctx {
     }
   }
 
+  // ---------------------------------------------------------------- allocation-free Pair::compute
+  // One Pair::compute(eflag, vflag) on device arrays with every per-atom array carved out of this
+  // GPU's `scratch`: no hipMalloc/hipFree, no allocating library call.  Each call wraps, sorts and
+  // lists the atoms afresh (nothing is kept between calls; what a kernel needs zeroed is zeroed
+  // here).  The ILP rows get whatever scratch is left after the other arrays.
+  struct Carve {
+    unsigned char *base;
+    size_t cap, used;
+    template <typename T>
+    T *take(size_t count, const char *what) {
+      const size_t off = (used + 255) & ~(size_t)255;
+      const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+      if (off + bytes > cap)
+        fail(LMFF_ERR_MEMORY, "the %zu MiB scratch of the allocation-free entry point is too small (%s)",
+             cap >> 20, what);
+      used = off + bytes;
+      return (T *)(base + off);
+    }
+    size_t left() const { return cap - std::min(cap, (used + 255) & ~(size_t)255); }
+  };
+
+  void raise_device_errors(int e) {
+    if (e & EB_NONFINITE) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords - simulation unstable");
+    if (e & EB_NORMAL_COUNT)
+      fail(LMFF_ERR_NORMAL, "There are too many neighbors for some atoms, please check your configuration");
+    if (e & EB_NORMAL_ZERO) fail(LMFF_ERR_NORMAL, "The magnitude of the normal vector is zero");
+    if (e & EB_SHORT) fail(LMFF_ERR_SHORT, "more than %d Tersoff neighbours within cutmax", MAXSH);
+    if (e & EB_GRID) fail(LMFF_ERR_STATE, "bin grid too coarse for the neighbor stencil");
+  }
+
+  void pair_compute_noalloc(int64_t nlocal, const double *x_d, const int32_t *type_d, const int64_t *tag_d,
+                            const int64_t *mol_d, const double lo[3], const double hi[3], const int per[3], int eflag,
+                            int vflag, double *f_d, double *glob_d) {
+    if (!scratch_dev) fail(LMFF_ERR_STATE, "no scratch on this device");
+    if (params_dirty) fail(LMFF_ERR_STATE, "pair_style, pair_coeff and masses must be complete first");
+    if ((eflag & 2) || (vflag & 4))
+      fail(LMFF_ERR_ARG, "per-atom energy or virial: use lmff_pair_compute (this entry point keeps no per-atom tallies)");
+    if (nlocal < 0 || nlocal >= (int64_t)IDX_MASK) fail(LMFF_ERR_ARG, "bad atom count %lld", (long long)nlocal);
+    if (nlocal > 0 && (!x_d || !type_d || !tag_d || !mol_d)) fail(LMFF_ERR_ARG, "bad atom arrays");
+    check_box(lo, hi, per, cutneigh);
+    const int m = (int)nlocal;
+    // bins from the box alone (non-periodic sides: the box bounds; atoms beyond them fall into
+    // the edge bins, which reach to infinity)
+    SysHost g{};
+    for (int d = 0; d < 3; d++) {
+      g.lo[d] = lo[d];
+      g.hi[d] = hi[d];
+      g.per[d] = per[d] ? 1 : 0;
+    }
+    setup_grid(g, nullptr, 0, cutneigh);
+    SysDev sd{};
+    for (int d = 0; d < 3; d++) {
+      sd.lo[d] = g.lo[d];
+      sd.prd[d] = g.hi[d] - g.lo[d];
+      sd.per[d] = g.per[d];
+      sd.nc[d] = g.nc[d];
+      sd.glo[d] = g.glo[d];
+      sd.inv_bs[d] = 1.0 / g.bs[d];
+    }
+    sd.begin = 0;
+    sd.end = m;
+    const int nch = std::max(1, (m + CHUNK - 1) / CHUNK);
+    int ebit = 1;
+    while ((1ll << ebit) <= (long long)g.ncells) ebit++;
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr, (int *)nullptr,
+                                              (int *)nullptr, std::max(m, 1), 0, ebit, stream));
+    const size_t M = (size_t)std::max(m, 1);
+    const int scap_nf = std::max(scap, 64);
+    Carve cv{scratch_dev, LMFF_SCRATCH_BYTES, 0};
+    SysDev *systab_d = cv.take<SysDev>(1, "system table");
+    Chunk *chunks_d = cv.take<Chunk>(nch, "chunks");
+    int *cbeg_d = cv.take<int>(2, "chunks");
+    double2 *partial_d = cv.take<double2>((size_t)nch * 9, "partial sums");
+    double *sums_d = cv.take<double>(16, "sums");
+    int *word_d = cv.take<int>(16, "status words");  // [0,4) errors, [4,8) list overflow, [8] far flag
+    double4 *xa = cv.take<double4>(M, "positions"), *xb = cv.take<double4>(M, "positions");
+    double4 *fa = cv.take<double4>(M, "forces");
+    int *taga = cv.take<int>(M, "tags"), *tagb = cv.take<int>(M, "tags");
+    int *mola = cv.take<int>(M, "molecules"), *molb = cv.take<int>(M, "molecules");
+    int *imga = cv.take<int>(M, "images"), *origa = cv.take<int>(M, "order"), *origb = cv.take<int>(M, "order");
+    int *sys_d = cv.take<int>(M, "system ids");
+    uint32_t *key_d = cv.take<uint32_t>(M, "keys"), *key2_d = cv.take<uint32_t>(M, "keys");
+    int *val_d = cv.take<int>(M, "sort values"), *perm_d = cv.take<int>(M, "sort values");
+    int *cstart_d = cv.take<int>((size_t)g.ncells + 1, "cell ranges");
+    void *sorttmp_d = cv.take<unsigned char>(tb, "sort");
+    DevPtrs<double> P{};
+    P.n = m;
+    P.x = xb;
+    P.v = xa;  // no kernel of a plain evaluation touches velocities
+    P.f = fa;
+    P.mol = molb;
+    P.tag = tagb;
+    P.sys = sys_d;
+    P.sys_tab = systab_d;
+    P.scnt = cv.take<int>(M, "list counts");
+    P.icnt = cv.take<int>(M, "list counts");
+    P.ocnt = cv.take<int>(M, "list counts");
+    P.scap = scap_nf;
+    P.slist = cv.take<uint32_t>(M * scap_nf, "short lists");
+    P.nrm = cv.take<double4>(M, "normals");
+    P.dn = cv.take<double>(M * 27, "normal derivatives");
+    P.dndri = cv.take<double>(M * 9, "normal derivatives");
+    P.vet = cv.take<double>(M * 9, "normal neighbours");
+    P.nnb = cv.take<uint32_t>(M * 4, "normal neighbours");
+    P.tsh = cv.take<uint32_t>(M * MAXSH, "Tersoff neighbours");
+    P.tcnt = cv.take<int>(M, "Tersoff neighbours");
+    P.rtc = cv.take<uint32_t>(M * MAXSH, "reverse candidates");
+    P.rtcnt = cv.take<int>(M, "reverse candidates");
+    P.rnc = cv.take<uint32_t>(M * MAXSH, "reverse candidates");
+    P.rncnt = cv.take<int>(M, "reverse candidates");
+    P.eb = cv.take<double>(M * MAXSH, "bond energies");
+    P.fb = cv.take<double>(M * MAXSH * MAXSH * 3, "bond forces");
+    P.ft = cv.take<double4>(M, "Tersoff forces");
+    P.fil = cv.take<double4>(M, "ILP forces");
+    P.S = cv.take<double4>(M, "normal weights");
+    P.th = cv.take<double>(M * NTH, "thermo columns");
+    P.wb = nullptr;
+    P.T = nullptr;
+    P.vatom = nullptr;
+    const size_t row = cv.left() / (4 * M);
+    const int icap_nf = (int)std::min<size_t>(row / 64 * 64, (size_t)1 << 16);
+    if (icap_nf < 64)
+      fail(LMFF_ERR_MEMORY, "%lld atoms do not fit the %zu MiB scratch of the allocation-free entry point",
+           (long long)nlocal, LMFF_SCRATCH_BYTES >> 20);
+    P.icap = icap_nf;
+    P.ilist = cv.take<uint32_t>(M * icap_nf, "ILP lists");
+    P.err = word_d;
+    P.ovf = word_d + 4;
+    P.farflag = word_d + 8;
+    scalars(P);
+    const bool prof_was = prof;
+    prof = false;  // no timing events: they would be created on demand
+    struct Restore {
+      bool &p;
+      bool v;
+      ~Restore() { p = v; }
+    } restore{prof, prof_was};
+    const int nb = (m + BS - 1) / BS;
+    hipLaunchKernelGGL(k_store_sysdev, dim3(1), dim3(64), 0, stream, sd, systab_d);
+    hipLaunchKernelGGL(k_one_system_chunks, dim3((nch + BS - 1) / BS), dim3(BS), 0, stream, m, CHUNK, chunks_d, nch,
+                       cbeg_d);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 16, word_d, 0);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_fill_int, dim3(nb), dim3(BS), 0, stream, m, sys_d, 0);
+      hipLaunchKernelGGL(k_hb_load, dim3(nb), dim3(BS), 0, stream, m, x_d, type_d, (const long long *)tag_d,
+                         (const long long *)mol_d, xa, xb, taga, mola, origa, imga);
+      hipLaunchKernelGGL(k_check_types, dim3(nb), dim3(BS), 0, stream, m, (const double4 *)xa, ntypes,
+                         (const long long *)tag_d, (const long long *)mol_d, word_d + 9);
+      // the list build and the force kernels index parameter tables by type: check first
+      HIPCHK(hipMemcpyAsync(note_host + 9, word_d + 9, sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (note_host[9]) fail(LMFF_ERR_ARG, "atom types must be 1..%d, IDs 1..2^31-1, molecule IDs 32-bit", ntypes);
+      hipLaunchKernelGGL(k_wrap_key, dim3(nb), dim3(BS), 0, stream, m, xa, imga, (const int *)sys_d,
+                         (const SysDev *)systab_d, key_d, val_d, word_d);
+      size_t tb2 = tb;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(sorttmp_d, tb2, key_d, key2_d, val_d, perm_d, m, 0, ebit, stream));
+      hipLaunchKernelGGL(k_permute<double4>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d,
+                         (const double4 *)xa, xb);
+      hipLaunchKernelGGL(k_permute<int>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d, (const int *)taga, tagb);
+      hipLaunchKernelGGL(k_permute<int>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d, (const int *)mola, molb);
+      hipLaunchKernelGGL(k_permute<int>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d, (const int *)origa,
+                         origb);
+      hipLaunchKernelGGL(k_cell_start, dim3((m + 1 + BS - 1) / BS), dim3(BS), 0, stream, m, g.ncells,
+                         (const uint32_t *)key2_d, cstart_d);
+      hipLaunchKernelGGL(k_build_lists<double>, dim3((m + 3) / 4), dim3(BS), 0, stream, P, (const int *)cstart_d);
+      // lists must be whole before any force is added into the caller's array
+      HIPCHK(hipMemcpyAsync(note_host, word_d, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      raise_device_errors(note_host[0]);
+      if (note_host[4] > 0)
+        fail(LMFF_ERR_MEMORY, "short neighbor lists need %d entries per atom (the allocation-free entry point holds %d)",
+             note_host[4], scap_nf);
+      if (note_host[5] > 0)
+        fail(LMFF_ERR_MEMORY, "ILP lists need %d entries per atom; %lld atoms leave room for %d in the %zu MiB scratch",
+             note_host[5], (long long)nlocal, icap_nf, LMFF_SCRATCH_BYTES >> 20);
+      const int vf = (vflag % 4) != 0 ? 1 : 0;
+      const int ef = (eflag != 0 || vf) ? 1 : 0;
+      if (precision == LMFF_MIXED) {
+        DevPtrs<float> Q{};
+        std::memcpy((void *)&Q, (const void *)&P, sizeof(Q));  // same layout, R-typed arrays reinterpreted
+        Q.nrm = (float4 *)P.nrm;
+        Q.dn = (float *)P.dn;
+        Q.dndri = (float *)P.dndri;
+        Q.wb = nullptr;
+        scalars(Q);
+        force_kernels<float>(Q, ef, vf, 0, 0, 0);
+      } else {
+        force_kernels<double>(P, ef, vf, 0, 0, 0);
+      }
+      if (f_d) hipLaunchKernelGGL(k_add_f_caller, dim3(nb), dim3(BS), 0, stream, m, (const int *)origb,
+                                  (const double4 *)fa, f_d);
+      if ((eflag & 1) || vf) {
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(nch), dim3(BS), 0, stream, (const double *)P.th, (size_t)m, 9,
+                           (const Chunk *)chunks_d, partial_d, 0);
+        hipLaunchKernelGGL(k_reduce_systems, dim3(1), dim3(64), 0, stream, (const double2 *)partial_d, 9,
+                           (const int *)cbeg_d, 1, sums_d, 0);
+      }
+    }
+    if (glob_d)
+      hipLaunchKernelGGL(k_pair_globals, dim3(1), dim3(64), 0, stream, (const double *)sums_d, m > 0 ? eflag : 0,
+                         m > 0 ? vflag : 0, glob_d);
+    HIPCHK(hipMemcpyAsync(note_host, word_d, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    raise_device_errors(note_host[0]);
+  }
+
   // ---------------------------------------------------------------- downloads
   template <typename T>
   std::vector<T> download(const DBuf &b, size_t off, size_t count) {
@@ -1366,7 +1595,8 @@ void check_box(const double lo[3], const double hi[3], const int per[3], double 
 
 // Bins: at least a sixth of the list cutoff along x and a third along y and z (so the list
 // build's stencil has at most 7 x 7 rows; the factor keeps ceil(cut / bin) at 3 whatever the
-// rounding).
+// rounding).  A non-periodic side spans the atoms' extent, or the box bounds when x is null
+// (atoms beyond the bins fall into the edge bins, which reach to infinity).
 void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
   const double target[3] = {cutneigh / 6.0, cutneigh / 3.0 * (1.0 + 1e-6), cutneigh / 3.0 * (1.0 + 1e-6)};
   s.ncells = 1;
@@ -1379,11 +1609,16 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
       s.bs[d] = L / s.nc[d];
     } else {
       double mn = 1e300, mx = -1e300;
-      for (int64_t i = 0; i < n; i++) {
-        mn = std::min(mn, x[3 * i + d]);
-        mx = std::max(mx, x[3 * i + d]);
+      if (!x) {
+        mn = s.lo[d];
+        mx = s.hi[d];
+      } else {
+        for (int64_t i = 0; i < n; i++) {
+          mn = std::min(mn, x[3 * i + d]);
+          mx = std::max(mx, x[3 * i + d]);
+        }
+        if (n == 0) mn = mx = 0.0;
       }
-      if (n == 0) mn = mx = 0.0;
       const double ext = mx - mn;
       s.nc[d] = std::max(1, (int)(ext / half));
       s.glo[d] = mn;
@@ -1409,9 +1644,21 @@ int lmff_create(lmff_ctx **out, int device) {
   *out = nullptr;
   lmff_ctx *ctx = new lmff_ctx();
   int rc = guard(ctx, [&] {
+    if (device < 0 || device >= MAX_DEVICES) fail(LMFF_ERR_ARG, "device %d out of range", device);
     ctx->c.device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    {
+      // this device's copy of the allocation-free entry point's scratch (with the device current)
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = p;
+      }
+      ctx->c.scratch_dev = (unsigned char *)g_scratch_ptr[device];
+    }
+    HIPCHK(hipHostMalloc((void **)&ctx->c.note_host, 16 * sizeof(int), hipHostMallocDefault));
   });
   if (rc) {
     delete ctx;
@@ -1957,6 +2204,18 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
     }
   });
 }
+
+int lmff_pair_compute_noalloc(lmff_ctx *ctx, int64_t nlocal, const double *x, const int32_t *type, const int64_t *tag,
+                              const int64_t *mol, const double boxlo[3], const double boxhi[3], const int periodic[3],
+                              int eflag, int vflag, double *f, double *globals) {
+  return guard(ctx, [&] {
+    if (!boxlo || !boxhi || !periodic) fail(LMFF_ERR_ARG, "box arguments missing");
+    HIPCHK(hipSetDevice(ctx->c.device));
+    ctx->c.pair_compute_noalloc(nlocal, x, type, tag, mol, boxlo, boxhi, periodic, eflag, vflag, f, globals);
+  });
+}
+
+int64_t lmff_noalloc_scratch_bytes(void) { return (int64_t)LMFF_SCRATCH_BYTES; }
 
 int lmff_profile(lmff_ctx *ctx, int enable) {
   return guard(ctx, [&] { ctx->c.prof = enable != 0; });

@@ -356,6 +356,45 @@ __global__ void k_hb_load(int m, const double *xin, const int *type, const long 
   img[q] = IMG_ZERO;
 }
 
+// Forces of the system added into the caller's device array in caller order (f += semantics
+// of Pair::compute), for the entry point that takes device arrays.
+__global__ void k_add_f_caller(int m, const int *orig, const double4 *f, double *fout) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t i = (size_t)orig[q];
+  const double4 a = f[q];
+  fout[3 * i] += a.x;
+  fout[3 * i + 1] += a.y;
+  fout[3 * i + 2] += a.z;
+}
+
+// Atom types (in x.w), tags and molecule IDs the device layout can hold: flag != 0 otherwise.
+__global__ void k_check_types(int m, const double4 *x, int ntypes, const long long *tag, const long long *mol,
+                              int *flag) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const double t = x[q].w;
+  const long long tg = tag[q], mo = mol[q];
+  if (!(t >= 1.0 && t <= (double)ntypes) || tg <= 0 || tg > 2147483647LL || mo < -2147483648LL || mo > 2147483647LL)
+    atomicOr(flag, 1);
+}
+
+// A one-system table written by a kernel from its launch argument (no host-to-device copy).
+__global__ void k_store_sysdev(SysDev s, SysDev *dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = s;
+}
+
+// Pair::compute's global outputs from the per-system column sums (e_vdw, e_rep, e_tersoff,
+// virial[6]): out = {eng_vdwl, virial[6], pvector[2]}, zero where the flags ask for nothing.
+__global__ void k_pair_globals(const double *sums, int eflag, int vflag, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const bool e = (eflag & 1) != 0, v = (vflag % 4) != 0;
+  out[0] = e ? sums[0] + sums[1] + sums[2] : 0.0;
+  for (int c = 0; c < 6; c++) out[1 + c] = v ? sums[3 + c] : 0.0;
+  out[7] = e ? sums[0] : 0.0;
+  out[8] = e ? sums[1] : 0.0;
+}
+
 // cell_start[c] = first sorted position with key >= c, for c in [0, ncells]
 __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1691,6 +1730,16 @@ __global__ void k_advance_step(int nsys, SysDev *st) {
 struct Chunk {
   int begin, end, sys, pad;
 };
+
+// chunks of a single system of n atoms: [k chunk, min((k+1) chunk, n)), system 0
+__global__ void k_one_system_chunks(int n, int chunk, Chunk *chunks, int nchunks, int *cbeg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nchunks) chunks[k] = Chunk{k * chunk, min((k + 1) * chunk, n), 0, 0};
+  if (k == 0) {
+    cbeg[0] = 0;
+    cbeg[1] = nchunks;
+  }
+}
 
 __device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
   const double t = sum + v;

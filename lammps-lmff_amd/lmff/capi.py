@@ -31,6 +31,7 @@ SYMBOLS = [
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
     "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
     "lmff_minimize", "lmff_minimize_cg", "lmff_system_get_image", "lmff_system_set_image",
+    "lmff_pair_compute_noalloc", "lmff_noalloc_scratch_bytes",
 ]
 
 STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
@@ -103,6 +104,10 @@ def load(path: str = LIB_PATH):
     L.lmff_masses.argtypes = [vp, _d]
     L.lmff_pair_compute.argtypes = [vp, C.c_int64, _d, _i32, _i64, _i64, _d, _d, _int, C.c_int, C.c_int, C.c_int,
                                     _d, _d, _d, _d, _d, _d]
+    # device pointers (the allocation-free entry point takes device arrays)
+    L.lmff_pair_compute_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, _d, _d, _int, C.c_int, C.c_int, vp, vp]
+    L.lmff_noalloc_scratch_bytes.restype = C.c_int64
+    L.lmff_noalloc_scratch_bytes.argtypes = []
     L.lmff_system_add.argtypes = [vp, C.c_int64, _d, _d, _int, _i32, _i64, _i64, _d, _d, _int]
     L.lmff_system_remove.argtypes = [vp, C.c_int]
     L.lmff_system_integrator.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
@@ -225,6 +230,19 @@ class Context:
                                            _p(eatom, C.c_double), _p(vatom, C.c_double), _p(pv, C.c_double)))
         return {"f": f, "energy": float(eng[0]), "e_vdw": float(pv[0]), "e_rep": float(pv[1]),
                 "e_tersoff": float(eng[0] - pv[0] - pv[1]), "virial": vir, "eatom": eatom, "vatom": vatom}
+
+    def pair_compute_noalloc(self, n: int, x_dev: int, type_dev: int, tag_dev: int, mol_dev: int, boxlo, boxhi,
+                             periodic, eflag: int, vflag: int, f_dev: int, globals_dev: int):
+        """The allocation-free entry point: every array argument is a device address (int),
+        f_dev accumulates forces [n][3], globals_dev receives {eng_vdwl, virial[6], pvector[2]};
+        0 for an output leaves it out."""
+        lo = np.ascontiguousarray(boxlo, dtype=np.float64)
+        hi = np.ascontiguousarray(boxhi, dtype=np.float64)
+        per = np.ascontiguousarray(periodic, dtype=np.int32)
+        vp = C.c_void_p
+        self._chk(self.L.lmff_pair_compute_noalloc(self.h, int(n), vp(x_dev), vp(type_dev), vp(tag_dev), vp(mol_dev),
+                                                   _p(lo, C.c_double), _p(hi, C.c_double), _p(per, C.c_int),
+                                                   int(eflag), int(vflag), vp(f_dev or None), vp(globals_dev or None)))
 
     # ---- resident systems
     def add_system(self, s, v=None) -> int:
