@@ -1232,6 +1232,11 @@ struct Ctx {
     if (need_rebuild) rebuild();
     if (!forces_valid) {
       force(0, 0);
+      // FixLangevin::setup: drag and noise on the set-up force (systems that kept their last-step
+      // force get it back below, unchanged)
+      hipLaunchKernelGGL(k_langevin_setup, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, (const double4 *)x.p,
+                         (const double4 *)v.p, f.as<double4>(), (const int *)sysid.p, (const int *)tag.p,
+                         (const SysDev *)systab.p, (const double *)rmass.p);
       for (auto &r : restore_f)
         HIPCHK(hipMemcpyAsync(f.as<double4>() + r.first, fsave.as<double4>() + r.first,
                               (size_t)(r.second - r.first) * sizeof(double4), hipMemcpyDeviceToDevice, stream));

@@ -1471,6 +1471,10 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
 // already advanced the step counter, so the noise is keyed by step - 1, the step
 // k_final_integrate sees before k_advance_step.
 __device__ __forceinline__ void philox4x32(uint32_t ctr[4], uint32_t key[2]);
+// Langevin noise: Philox4x32-10 with counter {step low, step high, atom tag, word}, key = the
+// system's 64-bit seed; the per-step draws and the run set-up draw use different words
+constexpr uint32_t PHILOX_STEP_WORD = 0x4c4d4646u;   // "LMFF"
+constexpr uint32_t PHILOX_SETUP_WORD = 0x53455455u;  // "SETU"
 
 template <typename R, bool EF, bool VA, bool INT>
 __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad, int want_ke) {
@@ -1605,7 +1609,7 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
       const double gamma1 = -m / s.damp / FTM2V;
       const double gamma2 = sqrt(m) * sqrt(24.0 * BOLTZ / s.damp / s.dtv / MVV2E) / FTM2V * sqrt(s.t_target);
       const long long step = s.step - 1;
-      uint32_t ctr[4] = {(uint32_t)step, (uint32_t)((unsigned long long)step >> 32), (uint32_t)P.tag[a], 0x4c4d4646u};
+      uint32_t ctr[4] = {(uint32_t)step, (uint32_t)((unsigned long long)step >> 32), (uint32_t)P.tag[a], PHILOX_STEP_WORD};
       uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
       philox4x32(ctr, key);
       const double u0 = (ctr[0] + 0.5) * 2.3283064365386963e-10 - 0.5;
@@ -1699,7 +1703,7 @@ __global__ void k_final_integrate(int n, const double4 *x, double4 *v, double4 *
   if (s.style == 1) {
     const double gamma1 = -m / s.damp / FTM2V;
     const double gamma2 = sqrt(m) * sqrt(24.0 * BOLTZ / s.damp / s.dtv / MVV2E) / FTM2V * sqrt(s.t_target);
-    uint32_t ctr[4] = {(uint32_t)s.step, (uint32_t)((unsigned long long)s.step >> 32), (uint32_t)tag[p], 0x4c4d4646u};
+    uint32_t ctr[4] = {(uint32_t)s.step, (uint32_t)((unsigned long long)s.step >> 32), (uint32_t)tag[p], PHILOX_STEP_WORD};
     uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
     philox4x32(ctr, key);
     const double u0 = (ctr[0] + 0.5) * 2.3283064365386963e-10 - 0.5;
@@ -1716,6 +1720,33 @@ __global__ void k_final_integrate(int n, const double4 *x, double4 *v, double4 *
   vp.z += dtfm * fp.z;
   v[p] = vp;
   if (want_ke) ke[p] = 0.5 * MVV2E * m * (vp.x * vp.x + vp.y * vp.y + vp.z * vp.z);
+}
+
+// FixLangevin::setup applies post_force to the set-up force (LMFF/fix_langevin.cpp:376-377), so
+// the first half-kick of a run already carries drag and noise.  The set-up draw has its own
+// counter word, keyed by the step counter at set-up and the atom tag.
+__global__ void k_langevin_setup(int n, const double4 *x, const double4 *v, double4 *f, const int *sys,
+                                 const int *tag, const SysDev *st, const double *rmass_type) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const SysDev &s = st[sys[p]];
+  if (s.style != 1) return;
+  const double m = rmass_type[(int)x[p].w - 1];
+  const double4 vp = v[p];
+  double4 fp = f[p];
+  const double gamma1 = -m / s.damp / FTM2V;
+  const double gamma2 = sqrt(m) * sqrt(24.0 * BOLTZ / s.damp / s.dtv / MVV2E) / FTM2V * sqrt(s.t_target);
+  uint32_t ctr[4] = {(uint32_t)s.step, (uint32_t)((unsigned long long)s.step >> 32), (uint32_t)tag[p],
+                     PHILOX_SETUP_WORD};
+  uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
+  philox4x32(ctr, key);
+  const double u0 = (ctr[0] + 0.5) * 2.3283064365386963e-10 - 0.5;
+  const double u1 = (ctr[1] + 0.5) * 2.3283064365386963e-10 - 0.5;
+  const double u2 = (ctr[2] + 0.5) * 2.3283064365386963e-10 - 0.5;
+  fp.x += gamma1 * vp.x + gamma2 * u0;
+  fp.y += gamma1 * vp.y + gamma2 * u1;
+  fp.z += gamma1 * vp.z + gamma2 * u2;
+  f[p] = fp;
 }
 
 __global__ void k_advance_step(int nsys, SysDev *st) {
