@@ -66,6 +66,9 @@ int lmff_coeff_tersoff(lmff_ctx *ctx, const char *file, const char *const *eleme
 int lmff_coeff_ilp(lmff_ctx *ctx, const char *file, const char *const *elements);
 /* neighbor <skin> bin ; neigh_modify every <every> delay <delay> check <yes=1|no=0> */
 int lmff_neighbor(lmff_ctx *ctx, double skin, int every, int delay, int check);
+/* PairTersoff::init_one's cutoff (Baseline/MANYBODY/pair_tersoff.cpp:383-388): the largest
+ * R + D of the Tersoff entries the element map uses (after lmff_coeff_tersoff). */
+int lmff_tersoff_cutmax(lmff_ctx *ctx, double *cutmax);
 /* LMFF_FP64 or LMFF_MIXED (FP32 pair/bond-order math, FP64 displacements and accumulation) */
 int lmff_precision(lmff_ctx *ctx, int mode);
 /* mass <type> <value> for all types (index type-1) */

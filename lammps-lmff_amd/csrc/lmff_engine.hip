@@ -1750,6 +1750,14 @@ int lmff_neighbor(lmff_ctx *ctx, double skin, int every, int delay, int check) {
   });
 }
 
+int lmff_tersoff_cutmax(lmff_ctx *ctx, double *cutmax) {
+  return guard(ctx, [&] {
+    if (!cutmax) fail(LMFF_ERR_ARG, "null output");
+    if (!ctx->c.have_tersoff) fail(LMFF_ERR_STATE, "pair_coeff for tersoff has not been given");
+    *cutmax = ctx->c.ters.cutmax;
+  });
+}
+
 int lmff_precision(lmff_ctx *ctx, int mode) {
   return guard(ctx, [&] {
     if (mode != LMFF_FP64 && mode != LMFF_MIXED) fail(LMFF_ERR_ARG, "precision must be LMFF_FP64 or LMFF_MIXED");

@@ -31,7 +31,7 @@ SYMBOLS = [
     "lmff_profile_read", "lmff_memory", "lmff_system_bytes", "lmff_debug_ilp_list", "lmff_debug_normals",
     "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
     "lmff_minimize", "lmff_minimize_cg", "lmff_system_get_image", "lmff_system_set_image",
-    "lmff_pair_compute_noalloc", "lmff_noalloc_scratch_bytes",
+    "lmff_pair_compute_noalloc", "lmff_noalloc_scratch_bytes", "lmff_tersoff_cutmax",
 ]
 
 STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
     L.lmff_coeff_ilp.argtypes = [vp, C.c_char_p, C.POINTER(C.c_char_p)]
     L.lmff_neighbor.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int]
     L.lmff_precision.argtypes = [vp, C.c_int]
+    L.lmff_tersoff_cutmax.argtypes = [vp, _d]
     L.lmff_masses.argtypes = [vp, _d]
     L.lmff_pair_compute.argtypes = [vp, C.c_int64, _d, _i32, _i64, _i64, _d, _d, _int, C.c_int, C.c_int, C.c_int,
                                     _d, _d, _d, _d, _d, _d]
