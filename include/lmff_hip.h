@@ -214,6 +214,13 @@ int lmff_system_thermo(lmff_ctx *ctx, int sys_id, lmff_thermo *out);
 int lmff_run(lmff_ctx *ctx, int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out,
              int64_t out_capacity);
 
+/* Neighbor list builds and dangerous builds since the last reset (Neighbor::ncalls / ndanger,
+ * LMFF/neighbor.cpp:2026,2041, reported by Finish, LMFF/finish.cpp:814-819): a dangerous build
+ * is one the distance check (neigh_modify check yes) triggered on the first step it was
+ * allowed to, so some atom may have moved past half the skin before the list caught up.
+ * reset != 0 zeroes both after reading (Neighbor::init does so before every run). */
+int lmff_neighbor_stats(lmff_ctx *ctx, int reset, int64_t *builds, int64_t *dangerous);
+
 /* Device-side timing of the last lmff_run: total ms and per-kernel-family averages
  * (index: 0 list build, 1 normals, 2 ILP pair, 3 Tersoff bonds, 4 force gather,
  *  5 integrate, 6 reductions).  Requires lmff_profile(ctx, 1) before the run. */

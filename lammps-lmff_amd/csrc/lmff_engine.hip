@@ -161,6 +161,10 @@ struct Ctx {
   bool forces_valid = false;
   int ago = 0;
   long long global_step = 0;
+  // Neighbor::ncalls and ndanger (LMFF/neighbor.cpp:2026,2041): list builds, and builds the
+  // distance check asked for on the first step it was allowed to (an atom may already have
+  // moved past half the skin before), since the last reset
+  long long ncalls = 0, ndanger = 0;
   // the last step of every run also tallies per-atom energy and virial (for dumps of
   // compute pe/atom and stress/atom, which LAMMPS requests on the steps they are output)
   bool peratom_last = false;
@@ -705,6 +709,7 @@ struct Ctx {
     mark_end(0, mk);
     need_rebuild = false;
     ago = 0;
+    ncalls++;
   }
 
   void permute4(DBuf &b) {
@@ -1195,6 +1200,7 @@ struct Ctx {
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    if (h && ago == std::max(every, delay)) ndanger++;
     return h != 0;
   }
 
@@ -2229,6 +2235,14 @@ int lmff_pair_compute_noalloc(lmff_ctx *ctx, int64_t nlocal, const double *x, co
 }
 
 int64_t lmff_noalloc_scratch_bytes(void) { return (int64_t)LMFF_SCRATCH_BYTES; }
+
+int lmff_neighbor_stats(lmff_ctx *ctx, int reset, int64_t *builds, int64_t *dangerous) {
+  return guard(ctx, [&] {
+    if (builds) *builds = ctx->c.ncalls;
+    if (dangerous) *dangerous = ctx->c.ndanger;
+    if (reset) ctx->c.ncalls = ctx->c.ndanger = 0;
+  });
+}
 
 int lmff_profile(lmff_ctx *ctx, int enable) {
   return guard(ctx, [&] { ctx->c.prof = enable != 0; });

@@ -527,7 +527,7 @@ void write_dump(Dump &dp, const Data &d, int64_t step, const std::string &bounds
 
 }  // namespace
 
-int main(int argc, char **argv) {
+int run_deck(int argc, char **argv) {
   std::string deck;
   std::map<std::string, std::string> vars;
   for (int i = 1; i < argc; i++) {
@@ -546,10 +546,9 @@ int main(int argc, char **argv) {
   std::vector<std::vector<std::string>> coeffs;
   double skin = 2.0, dt = 0.001;
   int every = 1, delay = 10, check = 1, thermo_every = 0, precision = LMFF_FP64;
-  int style = LMFF_NVE;
   double t_target = 0, damp = 1;
   uint64_t lseed = 1;
-  bool have_fix = false;
+  std::map<std::string, std::string> fixes;  // fix ID -> "nve" | "langevin"
   std::unique_ptr<lmff::PairLMFFHip> pair;
   int sys = -1;
   int64_t step = 0;
@@ -696,25 +695,40 @@ int main(int argc, char **argv) {
       std::printf("  %lld atoms\n", (long long)data.n);
     } else if (cmd == "mass") {
       data.mass[std::stoi(w.at(1))] = std::stod(w.at(2));
+      if (pair) {
+        // after the first run or minimize the library holds the masses: forward the change
+        std::vector<double> m(data.ntypes);
+        for (int t = 1; t <= data.ntypes; t++) m[t - 1] = data.mass.count(t) ? data.mass[t] : 0.0;
+        if (lmff_masses(pair->context(), m.data())) die(lmff_last_error(pair->context()));
+      }
     } else if (cmd == "pair_style") {
       if (w.size() < 5 || w[1] != "hybrid/overlay" || (w[2] != "tersoff" && w[2] != "lmff") ||
           w[3] != "ilp/graphene/hbn" || w.size() > 6)
         die("lmff_run supports pair_style hybrid/overlay tersoff|lmff ilp/graphene/hbn <cut> [tap]");
+      if (pair) die("lmff_run cannot change pair_style after the first run or minimize");
       pair_args.assign(w.begin() + 4, w.end());
     } else if (cmd == "pair_coeff") {
       coeffs.push_back(std::vector<std::string>(w.begin() + 1, w.end()));
+      if (pair) {
+        std::vector<char *> b;
+        for (auto &q : coeffs.back()) b.push_back(const_cast<char *>(q.c_str()));
+        pair->coeff((int)b.size(), b.data());
+      }
     } else if (cmd == "neighbor") {
       skin = std::stod(w.at(1));
+      if (pair && lmff_neighbor(pair->context(), skin, every, delay, check)) die(lmff_last_error(pair->context()));
     } else if (cmd == "neigh_modify") {
       for (size_t k = 1; k + 1 < w.size(); k++) {
         if (w[k] == "every") every = std::stoi(w[k + 1]);
         else if (w[k] == "delay") delay = std::stoi(w[k + 1]);
         else if (w[k] == "check") check = w[k + 1] == "yes";
       }
+      if (pair && lmff_neighbor(pair->context(), skin, every, delay, check)) die(lmff_last_error(pair->context()));
     } else if (cmd == "timestep") {
       dt = std::stod(w.at(1));
     } else if (cmd == "lmff_precision") {
       precision = w.at(1) == "mixed" ? LMFF_MIXED : LMFF_FP64;
+      if (pair) pair->set_precision(precision);
     } else if (cmd == "velocity") {
       // velocity all create T seed [dist uniform|gaussian] [mom yes|no] [rot yes|no] and
       // velocity all scale T (Velocity::create / scale, LMFF/velocity.cpp:162-380, 579):
@@ -817,24 +831,54 @@ int main(int argc, char **argv) {
       for (auto &q : data.v) q *= sc;
       if (pair && lmff_system_set(pair->context(), sys, nullptr, data.v.data())) die(lmff_last_error(pair->context()));
     } else if (cmd == "fix") {
-      if (w.size() >= 4 && w[3] == "nve") {
-        have_fix = true;
-      } else if (w.size() >= 8 && w[3] == "langevin") {
-        style = LMFF_LANGEVIN;
+      // fix ID all nve | fix ID all langevin T T damp seed [keyword no ...] (FixNVE,
+      // FixLangevin: LMFF/fix_nve.cpp:30-48, LMFF/fix_langevin.cpp:58-150).  The device
+      // integrates every atom, so other groups, other fix styles and the keywords that change the
+      // dynamics are refused rather than skipped.
+      if (w.size() < 4) die("Illegal fix command");
+      const std::string &id = w[1], &grp = w[2], &fstyle = w[3];
+      if (fixes.count(id)) die("lmff_run cannot replace fix " + id + "; unfix it first");
+      if (fstyle != "nve" && fstyle != "langevin")
+        die("fix style " + fstyle + " is not supported by lmff_run (only fix nve and fix langevin)");
+      if (grp != "all") die("lmff_run integrates group all only: fix " + id + " " + grp + " " + fstyle);
+      for (const auto &f : fixes)
+        if (f.second == fstyle) die("lmff_run supports one fix " + fstyle + " (fix " + f.first + " exists)");
+      if (fstyle == "nve") {
+        if (w.size() != 4) die("Illegal fix nve command");
+      } else {
+        if (w.size() < 8 || (w.size() - 8) % 2) die("Illegal fix langevin command");
+        if (std::stod(w[4]) != std::stod(w[5])) die("lmff_run supports fix langevin with Tstart = Tstop");
+        for (size_t k = 8; k < w.size(); k += 2) {
+          const std::string &kw = w[k], &val = w[k + 1];
+          const bool known = kw == "zero" || kw == "tally" || kw == "gjf" || kw == "omega" || kw == "angmom";
+          if (!known) die("Illegal fix langevin command: " + kw);
+          if (val != "no") die("fix langevin " + kw + " " + val + " is not supported by lmff_run");
+        }
         t_target = std::stod(w[4]);
         damp = std::stod(w[6]);
         lseed = std::stoull(w[7]);
-      } else {
-        std::printf("WARNING: fix %s ignored by lmff_run\n", w.size() > 3 ? w[3].c_str() : "?");
       }
+      fixes[id] = fstyle;
+    } else if (cmd == "unfix") {
+      if (w.size() != 2) die("Illegal unfix command");
+      if (!fixes.erase(w[1])) die("Could not find fix ID " + w[1] + " to delete");
     } else if (cmd == "thermo") {
       thermo_every = std::stoi(w.at(1));
     } else if (cmd == "run") {
       const int64_t nsteps = std::stoll(w.at(1));
       setup_pair();
       lmff_ctx *ctx = pair->context();
-      if (!have_fix) std::printf("WARNING: No fixes defined, atoms won't move\n");
+      bool has_nve = false, has_langevin = false;
+      for (const auto &f : fixes) {
+        has_nve = has_nve || f.second == "nve";
+        has_langevin = has_langevin || f.second == "langevin";
+      }
+      // without fix nve LAMMPS moves no atom; the device loop always integrates, so such runs
+      // are refused (run 0 only evaluates)
+      if (nsteps > 0 && !has_nve) die("lmff_run needs fix ID all nve for a run of more than 0 steps");
+      const int style = has_langevin ? LMFF_LANGEVIN : LMFF_NVE;
       if (lmff_system_integrator(ctx, sys, style, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
+      if (lmff_neighbor_stats(ctx, 1, nullptr, nullptr)) die(lmff_last_error(ctx));  // Neighbor::init
       bool want_peratom = false;
       for (const auto &dp : dumps)
         for (const auto &c : dp.fields)
@@ -904,6 +948,12 @@ int main(int argc, char **argv) {
       std::printf("Performance: %.3f ns/day, %.3f hours/ns, %.3f timesteps/s\n", tps * dt * 86400.0 / 1000.0,
                   tps > 0 ? 1.0 / (tps * dt * 3.6) : 0.0, tps);
       std::printf("Performance: %.4g atom-steps/s\n", tps * data.n);
+      int64_t nbuild = 0, ndanger = 0;
+      if (lmff_neighbor_stats(ctx, 0, &nbuild, &ndanger)) die(lmff_last_error(ctx));
+      // Finish's neighbor lines (LMFF/finish.cpp:814-819); builds counted on the device
+      std::printf("Neighbor list builds = %lld\n", (long long)nbuild);
+      if (check) std::printf("Dangerous builds = %lld\n", (long long)ndanger);
+      else std::printf("Dangerous builds not checked\n");
     } else if (cmd == "compute") {
       if (w.size() >= 4 && (w[3] == "pe/atom" || w[3] == "stress/atom")) computes[w[1]] = w[3];
       else std::printf("WARNING: compute %s ignored by lmff_run\n", w.size() > 3 ? w[3].c_str() : "?");
@@ -1017,7 +1067,8 @@ int main(int argc, char **argv) {
         std::printf("WARNING: min_style %s is not available in lmff_run; minimizing with cg\n", min_style.c_str());
       setup_pair();
       lmff_ctx *ctx = pair->context();
-      if (lmff_system_integrator(ctx, sys, style, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
+      // fix langevin has no minimizer hook: the minimizers see the conservative force only
+      if (lmff_system_integrator(ctx, sys, LMFF_NVE, dt, t_target, damp, lseed)) die(lmff_last_error(ctx));
       lmff_min_stats ms;
       auto c0 = std::chrono::steady_clock::now();
       const int rc = min_style == "fire"
@@ -1039,8 +1090,7 @@ int main(int argc, char **argv) {
       std::printf("  Force two-norm initial, final = %g %g\n", ms.fnorm2_initial, ms.fnorm2_final);
       std::printf("  Force max component initial, final = %g %g\n", ms.fnorminf_initial, ms.fnorminf_final);
       std::printf("  Iterations, force evaluations = %lld %lld\n\n", (long long)ms.iterations, (long long)ms.evaluations);
-    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "unfix" || cmd == "fix_modify" ||
-               cmd == "thermo_modify") {
+    } else if (cmd == "group" || cmd == "thermo_style" || cmd == "fix_modify" || cmd == "thermo_modify") {
       std::printf("WARNING: %s ignored by lmff_run\n", cmd.c_str());
     } else {
       die("Unknown command: " + line);
@@ -1049,4 +1099,16 @@ int main(int argc, char **argv) {
   for (auto &dp : dumps)
     if (dp.fp) std::fclose(dp.fp);
   return 0;
+}
+
+int main(int argc, char **argv) {
+  // library errors from the Pair adaptor and malformed numbers end the run as LAMMPS'
+  // error->all does: the message and a non-zero exit
+  try {
+    return run_deck(argc, argv);
+  } catch (const lmff::PairError &e) {
+    die(e.what());
+  } catch (const std::exception &e) {
+    die(std::string("lmff_run: ") + e.what());
+  }
 }

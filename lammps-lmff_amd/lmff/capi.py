@@ -32,6 +32,7 @@ SYMBOLS = [
     "lmff_pair_stats", "lmff_system_set_step", "lmff_peratom_on_last_step", "lmff_fire_defaults",
     "lmff_minimize", "lmff_minimize_cg", "lmff_system_get_image", "lmff_system_set_image",
     "lmff_pair_compute_noalloc", "lmff_noalloc_scratch_bytes", "lmff_tersoff_cutmax",
+    "lmff_neighbor_stats",
 ]
 
 STOP = {0: "max iterations", 1: "max force evaluations", 2: "energy tolerance", 3: "force tolerance",
@@ -102,6 +103,7 @@ def load(path: str = LIB_PATH):
     L.lmff_neighbor.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int]
     L.lmff_precision.argtypes = [vp, C.c_int]
     L.lmff_tersoff_cutmax.argtypes = [vp, _d]
+    L.lmff_neighbor_stats.argtypes = [vp, C.c_int, _i64, _i64]
     L.lmff_masses.argtypes = [vp, _d]
     L.lmff_pair_compute.argtypes = [vp, C.c_int64, _d, _i32, _i64, _i64, _d, _d, _int, C.c_int, C.c_int, C.c_int,
                                     _d, _d, _d, _d, _d, _d]
@@ -348,6 +350,13 @@ class Context:
         buf = (Thermo * max(nrec * nsys, 1))()
         self._chk(self.L.lmff_run(self.h, int(nsteps), int(thermo_every), int(bool(use_graph)), buf, nrec * nsys))
         return [[buf[k * nrec + r].as_dict() for r in range(nrec)] for k in range(nsys)]
+
+    def neighbor_stats(self, reset: bool = False):
+        """(list builds, dangerous builds) since the last reset."""
+        b = np.zeros(1, dtype=np.int64)
+        d = np.zeros(1, dtype=np.int64)
+        self._chk(self.L.lmff_neighbor_stats(self.h, int(reset), _p(b, C.c_int64), _p(d, C.c_int64)))
+        return int(b[0]), int(d[0])
 
     def profile(self, enable: bool = True):
         self._chk(self.L.lmff_profile(self.h, int(enable)))
