@@ -907,6 +907,11 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
   const int itag = P.tag[i];
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
   double w[6] = {0, 0, 0, 0, 0, 0}, va[6] = {0, 0, 0, 0, 0, 0}, tt[6] = {0, 0, 0, 0, 0, 0};
+  // mixed precision outside the tallies: each lane sums its pairs' force and S terms in FP32
+  // (a lane sees one pair in 64 of the row: at most a few tens) and adds that sum to the FP64
+  // accumulators once, after the loop; the wave reduction and everything after it stay FP64
+  constexpr bool LANE32 = sizeof(R) == 4 && !VF && !VA;
+  R fxl = 0, fyl = 0, fzl = 0, sxl = 0, syl = 0, szl = 0;
   // software pipeline: the entry two chunks ahead and the position one chunk ahead are in
   // flight while a chunk computes (15 % off the FP32 kernel, 2 % off the FP64 one)
   uint32_t e_n1 = 0, e_n2 = 0;
@@ -931,12 +936,22 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
       ghost(xj, e, prd, gx, gy, gz);
     }
     const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
-    const double rsqd = rsq_ref(dxd, dyd, dzd);
-    if (rsqd >= P.ilp_cutsq) continue;
+    const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
+    R rsq;
+    if (sizeof(R) == 8) {
+      // FP64: the cutoff decision in the reference's operation order
+      const double rsqd = rsq_ref(dxd, dyd, dzd);
+      if (rsqd >= P.ilp_cutsq) continue;
+      rsq = (R)rsqd;
+    } else {
+      // mixed: the test on the FP32 distance of the FP64 displacement; a pair within rounding
+      // of the cutoff contributes Tap(rc) = dTap(rc) = 0 either way
+      rsq = dx * dx + dy * dy + dz * dz;
+      if (rsq >= (R)P.ilp_cutsq) continue;
+    }
     const int tj = (int)xj.w - 1;
     const IlpPair<R> &pij = s_ip[ti * MAXT + tj];
     const IlpPair<R> &pji = s_ip[tj * MAXT + ti];
-    const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd, rsq = (R)rsqd;
     const R rinv = M::rsqrt(rsq);
     const R r = rsq * rinv;
     const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
@@ -975,12 +990,21 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
     const R ta1 = a1 * tp.t, ta2 = a2 * tp.t;
     const R Cd = tp.t * (fpv + rinv * (pij.lambda * V1 + pji.lambda * V2) + fq1 + fq2) -
                  tp.dt * rinv * (Vvdw + V1 + V2);
-    fx += (double)(Cd * dx + (ta2 * nj4.x - ta1 * nix));
-    fy += (double)(Cd * dy + (ta2 * nj4.y - ta1 * niy));
-    fz += (double)(Cd * dz + (ta2 * nj4.z - ta1 * niz));
-    sx += (double)(ta1 * dx);
-    sy += (double)(ta1 * dy);
-    sz += (double)(ta1 * dz);
+    if (LANE32) {
+      fxl += Cd * dx + (ta2 * nj4.x - ta1 * nix);
+      fyl += Cd * dy + (ta2 * nj4.y - ta1 * niy);
+      fzl += Cd * dz + (ta2 * nj4.z - ta1 * niz);
+      sxl += ta1 * dx;
+      syl += ta1 * dy;
+      szl += ta1 * dz;
+    } else {
+      fx += (double)(Cd * dx + (ta2 * nj4.x - ta1 * nix));
+      fy += (double)(Cd * dy + (ta2 * nj4.y - ta1 * niy));
+      fz += (double)(Cd * dz + (ta2 * nj4.z - ta1 * niz));
+      sx += (double)(ta1 * dx);
+      sy += (double)(ta1 * dy);
+      sz += (double)(ta1 * dz);
+    }
     if (EF) {
       ev += 0.5 * (double)(Vvdw * tp.t);
       er += 0.5 * (double)(tp.t * V1) + 0.5 * (double)(tp.t * V2);
@@ -1047,6 +1071,10 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
         }
       }
     }
+  }
+  if (LANE32) {
+    fx += (double)fxl; fy += (double)fyl; fz += (double)fzl;
+    sx += (double)sxl; sy += (double)syl; sz += (double)szl;
   }
   fx = wsum(fx); fy = wsum(fy); fz = wsum(fz);
   sx = wsum(sx); sy = wsum(sy); sz = wsum(sz);
