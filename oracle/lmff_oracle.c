@@ -20,6 +20,9 @@
  *                        1436 (v_tally3), 1555-1604 (virial_fdotr_compute);
  *                        LMFF/pair_hybrid.cpp:93-210 (global virial by F.r)
  *   - integrator:        LMFF/fix_nve.cpp:51-143, LMFF/domain.cpp:501 (pbc wrap)
+ *   - the fused LMFF CPU path (orc_compute_lmff, the CPU baseline's timed path):
+ *                        LMFF/pair_lmff.cpp:1121-1613 without the quirks of its recalc,
+ *                        filtered short list and list-length-scaled normal
  *
  * There are no ghost atoms: every list entry carries the periodic image
  * shift s of its atom, and "x[j]" of a ghost is formed exactly as LAMMPS
@@ -1185,6 +1188,230 @@ int orc_compute(Orc *o, int eflag, int vflag, double *f, double *eng, double *vi
   for (int t = 0; t < nt; t++) { free(acc[t].f); free(acc[t].eatom); free(acc[t].vatom); }
   free(acc);
   free(nm);
+  return rc;
+}
+
+/*
+ * The reference's fused LMFF CPU path (LMFF/pair_lmff.cpp:1121-1613, computeLMFFTwice) restated,
+ * for the CPU baseline: per owned atom one pass over its full list finds the Tersoff short
+ * neighbours and the normal neighbours (1254-1295, calc_normal once), a second pass does the
+ * Tersoff repulsive half pair and, for every other-molecule neighbour in the ILP cutoff, the
+ * van der Waals half pair (by tag parity) and the repulsion with the atom's own normal, pushing
+ * the pair force to both atoms (1320-1465); the normal-derivative forces are accumulated as
+ * dprodni = sum_j -prodnorm1 fpair1 Tap del (1440-1442) and applied once per atom through
+ * dnormal and dnormdri (1490-1519); then the Tersoff three-body terms (1523-1608).
+ * Not reproduced (reference quirks): the recalc double count, the unbounded filtered short list,
+ * the normal scaled by the full list length (calc_normal here is the stock one).
+ * Returns forces and, with eflag, eng = {total, vdW, repulsion, Tersoff} (no per-atom tallies,
+ * no virial: the baseline times the force path).  Same results as orc_compute up to rounding.
+ */
+static int lmff_fused_range(const Orc *o, Acc *a, int i0, int i1, int eflag, char *err, int errlen) {
+  const double cutshortsq = o->cutmax * o->cutmax;
+  int maxshort = 16;
+  int *neighshort = (int *)malloc(sizeof(int) * maxshort);
+  unsigned char *shortcode = (unsigned char *)malloc(maxshort);
+  for (int i = i0; i < i1; i++) {
+    const long long itag = o->tag[i];
+    const int it = o->type[i];
+    const int itype = o->t_map[it], itm = o->i_map[it];
+    const double *xi = &o->x[3 * i];
+    const double xtmp = xi[0], ytmp = xi[1], ztmp = xi[2];
+    /* pass 1: short neighbours and normal neighbours */
+    int numshort = 0, nilp = 0;
+    Normal nm;
+    double vet[3][3] = {{0}};
+    for (int e = o->offs[i]; e < o->offs[i + 1]; e++) {
+      const int j = o->ent_j[e], code = o->ent_s[e];
+      double xj[3];
+      ghost_x(o, j, code, xj);
+      const double delx = xtmp - xj[0], dely = ytmp - xj[1], delz = ztmp - xj[2];
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      const int jt = o->type[j];
+      if (itype >= 0 && o->t_map[jt] >= 0 && rsq < cutshortsq) {
+        if (numshort >= maxshort - 1) {
+          maxshort += maxshort / 2;
+          neighshort = (int *)realloc(neighshort, sizeof(int) * maxshort);
+          shortcode = (unsigned char *)realloc(shortcode, maxshort);
+        }
+        neighshort[numshort] = j;
+        shortcode[numshort] = (unsigned char)code;
+        numshort++;
+      }
+      if (itm >= 0 && o->i_map[jt] >= 0 && rsq != 0 && rsq < o->cutILPsq[itm][o->i_map[jt]] && o->mol[i] == o->mol[j]) {
+        if (nilp >= 3) {
+          snprintf(err, errlen, "There are too many neighbors for some atoms, please check your configuration");
+          free(neighshort); free(shortcode);
+          return -1;
+        }
+        nm.nb[nilp] = j;
+        nm.ns[nilp] = (unsigned char)code;
+        vet[nilp][0] = -delx; vet[nilp][1] = -dely; vet[nilp][2] = -delz;
+        nilp++;
+      }
+    }
+    nm.cont = nilp;
+    if (itm >= 0 && calc_normal_one(vet, nilp, &nm, err, errlen)) { free(neighshort); free(shortcode); return -1; }
+    /* pass 2: Tersoff repulsive half pairs and ILP, pushed to both atoms */
+    double fxtmp = 0, fytmp = 0, fztmp = 0;
+    double dprodni[3] = {0, 0, 0};
+    for (int e = o->offs[i]; e < o->offs[i + 1]; e++) {
+      const int j = o->ent_j[e], code = o->ent_s[e];
+      const int jt = o->type[j];
+      double xj[3];
+      ghost_x(o, j, code, xj);
+      const double delx = xtmp - xj[0], dely = ytmp - xj[1], delz = ztmp - xj[2];
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      const int owner = !tag_skip(itag, o->tag[j], xi, xj);
+      const double r = sqrt(rsq), rinv = 1.0 / r;
+      if (owner && itype >= 0 && o->t_map[jt] >= 0) {
+        const TParam *p = &o->tp[o->t_e2p[itype][o->t_map[jt]][o->t_map[jt]]];
+        if (rsq < p->cutsq) {
+          double fpair, evdwl = 0.0;
+          ters_repulsive(p, rsq, &fpair, eflag, &evdwl);
+          fxtmp += delx * fpair; fytmp += dely * fpair; fztmp += delz * fpair;
+          add_f(a, o, j, code, -delx * fpair, -dely * fpair, -delz * fpair, 0);
+          if (eflag) a->eng_ters += evdwl;
+        }
+      }
+      if (itm >= 0 && o->i_map[jt] >= 0 && rsq < o->ilp_cutsq[it][jt] && o->mol[i] != o->mol[j]) {
+        const IParam *p = &o->ip[o->i_e2p[itm][o->i_map[jt]]];
+        double Tap, dTap;
+        if (o->tap_flag) {
+          const double Rcut = sqrt(o->ilp_cutsq[it][jt]);
+          Tap = calc_Tap(r, Rcut);
+          dTap = calc_dTap(r, Rcut);
+        } else { Tap = 1.0; dTap = 0.0; }
+        double fvdw = 0.0;
+        if (owner) {
+          const double r2inv = rinv * rinv, r6inv = r2inv * r2inv * r2inv, r8inv = r6inv * r2inv;
+          const double TSvdw = 1.0 + exp(-p->d * (r / p->seff - 1.0));
+          const double TSvdw2inv = 1.0 / (TSvdw * TSvdw);
+          const double vvdw = -p->C6 * r6inv / TSvdw;
+          const double fp = -6.0 * p->C6 * r8inv / TSvdw + p->C6 * p->d / p->seff * (TSvdw - 1.0) * TSvdw2inv * r8inv * r;
+          fvdw = fp * Tap - vvdw * dTap * rinv;
+          if (eflag) a->eng_vdw += vvdw * Tap;
+        }
+        const double prodnorm1 = nm.normal[0] * delx + nm.normal[1] * dely + nm.normal[2] * delz;
+        const double rhosq1 = rsq - prodnorm1 * prodnorm1;
+        const double rdsq1 = rhosq1 * p->delta2inv;
+        const double exp0 = exp(-p->lambda * (r - p->z0));
+        const double exp1 = exp(-rdsq1);
+        const double frho1 = exp1 * p->C;
+        const double Erep = 0.5 * p->epsilon + frho1;
+        const double Vilp = exp0 * Erep;
+        const double fpair = p->lambda * exp0 * rinv * Erep;
+        const double fpair1 = 2.0 * exp0 * frho1 * p->delta2inv;
+        const double fsum = fpair + fpair1;
+        const double fkcx = (delx * fsum - prodnorm1 * nm.normal[0] * fpair1) * Tap - Vilp * dTap * delx * rinv;
+        const double fkcy = (dely * fsum - prodnorm1 * nm.normal[1] * fpair1) * Tap - Vilp * dTap * dely * rinv;
+        const double fkcz = (delz * fsum - prodnorm1 * nm.normal[2] * fpair1) * Tap - Vilp * dTap * delz * rinv;
+        dprodni[0] += -prodnorm1 * fpair1 * Tap * delx;
+        dprodni[1] += -prodnorm1 * fpair1 * Tap * dely;
+        dprodni[2] += -prodnorm1 * fpair1 * Tap * delz;
+        const double ftx = fvdw * delx + fkcx, fty = fvdw * dely + fkcy, ftz = fvdw * delz + fkcz;
+        fxtmp += ftx; fytmp += fty; fztmp += ftz;
+        add_f(a, o, j, code, -ftx, -fty, -ftz, 0);
+        if (eflag) a->eng_rep += Tap * Vilp;
+      }
+    }
+    /* normal-derivative forces: on the normal neighbours and (two-neighbour normals) on i */
+    if (itm >= 0) {
+      for (int kk = 0; kk < nm.cont; kk++) {
+        double fk[3];
+        for (int c = 0; c < 3; c++)
+          fk[c] = nm.dnormal[0][c][kk] * dprodni[0] + nm.dnormal[1][c][kk] * dprodni[1] + nm.dnormal[2][c][kk] * dprodni[2];
+        add_f(a, o, nm.nb[kk], nm.ns[kk], fk[0], fk[1], fk[2], 0);
+      }
+      fxtmp += nm.dnormdri[0][0] * dprodni[0] + nm.dnormdri[1][0] * dprodni[1] + nm.dnormdri[2][0] * dprodni[2];
+      fytmp += nm.dnormdri[0][1] * dprodni[0] + nm.dnormdri[1][1] * dprodni[1] + nm.dnormdri[2][1] * dprodni[2];
+      fztmp += nm.dnormdri[0][2] * dprodni[0] + nm.dnormdri[1][2] * dprodni[1] + nm.dnormdri[2][2] * dprodni[2];
+    }
+    /* Tersoff three-body terms of i's short neighbours */
+    for (int jj = 0; jj < numshort; jj++) {
+      const int j = neighshort[jj], jcode = shortcode[jj];
+      const int jt = o->t_map[o->type[j]];
+      const int iparam_ij = o->t_e2p[itype][jt][jt];
+      double xj[3];
+      ghost_x(o, j, jcode, xj);
+      const double delr1[3] = {xj[0] - xtmp, xj[1] - ytmp, xj[2] - ztmp};
+      const double rsq1 = delr1[0] * delr1[0] + delr1[1] * delr1[1] + delr1[2] * delr1[2];
+      if (rsq1 >= o->tp[iparam_ij].cutsq) continue;
+      double fjx = 0, fjy = 0, fjz = 0, zeta_ij = 0.0;
+      for (int kk = 0; kk < numshort; kk++) {
+        if (jj == kk) continue;
+        const int k = neighshort[kk];
+        const int iparam_ijk = o->t_e2p[itype][jt][o->t_map[o->type[k]]];
+        double xk[3];
+        ghost_x(o, k, shortcode[kk], xk);
+        const double delr2[3] = {xk[0] - xtmp, xk[1] - ytmp, xk[2] - ztmp};
+        const double rsq2 = delr2[0] * delr2[0] + delr2[1] * delr2[1] + delr2[2] * delr2[2];
+        if (rsq2 >= o->tp[iparam_ijk].cutsq) continue;
+        zeta_ij += ters_zeta(&o->tp[iparam_ijk], rsq1, rsq2, delr1, delr2);
+      }
+      double fpair, prefactor, evdwl = 0.0;
+      ters_force_zeta(&o->tp[iparam_ij], rsq1, zeta_ij, &fpair, &prefactor, eflag, &evdwl);
+      fxtmp += delr1[0] * fpair; fytmp += delr1[1] * fpair; fztmp += delr1[2] * fpair;
+      fjx -= delr1[0] * fpair; fjy -= delr1[1] * fpair; fjz -= delr1[2] * fpair;
+      if (eflag) a->eng_ters += evdwl;
+      for (int kk = 0; kk < numshort; kk++) {
+        if (jj == kk) continue;
+        const int k = neighshort[kk], kcode = shortcode[kk];
+        const int iparam_ijk = o->t_e2p[itype][jt][o->t_map[o->type[k]]];
+        double xk[3];
+        ghost_x(o, k, kcode, xk);
+        const double delr2[3] = {xk[0] - xtmp, xk[1] - ytmp, xk[2] - ztmp};
+        const double rsq2 = delr2[0] * delr2[0] + delr2[1] * delr2[1] + delr2[2] * delr2[2];
+        if (rsq2 >= o->tp[iparam_ijk].cutsq) continue;
+        double fi[3], fj[3], fk[3];
+        ters_attractive(&o->tp[iparam_ijk], prefactor, rsq1, rsq2, delr1, delr2, fi, fj, fk);
+        fxtmp += fi[0]; fytmp += fi[1]; fztmp += fi[2];
+        fjx += fj[0]; fjy += fj[1]; fjz += fj[2];
+        add_f(a, o, k, kcode, fk[0], fk[1], fk[2], 0);
+      }
+      add_f(a, o, j, jcode, fjx, fjy, fjz, 0);
+    }
+    add_f(a, o, i, 13, fxtmp, fytmp, fztmp, 0);
+  }
+  free(neighshort);
+  free(shortcode);
+  return 0;
+}
+
+int orc_compute_lmff(Orc *o, int eflag, double *f, double *eng) {
+  const int n = o->n;
+  const int nt = o->nthreads;
+  Acc *acc = (Acc *)calloc(nt, sizeof(Acc));
+  char errs[8][256];
+  memset(errs, 0, sizeof errs);
+  int fails = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(+ : fails)
+#endif
+  {
+#ifdef _OPENMP
+    int t = omp_get_thread_num();
+#else
+    int t = 0;
+#endif
+    Acc *a = &acc[t];
+    a->f = (double *)calloc(3 * (size_t)(n ? n : 1), sizeof(double));
+    int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
+    if (lmff_fused_range(o, a, i0, i1, eflag & 1, errs[t & 7], 256)) fails++;
+  }
+  int rc = 0;
+  if (fails) {
+    for (int t = 0; t < 8; t++) if (errs[t][0]) { snprintf(o->err, sizeof o->err, "%s", errs[t]); break; }
+    rc = -1;
+  } else {
+    double et = 0, ev = 0, er = 0;
+    for (int t = 0; t < nt; t++) {
+      for (long long k = 0; k < 3LL * n; k++) f[k] += acc[t].f[k];
+      et += acc[t].eng_ters; ev += acc[t].eng_vdw; er += acc[t].eng_rep;
+    }
+    if (eng) { eng[0] = et + ev + er; eng[1] = ev; eng[2] = er; eng[3] = et; }
+  }
+  for (int t = 0; t < nt; t++) free(acc[t].f);
+  free(acc);
   return rc;
 }
 

@@ -58,6 +58,7 @@ def lib():
         L.orc_list_size.argtypes = [C.c_void_p]
         L.orc_get_list.argtypes = [C.c_void_p, _i, _i, _i]
         L.orc_compute.argtypes = [C.c_void_p, C.c_int, C.c_int, _d, _d, _d, _d, _d]
+        L.orc_compute_lmff.argtypes = [C.c_void_p, C.c_int, _d, _d]
         L.orc_normals.argtypes = [C.c_void_p, _d, _i, _d, _d]
         L.orc_run_nve.argtypes = [C.c_void_p, _d, _d, _d, C.c_double, C.c_int, C.c_int, C.c_int, _d]
         L.orc_tap.restype = C.c_double
@@ -160,6 +161,14 @@ class Oracle:
                                       _p(vatom, C.c_double) if vatom is not None else null))
         return {"f": f, "energy": eng[0], "e_vdw": eng[1], "e_rep": eng[2], "e_tersoff": eng[3], "virial": vir,
                 "eatom": eatom, "vatom": vatom}
+
+    def compute_lmff(self, eflag: int = 1):
+        """The reference's fused LMFF CPU path restated (LMFF/pair_lmff.cpp:1121-1613): forces and
+        energies only (the CPU baseline's path); the same results as compute() up to rounding."""
+        f = np.zeros((self.n, 3))
+        eng = np.zeros(4)
+        self._check(lib().orc_compute_lmff(self.h, int(eflag), _p(f, C.c_double), _p(eng, C.c_double)))
+        return {"f": f, "energy": eng[0], "e_vdw": eng[1], "e_rep": eng[2], "e_tersoff": eng[3]}
 
     def normals(self):
         n = self.n

@@ -496,6 +496,50 @@ def test_patch_with_a_24_A_rim_reproduces_the_full_system(oracle_lib):
     assert short["rel_max_over_rms"] > 1e-4
 
 
+# ---------------------------------------------------------------- the fused LMFF CPU path
+@pytest.mark.parametrize("case", ["bilayer", "hetero_trilayer", "ribbons", "asymmetric", "null_types"])
+def test_lmff_fused_restatement_matches_the_stock_oracle(oracle_lib, case, tmp_path):
+    """orc_compute_lmff (the reference's fused LMFF CPU path, LMFF/pair_lmff.cpp:1121-1613: one
+    short/normal pass, one fused Tersoff-repulsive + ILP pass that pushes each pair force to both
+    atoms and applies the normal-derivative forces once per atom through dprodni) is the CPU
+    baseline's path.  It must give the stock oracle's forces and energies up to rounding, also
+    with two-neighbour normals, asymmetric parameter rows and types left out of one term."""
+    import asym_tables
+    import oracle
+
+    if case == "bilayer":
+        s = small(("G", "G"), seed=11)
+        o = make_oracle(s, threads=4)
+    elif case == "hetero_trilayer":
+        s = small(("G", "BN", "G"), seed=12)
+        o = make_oracle(s, threads=4)
+    elif case == "ribbons":
+        s = S.layered_stack(12, 9, ["G", "G"], seed=13, jitter=0.06)
+        s.boxhi[0] += 30.0
+        s.x[:, 0] += 5.0
+        o = make_oracle(s, threads=4)
+    elif case == "asymmetric":
+        s = small(("G", "BN"), seed=14)
+        tf, fi = asym_tables.write(tmp_path)
+        o = oracle.Oracle(3, tf, s.elements, fi, s.elements, threads=4)
+        o.set_system(s)
+        o.build_list()
+    else:
+        # types 1 and 2 of a C-only bilayer: Tersoff only for type 2, ILP for both
+        s = small(("G", "G"), seed=15)
+        s.type = np.where(s.mol == 1, 1, 2).astype(np.int32)
+        s.elements = ["C", "C"]
+        o = oracle.Oracle(2, CT, ["NULL", "C(O)"], ILP, ["C", "C"], threads=4)
+        o.set_system(s)
+        o.build_list()
+    r = o.compute(1, 0)
+    q = o.compute_lmff(1)
+    rms = np.sqrt((r["f"] ** 2).sum(1).mean())
+    assert np.abs(q["f"] - r["f"]).max() <= 1e-12 * rms
+    for k in ("energy", "e_vdw", "e_rep", "e_tersoff"):
+        assert q[k] == pytest.approx(r[k], rel=1e-12, abs=1e-12)
+
+
 # ---------------------------------------------------------------- per-atom virial split
 def test_peratom_virial_matches_an_autograd_restatement(oracle_lib):
     """The oracle's per-atom virial (every tally, incl. the split of the normal-derivative
