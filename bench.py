@@ -536,6 +536,10 @@ def _capacity_dict(packed, seq, direct):
             "speedup_vs_sequential_through_server": packed["atom_steps_per_s"] / seq["atom_steps_per_s"],
             "packed_makespan_s": packed["makespan_s"], "sequential_makespan_s": direct["makespan_s"],
             "packed_max_resident": packed["max_resident"],
+            "packed_gpu_busy_fraction": packed["gpu_busy_s"] / packed["makespan_s"],
+            "packed_ticks": packed["ticks"],
+            "packed_admit_s": packed["admit_s"],
+            "sequential_through_server_gpu_busy_fraction": seq["gpu_busy_s"] / seq["makespan_s"],
             "note": "sequential = each job alone through the plain single-system path (one lmff_run call "
                     "per job); the server-code sequential run (max one resident job, a host round trip "
                     "every tick) is given beside it"}

@@ -156,6 +156,8 @@ class Server:
         t0 = clock()
         finished = []
         atom_steps = 0
+        gpu_s = 0.0      # device time of the ticks (HIP events around each lmff_run)
+        admit_s = 0.0    # host time spent admitting (staging + the re-pack at the next tick)
         pi = 0
         while pi < len(pending) or queue or self.resident:
             now = (clock() - t0) / time_scale
@@ -184,7 +186,9 @@ class Server:
                 if self.reserved + need > self.budget and self.resident:
                     break
                 queue.pop(0)
+                ta = time.perf_counter()
                 self._admit(job, now)
+                admit_s += time.perf_counter() - ta
             if not self.resident:
                 if pi < len(pending):
                     wait = pending[pi].arrival - now
@@ -194,6 +198,7 @@ class Server:
             nsys = len(self.resident)
             order = sorted(self.resident)  # packed order = admission order = sys id order
             rec = self.ctx.run(self.tick, self.tick, use_graph=True, nsys=nsys)
+            gpu_s += self.ctx.profile_read()[0] * 1e-3
             for k, sid in enumerate(order):
                 job = self.resident[sid]
                 job.done_steps += self.tick
@@ -213,6 +218,9 @@ class Server:
             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else float("nan"),
             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else float("nan"),
             "max_resident": max((n for _, n, _ in self.log), default=0),
+            "ticks": len(self.log),
+            "gpu_busy_s": gpu_s,
+            "admit_s": admit_s,
         }
 
 
