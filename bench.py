@@ -64,15 +64,20 @@ def one_shot(s, precision, repeat=1):
     try:
         capi.setup_for(ctx, s, precision=precision)
         r = ctx.pair_compute(s, 1, 2)
-        walls = {1: [], 0: []}
+        walls = {1: [], 0: [], "f": []}
         for nl in (1, 0):
             for _ in range(repeat - 1):
                 t0 = time.perf_counter()
                 ctx.pair_compute(s, 1, 2, new_list=nl)
                 walls[nl].append(time.perf_counter() - t0)
+        for _ in range(repeat - 1):
+            t0 = time.perf_counter()
+            ctx.pair_compute(s, 0, 0, new_list=0)
+            walls["f"].append(time.perf_counter() - t0)
         if walls[1]:
             r["wall_s"] = float(np.median(walls[1]))
             r["wall_reuse_s"] = float(np.median(walls[0]))
+            r["wall_reuse_force_s"] = float(np.median(walls["f"]))
         return r
     finally:
         ctx.close()
@@ -137,6 +142,7 @@ def cpu_baseline(s, seconds_budget=20.0):
     f_ref = None
     steps = 0
     t_force = 0.0
+    o.compute_lmff(1)  # untimed: maps the per-thread force arrays
     while t_force < seconds_budget or steps == 0:
         t1 = time.perf_counter()
         o.compute_lmff(1)
@@ -190,8 +196,10 @@ def patch_baseline(s, f_gpu, seed, frac=0.01, halo=24.0):
     t0 = time.perf_counter()
     o.build_list()
     t_list = time.perf_counter() - t0
-    # the timed path: the reference's fused LMFF CPU path restated (orc_compute_lmff); the stock
-    # four-pass oracle is the checker and is timed beside it
+    # the timed path: the reference's fused LMFF CPU path restated (orc_compute_lmff), timed on
+    # its second call (the first maps the per-thread force arrays); the stock four-pass oracle is
+    # the checker and is timed beside it
+    o.compute_lmff(1)
     t1 = time.perf_counter()
     o.compute_lmff(1)
     t_force = time.perf_counter() - t1
@@ -444,6 +452,7 @@ def bilayer_bench(args):
                     "x/type/tag/mol, list build, forces, download f) and reusing it (upload x, forces, "
                     "download f); per step of a LAMMPS loop that rebuilds every 10 steps; PCIe included",
             "wall_ms": r64["wall_s"] * 1e3, "wall_ms_reuse": r64["wall_reuse_s"] * 1e3,
+            "wall_ms_reuse_forces_only": r64["wall_reuse_force_s"] * 1e3,
             "ms_per_step_every10": (r64["wall_s"] + 9 * r64["wall_reuse_s"]) / 10 * 1e3,
             "atom_steps_per_s_every10": n / ((r64["wall_s"] + 9 * r64["wall_reuse_s"]) / 10)},
     }

@@ -2180,28 +2180,48 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
                          ps->hi[2] - ps->lo[2], c.hb_in.as<double>(), c.x.as<double4>() + ps->begin);
       c.far_check();
     }
-    c.evaluate(eflag, vflag);
-    HIPCHK(hipStreamSynchronize(c.stream));
+    c.evaluate(eflag, vflag);  // ends with the error check, a synchronisation
     SysHost *s = c.find(c.pair_sys);
     int k = 0;
     for (size_t q = 0; q < c.sys.size(); q++)
       if (c.sys[q].id == c.pair_sys) k = (int)q;
     const size_t m = (size_t)(s->end - s->begin);
-    if (f && m > 0) {
-      // forces gathered into caller order on the device, one copy into page-locked memory
-      c.hb_out.ensure(m * 3 * sizeof(double));
-      c.hb_host.ensure(m * 3 * sizeof(double));
+    const size_t total = (f && m > 0) ? 3 * m : 0;
+    // page-locked landing area: the forces, then the nine global sums
+    c.hb_host.ensure((total + 16) * sizeof(double));
+    double *hpin = (double *)c.hb_host.p;
+    double *h9 = hpin + total;
+    const bool want_sums = eflag || (vflag % 4);
+    if (want_sums)
+      HIPCHK(hipMemcpyAsync(h9, c.sums.as<double>() + (size_t)k * 9, 9 * sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream));
+    if (total) {
+      // forces gathered into caller order on the device and brought down in chunks; the host adds
+      // each chunk into the caller's array while the next one is in flight
+      c.hb_out.ensure(total * sizeof(double));
       hipLaunchKernelGGL(k_hb_gather_f, dim3((int)((m + BS - 1) / BS)), dim3(BS), 0, c.stream, (int)m,
                          c.orig.as<int>() + s->begin, c.f.as<double4>() + s->begin, c.hb_out.as<double>());
-      HIPCHK(hipMemcpyAsync(c.hb_host.p, c.hb_out.p, m * 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      HIPCHK(hipStreamSynchronize(c.stream));
-      const double *h = (const double *)c.hb_host.p;
-      for (size_t q = 0; q < 3 * m; q++) f[q] += h[q];
+      const int nchunk = total > ((size_t)1 << 18) ? 4 : 1;
+      std::vector<hipEvent_t> done(nchunk, nullptr);
+      for (int q = 0; q < nchunk; q++) {
+        const size_t b = total * q / nchunk, e = total * (q + 1) / nchunk;
+        HIPCHK(hipMemcpyAsync(hpin + b, c.hb_out.as<double>() + b, (e - b) * sizeof(double), hipMemcpyDeviceToHost,
+                              c.stream));
+        HIPCHK(hipEventCreateWithFlags(&done[q], hipEventDisableTiming));
+        HIPCHK(hipEventRecord(done[q], c.stream));
+      }
+      for (int q = 0; q < nchunk; q++) {
+        const size_t b = total * q / nchunk, e = total * (q + 1) / nchunk;
+        HIPCHK(hipEventSynchronize(done[q]));
+        for (size_t t = b; t < e; t++) f[t] += hpin[t];
+      }
+      for (auto ev : done) (void)hipEventDestroy(ev);
     }
+    HIPCHK(hipStreamSynchronize(c.stream));
     std::vector<int> org;
     if ((eatom && (eflag & 2)) || (vatom && (vflag & 4))) org = c.download<int>(c.orig, s->begin, m);
-    double h9[9] = {0};
-    if (eflag || (vflag % 4)) HIPCHK(hipMemcpy(h9, c.sums.as<double>() + (size_t)k * 9, sizeof h9, hipMemcpyDeviceToHost));
+    if (!want_sums)
+      for (int q = 0; q < 9; q++) h9[q] = 0.0;
     if (eng_vdwl) *eng_vdwl = (eflag & 1) ? h9[0] + h9[1] + h9[2] : 0.0;
     if (pvector) {
       pvector[0] = (eflag & 1) ? h9[0] : 0.0;
