@@ -41,6 +41,10 @@ class FakeCtx:
         self.runs.append(sorted(self.systems))
         return [[{"e_vdw": -1.0, "e_rep": 0.5, "pe": -7.0, "temp": 300.0}] for _ in range(nsys)]
 
+    def profile_read(self):
+        # device time of the last run (ms), per-kernel-family times and counts
+        return 0.0, np.zeros(7), np.zeros(7, dtype=np.int64)
+
     def close(self):
         pass
 
