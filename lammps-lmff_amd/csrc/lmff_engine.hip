@@ -123,6 +123,7 @@ struct SysHost {
   int nc[3];
   double glo[3], bs[3];
   int cell_off, ncells;
+  int row_off = 0;  // first of this system's rows of cells (fixed y and z bin)
   int begin, end;
   int style = LMFF_NVE;
   double dt = 0.001, t_target = 300.0, damp = 1.0;
@@ -156,6 +157,7 @@ struct Ctx {
   int next_id = 0;
   int64_t n = 0;             // total atoms
   int ncells = 0;
+  int nrows_cells = 0;  // rows of cells (fixed y and z bin) over all systems
   bool layout_dirty = true;  // packed arrays must be rebuilt from host staging
   bool need_rebuild = true;
   bool forces_valid = false;
@@ -179,7 +181,7 @@ struct Ctx {
   DBuf x, v, f, xhold, mol, tag, img, sysid, orig, key, key2, val, perm, tmp4, tmpi;
   DBuf slist, scnt, ilist, icnt, ocnt;
   DBuf nrm, dn, dndri, vet, nnb, tsh, tcnt, rtc, rtcnt, rnc, rncnt, eb, fb, wb, ft, fil, S, T, th, vatom;
-  DBuf cell_start, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
+  DBuf cell_start, row_mol, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
   DBuf hb_in, hb_out;  // host-buffer Pair::compute: caller-order arrays on the device
   PinnedBuf hb_host;   // and the forces on their way back
@@ -227,7 +229,7 @@ struct Ctx {
     DBuf *all[] = {&x, &v, &f, &xhold, &mol, &tag, &img, &sysid, &orig, &key, &key2, &val, &perm, &tmp4, &tmpi,
                    &slist, &scnt, &ilist, &icnt, &ocnt, &nrm, &dn, &dndri, &vet, &nnb, &tsh, &tcnt, &rtc, &rtcnt,
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
-                   &fil, &S, &T, &th, &vatom, &cell_start, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
+                   &fil, &S, &T, &th, &vatom, &cell_start, &row_mol, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
                    &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out};
     for (DBuf *b : all) b->tally = &bytes_in_use;
@@ -384,13 +386,15 @@ struct Ctx {
       src_stage.push_back(&st);
     }
     int64_t nn = 0;
-    int nc = 0;
+    int nc = 0, nr = 0;
     for (auto &s : next) {
       s.begin = (int)nn;
       nn += s.n;
       s.end = (int)nn;
       s.cell_off = nc;
       nc += s.ncells;
+      s.row_off = nr;
+      nr += s.nc[1] * s.nc[2];
     }
     if (nn >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)nn);
     const size_t m1 = std::max<size_t>((size_t)nn, 1);
@@ -456,6 +460,7 @@ struct Ctx {
     sys = next;
     n = nn;
     ncells = nc;
+    nrows_cells = nr;
     alloc_atoms((size_t)n);
     if (n) HIPCHK(hipMemcpyAsync(f.p, fsave.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     restore_f = had_new ? keep_f : std::vector<std::pair<int, int>>{};
@@ -476,6 +481,7 @@ struct Ctx {
     if (!ch.empty()) HIPCHK(hipMemcpy(chunks.p, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(cbeg.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
     cell_start.ensure(sizeof(int) * (size_t)(ncells + 1));
+    row_mol.ensure(sizeof(int2) * (size_t)std::max(nrows_cells, 1));
     end_bit = 1;
     while ((1ll << end_bit) <= (long long)ncells) end_bit++;
     size_t tb = 0;
@@ -563,6 +569,7 @@ struct Ctx {
         d.inv_bs[a] = 1.0 / s.bs[a];
       }
       d.cell_off = s.cell_off;
+      d.row_off = s.row_off;
       d.begin = s.begin;
       d.end = s.end;
       d.dtv = s.dt;
@@ -626,6 +633,7 @@ struct Ctx {
     P.rcut = std::sqrt(ilp_cut * ilp_cut);
     P.inv_rcut = 1.0 / P.rcut;
     P.tap_flag = tap_flag;
+    P.ntypes = ntypes;
     P.ters_rep_sym = ters_rep_sym ? 1 : 0;
     P.rmass = rmass.as<double>();
     {
@@ -703,6 +711,9 @@ struct Ctx {
     }
     hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
                        key2.as<uint32_t>(), cell_start.as<int>());
+    hipLaunchKernelGGL(k_row_mol, dim3((nrows_cells + 3) / 4), dim3(BS), 0, stream, nrows_cells, (int)sys.size(),
+                       (const SysDev *)systab.as<SysDev>(), (const int *)cell_start.as<int>(), (const int *)mol.as<int>(),
+                       row_mol.as<int2>());
     HIPCHK(hipMemcpyAsync(xhold.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 1, farflag.as<int>(), 0);
     build_lists();
@@ -730,7 +741,8 @@ struct Ctx {
       HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int) * 4, stream));
       auto P = ptrs<double>();
       const int nb = (int)((n + 3) / 4);
-      hipLaunchKernelGGL(k_build_lists<double>, dim3(nb), dim3(BS), 0, stream, P, (const int *)cell_start.as<int>());
+      hipLaunchKernelGGL(k_build_lists<double>, dim3(nb), dim3(BS), 0, stream, P, (const int *)cell_start.as<int>(),
+                         (const int2 *)row_mol.as<int2>());
       int o[4];
       HIPCHK(hipMemcpyAsync(o, ovf.p, sizeof o, hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -1436,6 +1448,7 @@ struct Ctx {
     uint32_t *key_d = cv.take<uint32_t>(M, "keys"), *key2_d = cv.take<uint32_t>(M, "keys");
     int *val_d = cv.take<int>(M, "sort values"), *perm_d = cv.take<int>(M, "sort values");
     int *cstart_d = cv.take<int>((size_t)g.ncells + 1, "cell ranges");
+    int2 *rmol_d = cv.take<int2>((size_t)g.nc[1] * g.nc[2], "row molecule ranges");
     void *sorttmp_d = cv.take<unsigned char>(tb, "sort");
     DevPtrs<double> P{};
     P.n = m;
@@ -1516,7 +1529,10 @@ struct Ctx {
                          origb);
       hipLaunchKernelGGL(k_cell_start, dim3((m + 1 + BS - 1) / BS), dim3(BS), 0, stream, m, g.ncells,
                          (const uint32_t *)key2_d, cstart_d);
-      hipLaunchKernelGGL(k_build_lists<double>, dim3((m + 3) / 4), dim3(BS), 0, stream, P, (const int *)cstart_d);
+      hipLaunchKernelGGL(k_row_mol, dim3((g.nc[1] * g.nc[2] + 3) / 4), dim3(BS), 0, stream, g.nc[1] * g.nc[2], 1,
+                         (const SysDev *)systab_d, (const int *)cstart_d, (const int *)molb, rmol_d);
+      hipLaunchKernelGGL(k_build_lists<double>, dim3((m + 3) / 4), dim3(BS), 0, stream, P, (const int *)cstart_d,
+                         (const int2 *)rmol_d);
       // lists must be whole before any force is added into the caller's array
       HIPCHK(hipMemcpyAsync(note_host, word_d, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -1609,7 +1625,11 @@ void check_box(const double lo[3], const double hi[3], const int per[3], double 
 // rounding).  A non-periodic side spans the atoms' extent, or the box bounds when x is null
 // (atoms beyond the bins fall into the edge bins, which reach to infinity).
 void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
-  const double target[3] = {cutneigh / 6.0, cutneigh / 3.0 * (1.0 + 1e-6), cutneigh / 3.0 * (1.0 + 1e-6)};
+  // bins of at least a sixth of the list cutoff in x and z and a third in y (the list build's
+  // stencil then has at most 7 rows in y and 13 in a periodic z); a non-periodic z (the normal
+  // of a stack) splits the atoms' extent into bins of at most a sixth instead, so that sheets
+  // 3.35 A apart fall into bins of their own (at most 15 rows: the extent over such bins)
+  const double target[3] = {cutneigh / 6.0, cutneigh / 3.0 * (1.0 + 1e-6), cutneigh / 6.0 * (1.0 + 1e-6)};
   s.ncells = 1;
   for (int d = 0; d < 3; d++) {
     const double half = target[d];
@@ -1631,10 +1651,15 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
         if (n == 0) mn = mx = 0.0;
       }
       const double ext = mx - mn;
-      s.nc[d] = std::max(1, (int)(ext / half));
       s.glo[d] = mn;
-      s.bs[d] = ext > 0 ? ext / s.nc[d] : half;
-      if (s.bs[d] < half) s.bs[d] = half;
+      if (d == 2 && ext >= half) {
+        s.nc[d] = (int)std::ceil(ext / half);  // bins in [half / 2, half]
+        s.bs[d] = ext / s.nc[d];
+      } else {
+        s.nc[d] = std::max(1, (int)(ext / half));
+        s.bs[d] = ext > 0 ? ext / s.nc[d] : half;
+        if (s.bs[d] < half) s.bs[d] = half;
+      }
     }
   }
   const int64_t cells = (int64_t)s.nc[0] * s.nc[1] * s.nc[2];

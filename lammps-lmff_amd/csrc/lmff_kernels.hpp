@@ -59,6 +59,7 @@ struct SysDev {
   int nc[3];
   double glo[3], inv_bs[3];
   int cell_off;
+  int row_off;        // first row of cells (fixed y and z bin) of this system
   int begin, end;
   double dtv, dtf;
   int style;          // 0 = NVE, 1 = Langevin
@@ -124,6 +125,7 @@ struct DevPtrs {
   double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
   int tap_flag;
+  int ntypes;             // atom types 1..ntypes
   int ters_rep_sym;       // repulsive Tersoff parameters equal in both orders of every type pair
   // Tap(r) = 1 + r^4 (c4 + r (c5 + r (c6 + r c7))) and dTap/dr = d7 r^3 (r - rcut)^3:
   // {c4, c5, c6, c7, d7}
@@ -404,20 +406,70 @@ __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start)
   for (int c = lo; c <= hi; c++) start[c] = q;
 }
 
+// Molecule range {min, max} of the atoms of each row of cells (one system, fixed y and z bin;
+// {INT_MAX, INT_MIN} when the row is empty), one wave per row.
+__global__ void k_row_mol(int nrows, int nsys, const SysDev *st, const int *cell_start, const int *mol, int2 *rm) {
+  const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nrows) return;
+  // the last system whose rows start at or before r (every system has at least one row)
+  int lo_s = 0, hi_s = nsys - 1;
+  while (lo_s < hi_s) {
+    const int mid = (lo_s + hi_s + 1) >> 1;
+    if (st[mid].row_off <= r) lo_s = mid;
+    else hi_s = mid - 1;
+  }
+  const SysDev &s = st[lo_s];
+  const int base = s.cell_off + (r - s.row_off) * s.nc[0];
+  const int a0 = cell_start[base], a1 = cell_start[base + s.nc[0]];
+  int lo = 0x7fffffff, hi = (int)0x80000000;
+  for (int q = a0 + lane; q < a1; q += 64) {
+    const int m = mol[q];
+    lo = min(lo, m);
+    hi = max(hi, m);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if (lane == 0) rm[r] = make_int2(lo, hi);
+}
+
+// unwrapped bin u of a periodic dimension of nc bins -> its bin in [0, nc) and the image shift
+// (u - bin) / nc; u is within a few bins of the grid, so one add or subtract does unless the
+// dimension has very few bins (no integer division on the common path)
+__device__ __forceinline__ int wrap_bin(int u, int nc, int &shift) {
+  int w = u, sh = 0;
+  if (w < 0) { w += nc; sh = -1; }
+  else if (w >= nc) { w -= nc; sh = 1; }
+  if (w < 0 || w >= nc) {
+    w = ((u % nc) + nc) % nc;
+    sh = (u - w) / nc;
+  }
+  shift = sh;
+  return w;
+}
+
 // ---------------------------------------------------------------------------------------
 // Neighbor build (npair_full_bin_ghost.cpp:35-155 semantics: every image of every atom with
 // rsq <= (cut+skin)^2, evaluated in the reference's operation order with no fused
-// multiply-adds, 97-102).  One wavefront per atom.  Bins are long in y and z (at least a
-// third of the list cutoff, so at most 7 x 7 rows) and short in x.  Lane l takes stencil row
-// l = (dy, dz): it drops the row when the row's nearest point is beyond the cutoff, else
-// keeps only the x bins the cutoff sphere reaches on that row, as at most three runs of
-// consecutive sorted atoms (one per periodic image in x).  The wave lays the runs end to end
-// in LDS and then tests the candidates 64 at a time across run boundaries, so no lane idles
-// at the end of a short row.  Bins in a non-periodic dimension hold every atom beyond the
-// grid in the edge bin, so edge rows and bins reach to infinity.
+// multiply-adds, 97-102).  One wavefront per atom.  Bins are at least a third of the list
+// cutoff in y (at most 7 rows) and at least a sixth in x and z (in a non-periodic z the atoms'
+// extent is split into bins of at most a sixth, so that the sheets of a stack fall into bins of
+// their own).  Lane l takes stencil rows l and l + 64 of the (dy, dz) rows the grid holds (at
+// most 7 x 15): it drops a row whose nearest point is beyond the cutoff, else keeps the x bins
+// the cutoff sphere reaches on it, only those within the short-list reach
+// when the whole row of bins holds only atoms of i's own molecule (no entry of the ILP list
+// can come from it), and keeps the rest as at most three runs of consecutive sorted atoms (one per periodic image in x).
+// The wave lays the runs end to end in LDS and then tests the candidates 64 at a time across
+// run boundaries, so no lane idles at the end of a short row.  Bins in a non-periodic
+// dimension hold every atom beyond the grid in the edge bin, so edge rows and bins reach to
+// infinity.
 
-constexpr int RUNS_PER_LANE = 3;
-constexpr int RUN_SLOTS = 64 * RUNS_PER_LANE;
+constexpr int RUNS_PER_ROW = 3;
+constexpr int ROWS_PER_LANE = 2;
+constexpr int RUN_SLOTS = 64 * RUNS_PER_ROW * ROWS_PER_LANE;
 
 // nearest distance from coordinate p to the extent of unwrapped bin u of one dimension
 __device__ __forceinline__ double bin_gap(double p, int u, int nc, bool per, double glo, double bs) {
@@ -427,10 +479,12 @@ __device__ __forceinline__ double bin_gap(double p, int u, int nc, bool per, dou
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *cell_start) {
-  __shared__ int run_beg[4][RUN_SLOTS];
+__global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *cell_start, const int2 *row_mol) {
+  // run k of the wave: its first sorted atom with the image shift of its bins in the shift bits
+  // of a list entry (so a candidate's entry is that word plus its position in the run), and the
+  // offset of its first candidate in the wave's candidate sequence
+  __shared__ uint32_t run_word[4][RUN_SLOTS];
   __shared__ int run_off[4][RUN_SLOTS + 1];
-  __shared__ int run_sh[4][RUN_SLOTS];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + wv);
   const int lane = threadIdx.x & 63;
@@ -455,79 +509,95 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   const double cut = sqrt(P.cutneighsq);
   const double bsy = 1.0 / s.inv_bs[1], bsz = 1.0 / s.inv_bs[2];
   const int ry = (int)ceil(cut * s.inv_bs[1]), rz = (int)ceil(cut * s.inv_bs[2]);
-  const int wy = 2 * ry + 1;
-  // ---- lane l: stencil row (dy, dz) -> up to three runs ----
-  int nrun = 0, rb[RUNS_PER_LANE], rc_[RUNS_PER_LANE], rs[RUNS_PER_LANE];
-  if (wy * (2 * rz + 1) > 64) {
+  // the stencil rows the grid holds: in a non-periodic dimension only the bins that exist
+  const int dylo = s.per[1] ? -ry : max(-ry, -c[1]), dyhi = s.per[1] ? ry : min(ry, s.nc[1] - 1 - c[1]);
+  const int dzlo = s.per[2] ? -rz : max(-rz, -c[2]), dzhi = s.per[2] ? rz : min(rz, s.nc[2] - 1 - c[2]);
+  const int wy = dyhi - dylo + 1, nrows = wy * (dzhi - dzlo + 1);
+  if (nrows > 64 * ROWS_PER_LANE) {
     if (lane == 0) atomicOr(P.err, EB_GRID);
     return;
   }
-  if (lane < wy * (2 * rz + 1)) {
-    const int dy = lane % wy - ry, dz = lane / wy - rz;
-    int uy = c[1] + dy, uz = c[2] + dz;
-    bool ok = true;
-    if (!s.per[1] && (uy < 0 || uy >= s.nc[1])) ok = false;
-    if (!s.per[2] && (uz < 0 || uz >= s.nc[2])) ok = false;
-    if (ok) {
+  // ---- lane l: stencil rows l, l + 64 -> up to three runs each, laid end to end ----
+  int total = 0, nruns = 0;  // candidates and runs laid out so far (wave-uniform)
+  const float inv_wy = 1.0f / (float)wy;
+  for (int it = 0; it < nrows; it += 64) {
+    const int row = it + lane;
+    int nrun = 0, rb[RUNS_PER_ROW], rc_[RUNS_PER_ROW];
+    uint32_t rs[RUNS_PER_ROW];
+    if (row < nrows) {
+      // row = qz * wy + qy (row < 128, wy <= 13: the float quotient is exact after truncation)
+      const int qz = (int)(((float)row + 0.5f) * inv_wy), qy = row - qz * wy;
+      int uy = c[1] + dylo + qy, uz = c[2] + dzlo + qz;
       const double gy = fmax(0.0, bin_gap(pos[1], uy, s.nc[1], s.per[1], s.glo[1], bsy) - slack);
       const double gz = fmax(0.0, bin_gap(pos[2], uz, s.nc[2], s.per[2], s.glo[2], bsz) - slack);
       const double d2 = gy * gy + gz * gz;
       if (d2 <= P.cutneighsq) {
         int sy = 0, sz = 0;
-        if (s.per[1]) { const int w = ((uy % s.nc[1]) + s.nc[1]) % s.nc[1]; sy = (uy - w) / s.nc[1]; uy = w; }
-        if (s.per[2]) { const int w = ((uz % s.nc[2]) + s.nc[2]) % s.nc[2]; sz = (uz - w) / s.nc[2]; uz = w; }
+        if (s.per[1]) uy = wrap_bin(uy, s.nc[1], sy);
+        if (s.per[2]) uz = wrap_bin(uz, s.nc[2], sz);
         const double w = sqrt(P.cutneighsq - d2) + slack;
         // bins in reach along x; in a periodic dimension at most nc + 2 of them (w < L/2), so
         // at most three images; in a non-periodic one the edge bins hold what lies beyond
         const int rx = (int)ceil(cut * s.inv_bs[0]) + 1;
         int xlo = (int)fmax(floor((pos[0] - w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] - rx));
         int xhi = (int)fmin(floor((pos[0] + w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] + rx));
-        if (!s.per[0]) {
+        const int2 rm = row_mol[s.row_off + uz * s.nc[1] + uy];
+        if (rm.x > rm.y) {
+          xhi = xlo - 1;  // no atoms in this row of bins
+        } else if (rm.x == mi && rm.y == mi) {
+          // only atoms of i's own molecule: just the short lists can take entries from this row
+          if (d2 <= P.shortsq) {
+            const double ws = sqrt(P.shortsq - d2) + slack;
+            xlo = max(xlo, (int)floor((pos[0] - ws - s.glo[0]) * s.inv_bs[0]));
+            xhi = min(xhi, (int)floor((pos[0] + ws - s.glo[0]) * s.inv_bs[0]));
+          } else {
+            xhi = xlo - 1;
+          }
+        }
+        if (!s.per[0] && xlo <= xhi) {
           xlo = min(max(xlo, 0), s.nc[0] - 1);
           xhi = min(max(xhi, 0), s.nc[0] - 1);
         }
         const int rowbase = s.cell_off + (uz * s.nc[1] + uy) * s.nc[0];
         int u = xlo;
-        while (u <= xhi && nrun < RUNS_PER_LANE) {
+        while (u <= xhi && nrun < RUNS_PER_ROW) {
           int sx = 0, wx = u, uend = xhi;
           if (s.per[0]) {
-            wx = ((u % s.nc[0]) + s.nc[0]) % s.nc[0];
-            sx = (u - wx) / s.nc[0];
+            wx = wrap_bin(u, s.nc[0], sx);
             uend = min(xhi, u + (s.nc[0] - 1 - wx));
           }
           const int a0 = cell_start[rowbase + wx], a1 = cell_start[rowbase + wx + (uend - u) + 1];
           rb[nrun] = a0;
           rc_[nrun] = a1 - a0;
-          rs[nrun] = (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4);
+          rs[nrun] = (uint32_t)((sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4)) << IDX_BITS;
           nrun++;
           u = uend + 1;
         }
       }
     }
-  }
-  // ---- lay the runs end to end: wave prefix sums of candidate and run counts ----
-  int mycnt = 0, mynr = 0;
-  for (int k = 0; k < nrun; k++)
-    if (rc_[k] > 0) { mycnt += rc_[k]; mynr++; }
-  int scnt_ = mycnt, snr = mynr;
+    // wave prefix sums of candidate and run counts place this pass's runs after the earlier ones
+    int mycnt = 0, mynr = 0;
+    for (int k = 0; k < nrun; k++)
+      if (rc_[k] > 0) { mycnt += rc_[k]; mynr++; }
+    int scnt_ = mycnt, snr = mynr;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int a = __shfl_up(scnt_, o, 64), b = __shfl_up(snr, o, 64);
-    if (lane >= o) { scnt_ += a; snr += b; }
-  }
-  const int total = __shfl(scnt_, 63, 64), nruns = __shfl(snr, 63, 64);
-  {
-    int off = scnt_ - mycnt, slot = snr - mynr;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int a = __shfl_up(scnt_, o, 64), b = __shfl_up(snr, o, 64);
+      if (lane >= o) { scnt_ += a; snr += b; }
+    }
+    int off = total + scnt_ - mycnt, slot = nruns + snr - mynr;
     for (int k = 0; k < nrun; k++) {
       if (rc_[k] <= 0) continue;
-      run_beg[wv][slot] = rb[k];
+      run_word[wv][slot] = (uint32_t)rb[k] | rs[k];
       run_off[wv][slot] = off;
-      run_sh[wv][slot] = rs[k];
       off += rc_[k];
       slot++;
     }
-    if (lane == 0) run_off[wv][nruns] = total;
+    // wave-uniform, so the candidate loop below is a scalar loop
+    total += __builtin_amdgcn_readfirstlane(__shfl(scnt_, 63, 64));
+    nruns += __builtin_amdgcn_readfirstlane(__shfl(snr, 63, 64));
   }
+  if (lane == 0) run_off[wv][nruns] = total;
   // the table is read by other lanes of this wave only
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
@@ -541,34 +611,35 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // box lengths in registers (the list stores below could alias the system table as far as
   // the compiler knows, which would reload them for every chunk)
   const double prd0 = s.prd[0], prd1 = s.prd[1], prd2 = s.prd[2];
+  // every type i can meet has the ILP with ti: the per-candidate type test is moot
+  const bool every_tj = (ilp_mask | ~((1u << P.ntypes) - 1u)) == 0xffffffffu;
   int r = 0;
   for (int v0 = 0; v0 < total; v0 += 64) {
-    const int v = v0 + lane;
-    bool in_i = false, in_o = false, in_s = false, in_sb = false;
-    uint32_t ent = 0;
-    if (v < total) {
-      while (run_off[wv][r + 1] <= v) r++;
-      const int j = run_beg[wv][r] + (v - run_off[wv][r]);
-      const int sh = run_sh[wv][r];
-      const int sx = (sh & 3) - 1, sy = ((sh >> 2) & 3) - 1, sz = ((sh >> 4) & 3) - 1;
-      ent = e_make(j, sx, sy, sz);
-      if (!(sh == 0x15 && j == i)) {
-        const double4 xj = P.x[j];
-        const int mj = P.mol[j];  // issued with the position, not after the distance test
-        const double gx = __dadd_rn(xj.x, (double)sx * prd0);
-        const double gy = __dadd_rn(xj.y, (double)sy * prd1);
-        const double gz = __dadd_rn(xj.z, (double)sz * prd2);
-        const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
-        const double rsq = rsq_ref(delx, dely, delz);
-        const int tj = (int)xj.w - 1;
-        const bool ilp = (rsq <= P.cutneighsq) && (mj != mi) && ((ilp_mask >> tj) & 1u);
-        in_i = ilp && rsq < P.ilp_splitsq;
-        in_o = ilp && !(rsq < P.ilp_splitsq);
-        in_s = (rsq <= P.shortsq) && rsq < P.short_splitsq;
-        in_sb = (rsq <= P.shortsq) && !(rsq < P.short_splitsq);
-      }
-    }
-    const unsigned long long bi = __ballot(in_i), bo = __ballot(in_o), bs = __ballot(in_s), bsb = __ballot(in_sb);
+    // lanes past the end repeat the last candidate and take no entry, so the candidate test
+    // below runs without a divergent branch
+    const bool live = v0 + lane < total;
+    const int v = live ? v0 + lane : total - 1;
+    while (run_off[wv][r + 1] <= v) r++;
+    const uint32_t ent = run_word[wv][r] + (uint32_t)(v - run_off[wv][r]);
+    const int j = e_idx(ent);
+    const double4 xj = P.x[j];
+    const int mj = P.mol[j];  // issued with the position, not after the distance test
+    const double gx = __dadd_rn(xj.x, (double)e_s(ent, 0) * prd0);
+    const double gy = __dadd_rn(xj.y, (double)e_s(ent, 1) * prd1);
+    const double gz = __dadd_rn(xj.z, (double)e_s(ent, 2) * prd2);
+    const double delx = __dsub_rn(xi.x, gx), dely = __dsub_rn(xi.y, gy), delz = __dsub_rn(xi.z, gz);
+    const double rsq = rsq_ref(delx, dely, delz);
+    const bool take = live && !((ent >> IDX_BITS) == 0x15u && j == i);
+    bool type_ok = true;
+    if (!every_tj) type_ok = ((ilp_mask >> ((int)xj.w - 1)) & 1u) != 0;
+    const bool ilp = take && (rsq <= P.cutneighsq) && (mj != mi) && type_ok;
+    const bool near_i = rsq < P.ilp_splitsq;
+    const bool in_i = ilp && near_i, in_o = ilp && !near_i;
+    const bool sht = take && (rsq <= P.shortsq);
+    const bool near_s = rsq < P.short_splitsq;
+    const bool in_s = sht && near_s, in_sb = sht && !near_s;
+    const unsigned long long bi = __builtin_amdgcn_ballot_w64(in_i), bo = __builtin_amdgcn_ballot_w64(in_o),
+                             bs = __builtin_amdgcn_ballot_w64(in_s), bsb = __builtin_amdgcn_ballot_w64(in_sb);
     // set bits of each mask below this lane (v_mbcnt)
     auto below = [](unsigned long long m) {
       return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
