@@ -944,6 +944,25 @@ __device__ __forceinline__ Tap<R> calc_tap(R r, const double *poly, double rcut,
   return o;
 }
 
+// The ILP cutoff test of a displacement (x_i - x_j), returning r^2 in the pair precision: FP64
+// in the reference's operation order; in mixed precision on the FP32 r^2 of the FP64
+// displacement (a pair within rounding of rc contributes Tap(rc) = dTap(rc) = 0 either way).
+// Both walks of a row use this one test, so they agree on every entry.
+template <typename R>
+__device__ __forceinline__ bool ilp_cut_test(double dxd, double dyd, double dzd, double cutsq, R &rsq) {
+  if (sizeof(R) == 8) {
+    const double r2 = rsq_ref(dxd, dyd, dzd);
+    rsq = (R)r2;
+    return r2 < cutsq;
+  } else {
+#pragma clang fp contract(off)
+    const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
+    const R r2 = (dx * dx + dy * dy) + dz * dz;
+    rsq = r2;
+    return r2 < (R)cutsq;
+  }
+}
+
 // minimum resident waves per SIMD requested for the plain (force-only) ILP pair kernel: with
 // the prefetched chunk in registers four waves fit without spilling (five spill and lose 20 %);
 // the tally variants keep their registers
@@ -957,10 +976,13 @@ template <typename R, bool EF, bool VF, bool VA, bool SYM>
 __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) void k_ilp(DevPtrs<R> P) {
   using M = MathOps<R>;
   // one wavefront per atom: the atom index is wave-uniform (scalar registers)
-  const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + wv);
   const int lane = threadIdx.x & 63;
   // the pair-parameter table (MAXT x MAXT entries) is read per lane and per entry: from LDS
   __shared__ IlpPair<R> s_ip[MAXT * MAXT];
+  // per wave: back-segment entries within rc waiting for a full batch
+  __shared__ uint32_t s_q[4][128];
   stage_table(s_ip, P.ip, MAXT * MAXT);
   __syncthreads();
   if (i >= P.n) return;
@@ -973,7 +995,6 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
   // the back segment (r > rc + margin at the build) can hold a pair within rc only after some
   // atom has moved more than margin/2 since the build
   const int nfront = P.icnt[i], nback = *P.farflag ? P.ocnt[i] : 0;
-  const int cnt = nfront + nback, back_off = P.icap - P.ocnt[i] - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
   const int itag = P.tag[i];
   double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
@@ -983,165 +1004,202 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
   // accumulators once, after the loop; the wave reduction and everything after it stay FP64
   constexpr bool LANE32 = sizeof(R) == 4 && !VF && !VA;
   R fxl = 0, fyl = 0, fzl = 0, sxl = 0, syl = 0, szl = 0;
-  // software pipeline: the entry two chunks ahead and the position one chunk ahead are in
-  // flight while a chunk computes (15 % off the FP32 kernel, 2 % off the FP64 one)
+  // the pair terms of list entry e (j = its atom, xj its position) for this lane
+  auto pair_terms = [&](const uint32_t e, const double4 xj) {
+      const int j = e_idx(e);
+      double gx, gy, gz;
+      // x_j + 0*prd == x_j exactly, so the image arithmetic is skipped when no lane needs it
+      if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
+        gx = xj.x;
+        gy = xj.y;
+        gz = xj.z;
+      } else {
+        ghost(xj, e, prd, gx, gy, gz);
+      }
+      const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
+      const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
+      R rsq;
+      if (!ilp_cut_test<R>(dxd, dyd, dzd, P.ilp_cutsq, rsq)) return;
+      const int tj = (int)xj.w - 1;
+      const IlpPair<R> &pij = s_ip[ti * MAXT + tj];
+      const IlpPair<R> &pji = s_ip[tj * MAXT + ti];
+      const R rinv = M::rsqrt(rsq);
+      const R r = rsq * rinv;
+      const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
+      // ---- van der Waals, one half per ordered pair ----
+      const IlpPair<R> *pv = &pij;
+      if (!SYM && !pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
+      // E = -C6 / (r^6 (1 + e)), e = exp(-d (r/seff - 1)) (ilp.cpp:510-530); fpv = -dE/dr / r
+      const R r2inv = rinv * rinv;
+      const R edamp = M::ex(-pv->d * (r * pv->inv_seff - (R)1));
+      const R TSinv = M::rcp((R)1 + edamp);
+      const R w6 = r2inv * r2inv * r2inv * TSinv;
+      const R Vvdw = -pv->C6 * w6;
+      const R fpv = r2inv * w6 * (pv->C6_d_seff * r * edamp * TSinv - (R)6 * pv->C6);
+      // ---- repulsion (i,j) with n_i ----
+      const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
+      const R pn1 = nix * dx + niy * dy + niz * dz;
+      const R rd1 = (rsq - pn1 * pn1) * pij.d2inv;
+      const R frho1 = M::ex(-rd1) * pij.C;
+      const R V1 = exp0 * (pij.half_eps + frho1);
+      const R fq1 = (R)2 * exp0 * frho1 * pij.d2inv;
+      const R a1 = pn1 * fq1;
+      // ---- repulsion (j,i) with n_j; del_ji = -del ----
+      const auto nj4 = P.nrm[j];
+      const R exp0b = (SYM || pij.esym) ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
+      const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dz);
+      const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
+      const R frho2 = M::ex(-rd2) * pji.C;
+      const R V2 = exp0b * (pji.half_eps + frho2);
+      const R fq2 = (R)2 * exp0b * frho2 * pji.d2inv;
+      const R a2 = pn2 * fq2;
+      // Force on i: fsum_vdw del + fkc_ij - fkc_ji (ilp.cpp:525-534, 640-648) with
+      //   fsum_vdw = fpv Tap - V_vdw dTap / r,  fkc_ij = (del fs1 - a1 n_i) Tap - V1 dTap del / r,
+      //   fkc_ji = (-del fs2 - a2 n_j) Tap + V2 dTap del / r,  fs = lambda V / r + fq,
+      // collected as  C del + Tap (a2 n_j - a1 n_i)  (one scalar per pair instead of three
+      // vectors).  The weight of the normal-derivative forces is w_ij = a1 Tap.
+      const R ta1 = a1 * tp.t, ta2 = a2 * tp.t;
+      const R Cd = tp.t * (fpv + rinv * (pij.lambda * V1 + pji.lambda * V2) + fq1 + fq2) -
+                   tp.dt * rinv * (Vvdw + V1 + V2);
+      if (LANE32) {
+        fxl += Cd * dx + (ta2 * nj4.x - ta1 * nix);
+        fyl += Cd * dy + (ta2 * nj4.y - ta1 * niy);
+        fzl += Cd * dz + (ta2 * nj4.z - ta1 * niz);
+        sxl += ta1 * dx;
+        syl += ta1 * dy;
+        szl += ta1 * dz;
+      } else {
+        fx += (double)(Cd * dx + (ta2 * nj4.x - ta1 * nix));
+        fy += (double)(Cd * dy + (ta2 * nj4.y - ta1 * niy));
+        fz += (double)(Cd * dz + (ta2 * nj4.z - ta1 * niz));
+        sx += (double)(ta1 * dx);
+        sy += (double)(ta1 * dy);
+        sz += (double)(ta1 * dz);
+      }
+      if (EF) {
+        ev += 0.5 * (double)(Vvdw * tp.t);
+        er += 0.5 * (double)(tp.t * V1) + 0.5 * (double)(tp.t * V2);
+      }
+      if (VF || VA) {
+        // the separate force terms, for the tallies
+        const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
+        const R fs1 = pij.lambda * rinv * V1 + fq1, b1 = V1 * tp.dt * rinv;
+        const R fs2 = pji.lambda * rinv * V2 + fq2, b2 = V2 * tp.dt * rinv;
+        const R fkx = (dx * fs1 - a1 * nix) * tp.t - b1 * dx;
+        const R fky = (dy * fs1 - a1 * niy) * tp.t - b1 * dy;
+        const R fkz = (dz * fs1 - a1 * niz) * tp.t - b1 * dz;
+        const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
+        const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
+        const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
+        const R wgt = ta1;
+        // vdW: half of del (x) del * fsum per ordered pair; repulsion (i,j): del (x) fkc
+        const double hv = 0.5 * (double)fsv;
+        const double dd[3] = {dxd, dyd, dzd};
+        const double fk[3] = {(double)fkx, (double)fky, (double)fkz};
+        const double gk[3] = {(double)gkx, (double)gky, (double)gkz};
+        if (VF) {
+          w[0] += hv * dd[0] * dd[0] + dd[0] * fk[0];
+          w[1] += hv * dd[1] * dd[1] + dd[1] * fk[1];
+          w[2] += hv * dd[2] * dd[2] + dd[2] * fk[2];
+          w[3] += hv * dd[0] * dd[1] + dd[0] * fk[1];
+          w[4] += hv * dd[0] * dd[2] + dd[0] * fk[2];
+          w[5] += hv * dd[1] * dd[2] + dd[1] * fk[2];
+        }
+        if (VA) {
+          // ev_tally (vdW) half, ev_tally_xyz(i,j) half for (i,j), and the j-side half of (j,i):
+          // 0.5 * del_ji (x) fkc_ji = 0.5 * (-del) (x) (-gk)... fkc_ji = gk, del_ji = -del
+          va[0] += hv * dd[0] * dd[0] + 0.5 * (dd[0] * fk[0] - dd[0] * gk[0]);
+          va[1] += hv * dd[1] * dd[1] + 0.5 * (dd[1] * fk[1] - dd[1] * gk[1]);
+          va[2] += hv * dd[2] * dd[2] + 0.5 * (dd[2] * fk[2] - dd[2] * gk[2]);
+          va[3] += hv * dd[0] * dd[1] + 0.5 * (dd[0] * fk[1] - dd[0] * gk[1]);
+          va[4] += hv * dd[0] * dd[2] + 0.5 * (dd[0] * fk[2] - dd[0] * gk[2]);
+          va[5] += hv * dd[1] * dd[2] + 0.5 * (dd[1] * fk[2] - dd[1] * gk[2]);
+          const double wd = (double)wgt;
+          tt[0] += wd * dd[0] * dd[0];
+          tt[1] += wd * dd[1] * dd[1];
+          tt[2] += wd * dd[2] * dd[2];
+          tt[3] += wd * dd[0] * dd[1];
+          tt[4] += wd * dd[0] * dd[2];
+          tt[5] += wd * dd[1] * dd[2];
+          // k-terms of pair (j,i): half of (x_k - x_i) (x) fk lands on i (ev_tally_xyz(k, i)),
+          // fk = -w_ji * D_k(j)^T del_ji, x_k - x_i = vet_k(j) + del_ji
+          const uint32_t *nj = P.nnb + (size_t)j * 4;
+          const int cj = (int)nj[0];
+          const double wji = (double)(a2 * tp.t);
+          for (int kk = 0; kk < cj; kk++) {
+            const R *D = P.dn + (size_t)j * 27 + kk * 9;
+            const double *vt = P.vet + (size_t)j * 9 + kk * 3;
+            double fkk[3];
+            for (int c = 0; c < 3; c++)
+              fkk[c] = -wji * (-(double)D[0 * 3 + c] * dd[0] - (double)D[1 * 3 + c] * dd[1] - (double)D[2 * 3 + c] * dd[2]);
+            const double rk[3] = {vt[0] - dd[0], vt[1] - dd[1], vt[2] - dd[2]};
+            va[0] += 0.5 * rk[0] * fkk[0];
+            va[1] += 0.5 * rk[1] * fkk[1];
+            va[2] += 0.5 * rk[2] * fkk[2];
+            va[3] += 0.5 * rk[0] * fkk[1];
+            va[4] += 0.5 * rk[0] * fkk[2];
+            va[5] += 0.5 * rk[1] * fkk[2];
+          }
+        }
+      }
+  };
+  // front segment (r < rc + margin at the build; most of it within rc): software pipeline, the
+  // entry two chunks ahead and the position one chunk ahead are in flight while a chunk
+  // computes (15 % off the FP32 kernel, 2 % off the FP64 one)
   uint32_t e_n1 = 0, e_n2 = 0;
   double4 x_n1 = {0, 0, 0, 0};
-  if (lane < cnt) e_n1 = row[ilp_slot(lane, nfront, back_off)];
-  if (lane + 64 < cnt) e_n2 = row[ilp_slot(lane + 64, nfront, back_off)];
-  if (lane < cnt) x_n1 = P.x[e_idx(e_n1)];
-  for (int k = lane; k < cnt; k += 64) {
+  if (lane < nfront) e_n1 = row[lane];
+  if (lane + 64 < nfront) e_n2 = row[lane + 64];
+  if (lane < nfront) x_n1 = P.x[e_idx(e_n1)];
+  for (int k = lane; k < nfront; k += 64) {
     const uint32_t e = e_n1;
     const double4 xj = x_n1;
     e_n1 = e_n2;
-    if (k + 64 < cnt) x_n1 = P.x[e_idx(e_n1)];
-    if (k + 128 < cnt) e_n2 = row[ilp_slot(k + 128, nfront, back_off)];
-    const int j = e_idx(e);
-    double gx, gy, gz;
-    // x_j + 0*prd == x_j exactly, so the image arithmetic is skipped when no lane needs it
-    if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
-      gx = xj.x;
-      gy = xj.y;
-      gz = xj.z;
-    } else {
-      ghost(xj, e, prd, gx, gy, gz);
-    }
-    const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
-    const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
-    R rsq;
-    if (sizeof(R) == 8) {
-      // FP64: the cutoff decision in the reference's operation order
-      const double rsqd = rsq_ref(dxd, dyd, dzd);
-      if (rsqd >= P.ilp_cutsq) continue;
-      rsq = (R)rsqd;
-    } else {
-      // mixed: the test on the FP32 distance of the FP64 displacement; a pair within rounding
-      // of the cutoff contributes Tap(rc) = dTap(rc) = 0 either way
-      rsq = dx * dx + dy * dy + dz * dz;
-      if (rsq >= (R)P.ilp_cutsq) continue;
-    }
-    const int tj = (int)xj.w - 1;
-    const IlpPair<R> &pij = s_ip[ti * MAXT + tj];
-    const IlpPair<R> &pji = s_ip[tj * MAXT + ti];
-    const R rinv = M::rsqrt(rsq);
-    const R r = rsq * rinv;
-    const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
-    // ---- van der Waals, one half per ordered pair ----
-    const IlpPair<R> *pv = &pij;
-    if (!SYM && !pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
-    // E = -C6 / (r^6 (1 + e)), e = exp(-d (r/seff - 1)) (ilp.cpp:510-530); fpv = -dE/dr / r
-    const R r2inv = rinv * rinv;
-    const R edamp = M::ex(-pv->d * (r * pv->inv_seff - (R)1));
-    const R TSinv = M::rcp((R)1 + edamp);
-    const R w6 = r2inv * r2inv * r2inv * TSinv;
-    const R Vvdw = -pv->C6 * w6;
-    const R fpv = r2inv * w6 * (pv->C6_d_seff * r * edamp * TSinv - (R)6 * pv->C6);
-    // ---- repulsion (i,j) with n_i ----
-    const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
-    const R pn1 = nix * dx + niy * dy + niz * dz;
-    const R rd1 = (rsq - pn1 * pn1) * pij.d2inv;
-    const R frho1 = M::ex(-rd1) * pij.C;
-    const R V1 = exp0 * (pij.half_eps + frho1);
-    const R fq1 = (R)2 * exp0 * frho1 * pij.d2inv;
-    const R a1 = pn1 * fq1;
-    // ---- repulsion (j,i) with n_j; del_ji = -del ----
-    const auto nj4 = P.nrm[j];
-    const R exp0b = (SYM || pij.esym) ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
-    const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dz);
-    const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
-    const R frho2 = M::ex(-rd2) * pji.C;
-    const R V2 = exp0b * (pji.half_eps + frho2);
-    const R fq2 = (R)2 * exp0b * frho2 * pji.d2inv;
-    const R a2 = pn2 * fq2;
-    // Force on i: fsum_vdw del + fkc_ij - fkc_ji (ilp.cpp:525-534, 640-648) with
-    //   fsum_vdw = fpv Tap - V_vdw dTap / r,  fkc_ij = (del fs1 - a1 n_i) Tap - V1 dTap del / r,
-    //   fkc_ji = (-del fs2 - a2 n_j) Tap + V2 dTap del / r,  fs = lambda V / r + fq,
-    // collected as  C del + Tap (a2 n_j - a1 n_i)  (one scalar per pair instead of three
-    // vectors).  The weight of the normal-derivative forces is w_ij = a1 Tap.
-    const R ta1 = a1 * tp.t, ta2 = a2 * tp.t;
-    const R Cd = tp.t * (fpv + rinv * (pij.lambda * V1 + pji.lambda * V2) + fq1 + fq2) -
-                 tp.dt * rinv * (Vvdw + V1 + V2);
-    if (LANE32) {
-      fxl += Cd * dx + (ta2 * nj4.x - ta1 * nix);
-      fyl += Cd * dy + (ta2 * nj4.y - ta1 * niy);
-      fzl += Cd * dz + (ta2 * nj4.z - ta1 * niz);
-      sxl += ta1 * dx;
-      syl += ta1 * dy;
-      szl += ta1 * dz;
-    } else {
-      fx += (double)(Cd * dx + (ta2 * nj4.x - ta1 * nix));
-      fy += (double)(Cd * dy + (ta2 * nj4.y - ta1 * niy));
-      fz += (double)(Cd * dz + (ta2 * nj4.z - ta1 * niz));
-      sx += (double)(ta1 * dx);
-      sy += (double)(ta1 * dy);
-      sz += (double)(ta1 * dz);
-    }
-    if (EF) {
-      ev += 0.5 * (double)(Vvdw * tp.t);
-      er += 0.5 * (double)(tp.t * V1) + 0.5 * (double)(tp.t * V2);
-    }
-    if (VF || VA) {
-      // the separate force terms, for the tallies
-      const R fsv = fpv * tp.t - Vvdw * tp.dt * rinv;
-      const R fs1 = pij.lambda * rinv * V1 + fq1, b1 = V1 * tp.dt * rinv;
-      const R fs2 = pji.lambda * rinv * V2 + fq2, b2 = V2 * tp.dt * rinv;
-      const R fkx = (dx * fs1 - a1 * nix) * tp.t - b1 * dx;
-      const R fky = (dy * fs1 - a1 * niy) * tp.t - b1 * dy;
-      const R fkz = (dz * fs1 - a1 * niz) * tp.t - b1 * dz;
-      const R gkx = (-dx * fs2 - a2 * nj4.x) * tp.t + b2 * dx;
-      const R gky = (-dy * fs2 - a2 * nj4.y) * tp.t + b2 * dy;
-      const R gkz = (-dz * fs2 - a2 * nj4.z) * tp.t + b2 * dz;
-      const R wgt = ta1;
-      // vdW: half of del (x) del * fsum per ordered pair; repulsion (i,j): del (x) fkc
-      const double hv = 0.5 * (double)fsv;
-      const double dd[3] = {dxd, dyd, dzd};
-      const double fk[3] = {(double)fkx, (double)fky, (double)fkz};
-      const double gk[3] = {(double)gkx, (double)gky, (double)gkz};
-      if (VF) {
-        w[0] += hv * dd[0] * dd[0] + dd[0] * fk[0];
-        w[1] += hv * dd[1] * dd[1] + dd[1] * fk[1];
-        w[2] += hv * dd[2] * dd[2] + dd[2] * fk[2];
-        w[3] += hv * dd[0] * dd[1] + dd[0] * fk[1];
-        w[4] += hv * dd[0] * dd[2] + dd[0] * fk[2];
-        w[5] += hv * dd[1] * dd[2] + dd[1] * fk[2];
+    if (k + 64 < nfront) x_n1 = P.x[e_idx(e_n1)];
+    if (k + 128 < nfront) e_n2 = row[k + 128];
+    pair_terms(e, xj);
+  }
+  // back segment (walked once the far flag is up): only a few of its entries have come within
+  // rc, but nearly every chunk of 64 holds one, so the entries are tested first and the pair
+  // terms run on full batches of 64 entries that passed, queued in LDS
+  if (nback > 0) {
+    const uint32_t *brow = row + (P.icap - nback);
+    int qn = 0;  // queued entries (wave-uniform, below 64 between chunks)
+    // pair terms of the first min(qn, 64) queued entries; the rest moves to the queue's front
+    auto drain = [&]() {
+      // the queue is written and read by lanes of this wave only
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const int take = qn < 64 ? qn : 64;
+      const uint32_t qe = s_q[wv][lane];
+      const uint32_t rest = lane + 64 < qn ? s_q[wv][lane + 64] : 0u;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (lane + 64 < qn) s_q[wv][lane] = rest;
+      qn -= take;
+      if (lane < take) pair_terms(qe, P.x[e_idx(qe)]);
+    };
+    for (int k0 = 0; k0 < nback; k0 += 64) {
+      const int k = k0 + lane;
+      bool in = false;
+      uint32_t e = 0;
+      if (k < nback) {
+        e = brow[k];
+        const double4 xj = P.x[e_idx(e)];
+        double gx, gy, gz;
+        ghost(xj, e, prd, gx, gy, gz);
+        R r2;
+        in = ilp_cut_test<R>(xi.x - gx, xi.y - gy, xi.z - gz, P.ilp_cutsq, r2);
       }
-      if (VA) {
-        // ev_tally (vdW) half, ev_tally_xyz(i,j) half for (i,j), and the j-side half of (j,i):
-        // 0.5 * del_ji (x) fkc_ji = 0.5 * (-del) (x) (-gk)... fkc_ji = gk, del_ji = -del
-        va[0] += hv * dd[0] * dd[0] + 0.5 * (dd[0] * fk[0] - dd[0] * gk[0]);
-        va[1] += hv * dd[1] * dd[1] + 0.5 * (dd[1] * fk[1] - dd[1] * gk[1]);
-        va[2] += hv * dd[2] * dd[2] + 0.5 * (dd[2] * fk[2] - dd[2] * gk[2]);
-        va[3] += hv * dd[0] * dd[1] + 0.5 * (dd[0] * fk[1] - dd[0] * gk[1]);
-        va[4] += hv * dd[0] * dd[2] + 0.5 * (dd[0] * fk[2] - dd[0] * gk[2]);
-        va[5] += hv * dd[1] * dd[2] + 0.5 * (dd[1] * fk[2] - dd[1] * gk[2]);
-        const double wd = (double)wgt;
-        tt[0] += wd * dd[0] * dd[0];
-        tt[1] += wd * dd[1] * dd[1];
-        tt[2] += wd * dd[2] * dd[2];
-        tt[3] += wd * dd[0] * dd[1];
-        tt[4] += wd * dd[0] * dd[2];
-        tt[5] += wd * dd[1] * dd[2];
-        // k-terms of pair (j,i): half of (x_k - x_i) (x) fk lands on i (ev_tally_xyz(k, i)),
-        // fk = -w_ji * D_k(j)^T del_ji, x_k - x_i = vet_k(j) + del_ji
-        const uint32_t *nj = P.nnb + (size_t)j * 4;
-        const int cj = (int)nj[0];
-        const double wji = (double)(a2 * tp.t);
-        for (int kk = 0; kk < cj; kk++) {
-          const R *D = P.dn + (size_t)j * 27 + kk * 9;
-          const double *vt = P.vet + (size_t)j * 9 + kk * 3;
-          double fkk[3];
-          for (int c = 0; c < 3; c++)
-            fkk[c] = -wji * (-(double)D[0 * 3 + c] * dd[0] - (double)D[1 * 3 + c] * dd[1] - (double)D[2 * 3 + c] * dd[2]);
-          const double rk[3] = {vt[0] - dd[0], vt[1] - dd[1], vt[2] - dd[2]};
-          va[0] += 0.5 * rk[0] * fkk[0];
-          va[1] += 0.5 * rk[1] * fkk[1];
-          va[2] += 0.5 * rk[2] * fkk[2];
-          va[3] += 0.5 * rk[0] * fkk[1];
-          va[4] += 0.5 * rk[0] * fkk[2];
-          va[5] += 0.5 * rk[1] * fkk[2];
-        }
-      }
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+      const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      if (in) s_q[wv][qn + below] = e;
+      qn += (int)__popcll(m);
+      if (qn >= 64) drain();
     }
+    if (qn > 0) drain();
   }
   if (LANE32) {
     fx += (double)fxl; fy += (double)fyl; fz += (double)fzl;
