@@ -163,12 +163,31 @@ def test_mixed_force_error(name):
     assert abs(g["energy"] - r["energy"]) <= 1e-6 * abs(r["energy"])
 
 
-def test_neighbor_sets_bit_exact():
-    s = S.layered_stack(15, 9, ["G", "BN", "G"], seed=21, jitter=0.07)
+def _periodic_z_stack():
+    """12 alternating sheets 3.35 A apart in a box periodic in z (40.2 A): stencil rows wrap in z."""
+    s = S.layered_stack(15, 9, ["G", "BN"] * 6, seed=23, jitter=0.07)
+    s.boxlo[2], s.boxhi[2] = -1.675, 12 * 3.35 - 1.675
+    s.periodic = (1, 1, 1)
+    return s
+
+
+def _bilayers_far_apart():
+    """Two bilayers 40 A apart in a non-periodic z: 16 z-bins, empty rows of bins between them."""
+    s = S.layered_stack(15, 9, ["G", "BN", "G", "BN"], seed=24, jitter=0.07)
+    s.x[s.mol >= 3, 2] += 40.0
+    s.boxhi[2] += 40.0
+    return s
+
+
+@pytest.mark.parametrize("geometry", ["trilayer", "periodic_z", "bilayers_far_apart"])
+def test_neighbor_sets_bit_exact(geometry):
+    s = {"trilayer": lambda: S.layered_stack(15, 9, ["G", "BN", "G"], seed=21, jitter=0.07),
+         "periodic_z": _periodic_z_stack, "bilayers_far_apart": _bilayers_far_apart}[geometry]()
+    print(f"neighbor sets, {geometry}: {s.n} atoms", flush=True)
     o = oracle_for(s)
     offs, ent, sh = o.neighbor_list()
     with ctx_for(s) as ctx:
-        ctx.pair_compute(s, 1, 0)
+        g = ctx.pair_compute(s, 1, 2)
         for i in np.random.default_rng(2).choice(s.n, 64, replace=False):
             rows = ctx.ilp_list(0, int(i))
             want = sorted((int(j), *map(int, sh[e])) for e, j in zip(range(offs[i], offs[i + 1]),
@@ -176,6 +195,8 @@ def test_neighbor_sets_bit_exact():
                           if s.mol[j] != s.mol[i])
             got = sorted(map(tuple, rows.tolist()))
             assert got == want, i
+    # the short lists (Tersoff, normals) through the forces of every atom
+    _check_fp64(g, o.compute(1, 2))
 
 
 def test_normals_match_oracle():
