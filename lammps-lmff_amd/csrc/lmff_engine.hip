@@ -709,7 +709,7 @@ struct Ctx {
       hipLaunchKernelGGL(k_image_follow, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)x.p,
                          cg_x0.as<double4>(), sysid.as<int>(), systab.as<SysDev>());
     }
-    hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
+    hipLaunchKernelGGL(k_cell_start, dim3((ncells + 1 + BS - 1) / BS), dim3(BS), 0, stream, (int)n, ncells,
                        key2.as<uint32_t>(), cell_start.as<int>());
     hipLaunchKernelGGL(k_row_mol, dim3((nrows_cells + 3) / 4), dim3(BS), 0, stream, nrows_cells, (int)sys.size(),
                        (const SysDev *)systab.as<SysDev>(), (const int *)cell_start.as<int>(), (const int *)mol.as<int>(),
@@ -1527,7 +1527,7 @@ struct Ctx {
       hipLaunchKernelGGL(k_permute<int>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d, (const int *)mola, molb);
       hipLaunchKernelGGL(k_permute<int>, dim3(nb), dim3(BS), 0, stream, m, (const int *)perm_d, (const int *)origa,
                          origb);
-      hipLaunchKernelGGL(k_cell_start, dim3((m + 1 + BS - 1) / BS), dim3(BS), 0, stream, m, g.ncells,
+      hipLaunchKernelGGL(k_cell_start, dim3((g.ncells + 1 + BS - 1) / BS), dim3(BS), 0, stream, m, g.ncells,
                          (const uint32_t *)key2_d, cstart_d);
       hipLaunchKernelGGL(k_row_mol, dim3((g.nc[1] * g.nc[2] + 3) / 4), dim3(BS), 0, stream, g.nc[1] * g.nc[2], 1,
                          (const SysDev *)systab_d, (const int *)cstart_d, (const int *)molb, rmol_d);

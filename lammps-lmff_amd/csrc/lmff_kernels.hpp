@@ -397,13 +397,18 @@ __global__ void k_pair_globals(const double *sums, int eflag, int vflag, double 
   out[8] = e ? sums[1] : 0.0;
 }
 
-// cell_start[c] = first sorted position with key >= c, for c in [0, ncells]
+// cell_start[c] = first sorted position with key >= c, for c in [0, ncells]: one thread per cell,
+// a binary search over the sorted keys (a run of empty cells costs nothing extra)
 __global__ void k_cell_start(int n, int ncells, const uint32_t *key, int *start) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q > n) return;
-  int lo = (q == 0) ? 0 : (int)key[q - 1] + 1;
-  int hi = (q == n) ? ncells : (int)key[q];
-  for (int c = lo; c <= hi; c++) start[c] = q;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncells) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int)key[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = lo;
 }
 
 // Molecule range {min, max} of the atoms of each row of cells (one system, fixed y and z bin;
