@@ -483,6 +483,18 @@ def _fresh(js, rate=None, seed=77):
     return out
 
 
+def _progress(label, every_s=30.0):
+    """on_tick callback of Server.run: a progress line at most every every_s seconds."""
+    last = [time.perf_counter()]
+
+    def cb(srv, now):
+        t = time.perf_counter()
+        if t - last[0] >= every_s:
+            last[0] = t
+            log(f"{label}: {len(srv.resident)} jobs resident, {now:.0f} s into the stream")
+    return cb
+
+
 def sequential_direct(js, precision=capi.LMFF_MIXED, tick=10):
     """The jobs one at a time through the plain single-system path: one context, each job added,
     advanced by one lmff_run call over all its steps (graph-replayed, thermo at the end only)
@@ -496,13 +508,17 @@ def sequential_direct(js, precision=capi.LMFF_MIXED, tick=10):
         ctx.run(2 * tick, 0, use_graph=True)
         ctx.remove_system(sid)
         t0 = time.perf_counter()
+        last = t0
         atom_steps = 0
-        for j in js:
+        for k, j in enumerate(js):
             sid = ctx.add_system(j.system)
             ctx.integrator(sid, j.style, dt=0.001, t_target=j.t_target, damp=j.damp, seed=j.seed)
             ctx.run(j.steps, j.steps, use_graph=True)
             ctx.remove_system(sid)
             atom_steps += j.natoms * j.steps
+            if time.perf_counter() - last >= 30.0:
+                last = time.perf_counter()
+                log(f"one at a time: {k + 1} of {len(js)} jobs done")
         wall = time.perf_counter() - t0
         return {"makespan_s": wall, "atom_steps_per_s": atom_steps / wall}
     finally:
@@ -517,12 +533,14 @@ def stream_capacity(js, tick=10):
 
     srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=tick)
     srv.run(_fresh(js[:2]))   # warm the context (module load, first graph capture)
-    packed = srv.run(_fresh(js))
+    packed = srv.run(_fresh(js), on_tick=_progress("packed"))
     srv.close()
+    log(f"packed: {packed['makespan_s']:.1f} s; one at a time through the server")
     seq_srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=tick, max_resident=1)
     seq_srv.run(_fresh(js[:1]))
-    seq = seq_srv.run(_fresh(js))
+    seq = seq_srv.run(_fresh(js), on_tick=_progress("one at a time through the server"))
     seq_srv.close()
+    log(f"through the server: {seq['makespan_s']:.1f} s; one at a time, plain path")
     direct = sequential_direct(_fresh(js), tick=tick)
     return packed, seq, direct
 
@@ -532,7 +550,7 @@ def stream_latency(js, rate, tick=10):
 
     srv = serving.Server(precision=capi.LMFF_MIXED, hbm_budget_bytes=200e9, tick=tick)
     srv.run(_fresh(js[:2]))
-    lat = srv.run(_fresh(js, rate=rate))
+    lat = srv.run(_fresh(js, rate=rate), on_tick=_progress("latency run"))
     srv.close()
     return lat
 

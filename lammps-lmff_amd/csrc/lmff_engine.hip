@@ -1859,9 +1859,8 @@ int lmff_system_remove(lmff_ctx *ctx, int sys_id) {
     auto &c = ctx->c;
     SysHost *s = c.find(sys_id);
     if (!s) fail(LMFF_ERR_STATE, "no system %d", sys_id);
-    // resident data of the other systems must be current before re-packing
-    c.relayout();
-    s = c.find(sys_id);
+    // marked only: the next re-pack (relayout) drops every removed system in one pass and keeps
+    // the others' resident data; until then the system tables keep their order
     s->live = false;
     c.layout_dirty = true;
     if (c.pair_sys == sys_id) c.pair_sys = -1;

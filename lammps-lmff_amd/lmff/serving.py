@@ -126,12 +126,16 @@ class Server:
         self.resident[sid] = job
         self.reserved += self._job_bytes(job)
 
-    def _retire(self, job: Job, now: float):
+    def _collect(self, job: Job):
+        """The finished job's result, read while the packed layout is still intact (a removal
+        marks it for a re-pack, which the next read would pay for every remaining job)."""
         x, v, _ = self.ctx.get(job.sys_id)
         img = self.ctx.get_image(job.sys_id)
         last = job.trace[-1] if job.trace else {}
         job.result = {"e_interlayer": last.get("e_vdw", float("nan")) + last.get("e_rep", float("nan")),
                       "pe": last.get("pe"), "temp": last.get("temp"), "x": x, "v": v, "image": img}
+
+    def _retire(self, job: Job, now: float):
         self.ctx.remove_system(job.sys_id)
         self.reserved -= self._job_bytes(job)
         del self.resident[job.sys_id]
@@ -175,8 +179,12 @@ class Server:
                 for job in [j for j in self.resident.values() if j.job_id in self._cancel]:
                     self._drop(job, now)
                 self._cancel -= {j.job_id for j in self.cancelled}
-            # retire finished jobs
-            for job in [j for j in self.resident.values() if j.done_steps >= j.steps]:
+            # retire finished jobs: read every result first, then remove them all (one re-pack at
+            # the next tick instead of one per retired job)
+            done = [j for j in self.resident.values() if j.done_steps >= j.steps]
+            for job in done:
+                self._collect(job)
+            for job in done:
                 self._retire(job, now)
                 finished.append(job)
             # admit in arrival order while the budget allows
