@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -39,6 +40,7 @@ namespace {
 
 constexpr int CHUNK = 4096;
 constexpr int BS = 256;
+constexpr int HALF_TILE_X = 6, HALF_TILE_Y = 3;  // bins per half-pair ILP tile
 constexpr int MAX_DEVICES = 64;
 
 // device address of `scratch` per GPU, looked up once per device with that device current
@@ -124,6 +126,8 @@ struct SysHost {
   double glo[3], bs[3];
   int cell_off, ncells;
   int row_off = 0;  // first of this system's rows of cells (fixed y and z bin)
+  int tile[2] = {0, 0}, ntile[2] = {0, 0};  // half-pair ILP tiles (bins per tile, tiles)
+  int tile_off = 0;                          // first tile x z-bin workgroup / window of the system
   int begin, end;
   int style = LMFF_NVE;
   double dt = 0.001, t_target = 300.0, damp = 1.0;
@@ -184,6 +188,15 @@ struct Ctx {
   DBuf cell_start, row_mol, sort_tmp, systab, chunks, cbeg, partial, sums, rmass;
   DBuf ip_d, ip_f, tp_d, tp_f, t3, errw, ovf, dflag, fsave, farflag;
   DBuf hb_in, hb_out;  // host-buffer Pair::compute: caller-order arrays on the device
+  // half-pair ILP: window slots of list entries, per-dz segments, window tables, per-system
+  // largest window and mode, partner sums
+  DBuf lidx, segf, segb, win, winmax, syshalf, acc, ipart;
+  int npass = 0;               // z-bin offsets the half-pair kernel walks (0 .. npass - 1)
+  int ntiles = 0;              // tile x z-bin workgroups (= windows) over every system
+  std::vector<int> half_sys;   // per system: 1 when its ILP pairs run in the half-pair kernel
+  bool half_any = false, pull_any = true;
+  int wcap = 0;                // window atoms the half-pair kernel's LDS holds (largest half window)
+  bool half_enabled = true;    // LMFF_HALF=0 in the environment keeps every system in the pull kernel
   PinnedBuf hb_host;   // and the forces on their way back
   int ntp = 0;  // Tersoff parameter records
   bool ilp_sym = false;  // every mapped ILP type pair symmetric (k_ilp's SYM variant applies)
@@ -231,8 +244,10 @@ struct Ctx {
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &row_mol, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
-                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out};
+                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart};
     for (DBuf *b : all) b->tally = &bytes_in_use;
+    const char *h = std::getenv("LMFF_HALF");
+    if (h && std::strcmp(h, "0") == 0) half_enabled = false;
   }
   ~Ctx() {
     if (note_host) (void)hipHostFree(note_host);
@@ -386,7 +401,7 @@ struct Ctx {
       src_stage.push_back(&st);
     }
     int64_t nn = 0;
-    int nc = 0, nr = 0;
+    int nc = 0, nr = 0, nt = 0;
     for (auto &s : next) {
       s.begin = (int)nn;
       nn += s.n;
@@ -395,6 +410,8 @@ struct Ctx {
       nc += s.ncells;
       s.row_off = nr;
       nr += s.nc[1] * s.nc[2];
+      s.tile_off = nt;
+      if (s.tile[0] > 0) nt += s.ntile[0] * s.ntile[1] * s.nc[2];
     }
     if (nn >= (int64_t)IDX_MASK) fail(LMFF_ERR_MEMORY, "too many atoms (%lld) for 26-bit list indices", (long long)nn);
     const size_t m1 = std::max<size_t>((size_t)nn, 1);
@@ -461,7 +478,16 @@ struct Ctx {
     n = nn;
     ncells = nc;
     nrows_cells = nr;
+    ntiles = nt;
     alloc_atoms((size_t)n);
+    win.ensure(sizeof(int) * (size_t)std::max(ntiles, 1) * WSTR);
+    winmax.ensure(sizeof(int) * std::max<size_t>(sys.size(), 1));
+    syshalf.ensure(sizeof(int) * std::max<size_t>(sys.size(), 1));
+    HIPCHK(hipMemsetAsync(syshalf.p, 0, sizeof(int) * std::max<size_t>(sys.size(), 1), stream));
+    half_sys.assign(sys.size(), 0);
+    half_any = false;
+    pull_any = true;
+    wcap = 0;
     if (n) HIPCHK(hipMemcpyAsync(f.p, fsave.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     restore_f = had_new ? keep_f : std::vector<std::pair<int, int>>{};
     upload_systab();
@@ -502,7 +528,7 @@ struct Ctx {
 
   // Bytes per atom of everything alloc_atoms() sizes (term for term), plus the radix sort's
   // alternate key/value buffers and list rows of scap + icap entries.
-  static double atom_bytes(int scap_, int icap_) {
+  static double atom_bytes(int scap_, int icap_, int nlayers = 2) {
     const double d4 = 9 * 32.0;                                     // x v f xhold tmp4 nrm ft fil S
     const double i32 = 16 * 4.0;                                    // mol tag img sysid orig tmpi key key2
                                                                     // val perm scnt icnt ocnt tcnt rtcnt rncnt
@@ -510,7 +536,12 @@ struct Ctx {
                                                                     // vatom eb fb
     const double u32 = 4.0 * (4 + 3 * MAXSH);                       // nnb tsh rtc rnc
     const double sort = 8.0;
-    return d4 + i32 + f64 + u32 + sort + 4.0 * ((double)scap_ + (double)icap_);
+    // half-pair ILP: partner sums (6 int64), per-dz segments (2 x NSEG uint16), one window slot
+    // per ILP list entry (uint16), window tables (WSTR ints per 6 x 3-bin tile and z bin, about
+    // 60 atoms per table in a stack of sheets with an empty z bin between some of them)
+    const double half = 48.0 + 4.0 * NSEG + 2.0 * (double)icap_ + 4.0 * WSTR / 60.0 +
+                        8.0 * half_pstr(true, true) * std::min(8, std::max(2, nlayers + 1));  // + pass sums
+    return d4 + i32 + f64 + u32 + sort + half + 4.0 * ((double)scap_ + (double)icap_);
   }
   // ILP row capacity the list build settles on for the worst atom of an AB-stacked pile of
   // nlayers sheets 3.2 A apart with at most 0.39 atoms/A^2 per sheet (graphene 0.382, hBN
@@ -553,6 +584,12 @@ struct Ctx {
     ft.ensure(m * sizeof(double4));
     fil.ensure(m * sizeof(double4)); S.ensure(m * sizeof(double4)); T.ensure(m * 6 * sizeof(double));
     th.ensure(m * NTH * sizeof(double)); vatom.ensure(m * 6 * sizeof(double));
+    segf.ensure(m * NSEG * sizeof(uint16_t)); segb.ensure(m * NSEG * sizeof(uint16_t));
+    if (acc.bytes < m * 6 * sizeof(unsigned long long)) {
+      acc.ensure(m * 6 * sizeof(unsigned long long));
+      // the half-pair kernel's partner sums start from zero; k_ilp_combine clears them after use
+      HIPCHK(hipMemsetAsync(acc.p, 0, acc.bytes, stream));
+    }
   }
 
   void upload_systab() {
@@ -570,6 +607,11 @@ struct Ctx {
       }
       d.cell_off = s.cell_off;
       d.row_off = s.row_off;
+      d.tile[0] = s.tile[0];
+      d.tile[1] = s.tile[1];
+      d.ntile[0] = s.ntile[0];
+      d.ntile[1] = s.ntile[1];
+      d.tile_off = s.tile_off;
       d.begin = s.begin;
       d.end = s.end;
       d.dtv = s.dt;
@@ -611,6 +653,18 @@ struct Ctx {
     P.err = errw.as<int>();
     P.ovf = ovf.as<int>();
     P.farflag = farflag.as<int>();
+    P.cell_start = cell_start.as<int>();
+    P.lidx = lidx.as<uint16_t>();
+    P.segf = segf.as<uint16_t>();
+    P.segb = segb.as<uint16_t>();
+    P.win = ntiles > 0 ? win.as<int>() : nullptr;
+    P.winmax = winmax.as<int>();
+    P.syshalf = syshalf.as<int>();
+    P.acc = acc.as<unsigned long long>();
+    P.ipart = ipart.as<double>();
+    P.npass = npass;
+    P.nsys = (int)sys.size();
+    P.half_on = 0;
     scalars(P);
     return P;
   }
@@ -714,6 +768,13 @@ struct Ctx {
     hipLaunchKernelGGL(k_row_mol, dim3((nrows_cells + 3) / 4), dim3(BS), 0, stream, nrows_cells, (int)sys.size(),
                        (const SysDev *)systab.as<SysDev>(), (const int *)cell_start.as<int>(), (const int *)mol.as<int>(),
                        row_mol.as<int2>());
+    if (ntiles > 0) {
+      hipLaunchKernelGGL(k_fill_int, dim3(((int)sys.size() + BS - 1) / BS), dim3(BS), 0, stream, (int)sys.size(),
+                         winmax.as<int>(), 0);
+      hipLaunchKernelGGL(k_win_table, dim3((ntiles + 3) / 4), dim3(BS), 0, stream, ntiles, (int)sys.size(),
+                         (const SysDev *)systab.as<SysDev>(), (const int *)cell_start.as<int>(),
+                         std::sqrt(cutneigh * cutneigh), win.as<int>(), winmax.as<int>());
+    }
     HIPCHK(hipMemcpyAsync(xhold.p, x.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 1, farflag.as<int>(), 0);
     build_lists();
@@ -734,10 +795,58 @@ struct Ctx {
     HIPCHK(hipMemcpyAsync(b.p, tmpi.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
   }
 
+  // Which systems run their ILP pairs in the half-pair kernel: those with tiles whose largest
+  // window fits the kernel's LDS, whose stencil reaches at most NSEG - 2 z bins up, and whose
+  // list rows fit the 16-bit segment counts.  Decided per system from its own geometry, so a
+  // system's results do not depend on what else is packed with it.
+  void choose_half(const std::vector<int> &wm) {
+    std::vector<int> hs(sys.size(), 0);
+    int wc = 0, np = 0;
+    const double cut = std::sqrt(cutneigh * cutneigh);
+    for (size_t k = 0; k < sys.size(); k++) {
+      const SysHost &q = sys[k];
+      const int rz = (int)std::ceil(cut * (1.0 / q.bs[2]));
+      if (half_enabled && have_ilp && q.tile[0] > 0 && q.n > 0 && wm[k] > 0 && wm[k] <= WCAP_MAX &&
+          rz + 1 <= NSEG - 2 && icap < 0xffff) {
+        hs[k] = 1;
+        wc = std::max(wc, wm[k]);
+        // passes: z-bin offsets 0 .. rz (within the bins in a non-periodic z)
+        np = std::max(np, q.per[2] ? rz + 1 : std::min(rz + 1, q.nc[2]));
+      }
+    }
+    bool any = false, pull = false;
+    for (size_t k = 0; k < sys.size(); k++) {
+      if (hs[k]) any = true;
+      else if (sys[k].n > 0) pull = true;
+    }
+    wc = (wc + 63) / 64 * 64;
+    if (any) {
+      half_attrs();  // here, never inside a graph capture
+      // pass sums at the widest layout (with the virial), so that no later evaluation moves the
+      // buffer under a captured graph
+      if (ipart.bytes < sizeof(double) * (size_t)np * (size_t)n * half_pstr(true, true)) {
+        ipart.ensure(sizeof(double) * (size_t)np * (size_t)n * half_pstr(true, true));
+        gen++;
+      }
+    }
+    if (hs != half_sys || wc != wcap || any != half_any || pull != pull_any || np != npass) {
+      half_sys = hs;
+      wcap = wc;
+      npass = np;
+      half_any = any;
+      pull_any = pull;
+      if (!hs.empty())
+        HIPCHK(hipMemcpyAsync(syshalf.p, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      gen++;  // the launches of a captured step change
+    }
+  }
+
   void build_lists() {
     for (int attempt = 0; attempt < 4; attempt++) {
       slist.ensure((size_t)n * scap * sizeof(uint32_t));
       ilist.ensure((size_t)n * std::max(icap, 1) * sizeof(uint32_t));
+      if (ntiles > 0) lidx.ensure((size_t)n * std::max(icap, 1) * sizeof(uint16_t));
       HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int) * 4, stream));
       auto P = ptrs<double>();
       const int nb = (int)((n + 3) / 4);
@@ -745,7 +854,11 @@ struct Ctx {
                          (const int2 *)row_mol.as<int2>());
       int o[4];
       HIPCHK(hipMemcpyAsync(o, ovf.p, sizeof o, hipMemcpyDeviceToHost, stream));
+      std::vector<int> wm(sys.size(), 0);
+      if (ntiles > 0 && !sys.empty())
+        HIPCHK(hipMemcpyAsync(wm.data(), winmax.p, sizeof(int) * sys.size(), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
+      if (o[0] <= 0 && o[1] <= 0) choose_half(wm);
       bool grow = false;
       if (o[0] > 0) {
         scap = ((int)(o[0] * 1.25) + 63) / 64 * 64;
@@ -784,13 +897,49 @@ struct Ctx {
   }
 
   template <typename R>
-  void force_impl(int ef, int vf, int va, int integ, int want_ke) {
-    force_kernels<R>(ptrs<R>(), ef, vf, va, integ, want_ke);
+  void force_impl(int ef, int vf, int va, int integ, int want_ke, int half) {
+    force_kernels<R>(ptrs<R>(), ef, vf, va, integ, want_ke, half);
+  }
+
+  template <typename R, bool EF, bool VF, bool SYM>
+  void launch_half_one(const DevPtrs<R> &P) {
+    const size_t lds = sizeof(unsigned long long) * 6 * (size_t)wcap;
+    hipLaunchKernelGGL((k_ilp_half<R, EF, VF, SYM>), dim3(ntiles), dim3(64 * HalfWaves<R>::n), lds, stream, P, wcap);
+  }
+  template <typename R, bool SYM>
+  void launch_half(const DevPtrs<R> &P, int ef, int vf) {
+    if (vf) launch_half_one<R, true, true, SYM>(P);
+    else if (ef) launch_half_one<R, true, false, SYM>(P);
+    else launch_half_one<R, false, false, SYM>(P);
+  }
+  // the half-pair kernel's window sums take more than the default 64 KB of LDS per workgroup
+  template <typename R, bool EF, bool VF, bool SYM>
+  static void half_attr_one() {
+    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
+  }
+  template <typename R>
+  static void half_attr_prec() {
+    half_attr_one<R, false, false, false>();
+    half_attr_one<R, true, false, false>();
+    half_attr_one<R, true, true, false>();
+    half_attr_one<R, false, false, true>();
+    half_attr_one<R, true, false, true>();
+    half_attr_one<R, true, true, true>();
+  }
+  bool half_attr_done = false;
+  void half_attrs() {
+    if (half_attr_done) return;
+    half_attr_prec<double>();
+    half_attr_prec<float>();
+    half_attr_done = true;
   }
 
   // normals -> ILP pair -> Tersoff bonds -> gather over the arrays P points at
+  // half: the systems marked half run their ILP pairs in the half-pair kernel (the others, and
+  // every system in an evaluation that asks for per-atom tallies, in the pull kernel)
   template <typename R>
-  void force_kernels(const DevPtrs<R> &P, int ef, int vf, int va, int integ, int want_ke) {
+  void force_kernels(const DevPtrs<R> &P, int ef, int vf, int va, int integ, int want_ke, int half = 0) {
     const int n = P.n;
     const int nb1 = (int)((n + BS - 1) / BS);
     const int nbw = (int)((n + 3) / 4);
@@ -800,8 +949,22 @@ struct Ctx {
     else hipLaunchKernelGGL((k_normals<R, false>), dim3(nb1), dim3(BS), 0, stream, P);
     mark_end(1, mk);
     mark_begin(2, mk);
-    if (ilp_sym) launch_ilp<R, true>(P, ef, vf, va, nbw);
-    else launch_ilp<R, false>(P, ef, vf, va, nbw);
+    if (half) {
+      DevPtrs<R> H = P;
+      H.half_on = 1;
+      if (pull_any) {
+        if (ilp_sym) launch_ilp<R, true>(H, ef, vf, va, nbw);
+        else launch_ilp<R, false>(H, ef, vf, va, nbw);
+      }
+      if (ilp_sym) launch_half<R, true>(H, ef, vf);
+      else launch_half<R, false>(H, ef, vf);
+      if (vf) hipLaunchKernelGGL((k_ilp_combine<R, true, true>), dim3(nb1), dim3(BS), 0, stream, H);
+      else if (ef) hipLaunchKernelGGL((k_ilp_combine<R, true, false>), dim3(nb1), dim3(BS), 0, stream, H);
+      else hipLaunchKernelGGL((k_ilp_combine<R, false, false>), dim3(nb1), dim3(BS), 0, stream, H);
+    } else {
+      if (ilp_sym) launch_ilp<R, true>(P, ef, vf, va, nbw);
+      else launch_ilp<R, false>(P, ef, vf, va, nbw);
+    }
     mark_end(2, mk);
     mark_begin(3, mk);
     const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom
@@ -828,8 +991,11 @@ struct Ctx {
     if (va) wb.ensure((size_t)n * MAXSH * MAXSH * 6 * sizeof(double));
     const int vf = ((vflag % 4) != 0 || va) ? 1 : 0;
     const int ef = (eflag != 0 || vf) ? 1 : 0;
-    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va, integ, want_ke);
-    else force_impl<double>(ef, vf, va, integ, want_ke);
+    // per-atom energy or virial: the pull kernel tallies both halves of every pair as the
+    // reference's ev_tally does; the half-pair kernel gives a pair's energy to one atom
+    const int half = (half_any && !va && !(eflag & 2)) ? 1 : 0;
+    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va, integ, want_ke, half);
+    else force_impl<double>(ef, vf, va, integ, want_ke, half);
     last_eflag = ef ? (eflag | 1) : 0;
     last_vflag = va ? 4 | 2 : (vf ? 2 : 0);
     forces_valid = true;
@@ -847,6 +1013,8 @@ struct Ctx {
     if (e[0] & EB_NORMAL_ZERO) fail(LMFF_ERR_NORMAL, "The magnitude of the normal vector is zero");
     if (e[0] & EB_SHORT) fail(LMFF_ERR_SHORT, "more than %d Tersoff neighbours within cutmax", MAXSH);
     if (e[0] & EB_GRID) fail(LMFF_ERR_STATE, "bin grid too coarse for the neighbor stencil");
+    if (e[0] & EB_FIXED)
+      fail(LMFF_ERR_NONFINITE, "ILP pair terms beyond the fixed-point range (atoms of different molecules too close)");
   }
 
   // ---------------------------------------------------------------- thermo
@@ -1665,6 +1833,14 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
   const int64_t cells = (int64_t)s.nc[0] * s.nc[1] * s.nc[2];
   if (cells > (int64_t)1 << 28) fail(LMFF_ERR_BOX, "box needs %lld neighbor bins (limit 2^28)", (long long)cells);
   s.ncells = (int)cells;
+  // half-pair ILP tiles: 6 x 3 bins (about 18 x 18 A); the window around a tile (the stencil's
+  // reach on each side, rx <= 7 bins in x and ry <= 3 rows in y) must fit its table
+  const int rx = (int)std::ceil((cutneigh + 2e-6) / s.bs[0]) + 1, ry = (int)std::ceil(cutneigh / s.bs[1]) + 1;
+  s.tile[0] = HALF_TILE_X;
+  s.tile[1] = HALF_TILE_Y;
+  if ((s.tile[0] + 2 * rx) * (s.tile[1] + 2 * ry) >= WSTR) s.tile[0] = s.tile[1] = 0;
+  s.ntile[0] = s.tile[0] ? (s.nc[0] + s.tile[0] - 1) / s.tile[0] : 0;
+  s.ntile[1] = s.tile[1] ? (s.nc[1] + s.tile[1] - 1) / s.tile[1] : 0;
 }
 
 }  // namespace
@@ -2317,7 +2493,7 @@ int64_t lmff_system_bytes(lmff_ctx *ctx, int64_t natoms, int nlayers_hint) {
   // List rows are shared by every packed system, so a new system pays the context's current
   // capacities, or what its own stack needs if that is more.
   const int icap = std::max(c.icap, Ctx::ilp_row_estimate(c.cutneigh, std::max(nlayers_hint, 2)));
-  return (int64_t)std::ceil((double)natoms * Ctx::atom_bytes(std::max(c.scap, 64), icap));
+  return (int64_t)std::ceil((double)natoms * Ctx::atom_bytes(std::max(c.scap, 64), icap, std::max(nlayers_hint, 2)));
 }
 
 int lmff_debug_ilp_list(lmff_ctx *ctx, int sys_id, int64_t atom, int32_t *rows, int capacity, int *nout) {

@@ -51,7 +51,36 @@ enum ErrBits : int {
   EB_SHORT = 4,
   EB_NONFINITE = 8,
   EB_GRID = 16,       // bin grid coarser than the list build's stencil allows
+  EB_FIXED = 32,      // an ILP pair term beyond the range of the fixed-point partner sums
 };
+
+// Half-pair ILP: per ILP list row, the entries with z-bin offset dz < 0 come first, then dz = 0,
+// 1, 2, ... (the build's stencil order); seg[q] counts the entries with dz <= q - 1 (so the
+// entries of dz = d >= 0 are [seg[d], seg[d + 1])), for the front and back segments separately.
+constexpr int NSEG = 16;
+// window tables: per window (tile, partner z bin) the exclusive prefix of atoms over its bins
+// (row-major over (tile[1] + 2 ry) rows of (tile[0] + 2 rx) bins), the total at [bins]
+constexpr int WSTR = 256;
+// window atoms the half-pair kernel holds in LDS (six int64 sums each: 72 KB)
+constexpr int WCAP_MAX = 1536;
+// wavefronts per half-pair workgroup and the waves per SIMD the kernel is compiled for: two
+// workgroups fit a CU's LDS, so 8 waves give 4 per SIMD (128 VGPRs; 10-wave workgroups for FP32,
+// at 96 VGPRs, measured a third slower)
+template <typename R> struct HalfWaves {
+  static constexpr int n = 8;
+  static constexpr int per_simd = 4;
+};
+// partner sums in fixed point: units of 2^-44 eV/A (forces) and of the normal weight S, added as
+// exact 64-bit integers so that the order of the adds does not matter.  A term is converted by
+// one fma with 1.5 * 2^52 (its rounded integer lands in the low mantissa bits) and an integer
+// subtract; |term| < 2^7 keeps it exact to the unit (the sums hold 2^19).
+constexpr double FX_SCALE = 17592186044416.0;     // 2^44
+constexpr double FX_INV = 1.0 / 17592186044416.0;
+constexpr double FX_MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+constexpr unsigned long long FX_MAGIC_BITS = 0x4338000000000000ull;
+// a pair's partner terms are bounded by (rc + 2) max(|C|, |Tap a_ij|, |Tap a_ji|) < 2^7 when the
+// three coefficients stay below this (rc <= 18 A)
+constexpr double FX_COEF_MAX = 6.0;
 
 struct SysDev {
   double lo[3], prd[3];
@@ -60,6 +89,11 @@ struct SysDev {
   double glo[3], inv_bs[3];
   int cell_off;
   int row_off;        // first row of cells (fixed y and z bin) of this system
+  // tiles of the half-pair ILP kernel: tile[0] x tile[1] bins in x and y, ntile[] of them; one
+  // workgroup (and one window table) per tile and z bin, numbered from tile_off: (z ntile[1] +
+  // ty) ntile[0] + tx.  tile[0] == 0: no tiles (the system only takes the pull kernel).
+  int tile[2], ntile[2];
+  int tile_off;
   int begin, end;
   double dtv, dtf;
   int style;          // 0 = NVE, 1 = Langevin
@@ -124,6 +158,7 @@ struct DevPtrs {
   double ilp_splitsq;     // ILP rows hold entries with r^2 < ilp_splitsq (at build time) first
   double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
+  const int *cell_start;  // [cells + 1] first sorted atom of each cell (of the last list build)
   int tap_flag;
   int ntypes;             // atom types 1..ntypes
   int ters_rep_sym;       // repulsive Tersoff parameters equal in both orders of every type pair
@@ -132,6 +167,17 @@ struct DevPtrs {
   double tap_poly[5];
   // final integration fused into the force gather (the Verlet loop's plain and thermo steps)
   const double *rmass;    // [MAXT] mass per type
+  // half-pair ILP (systems with syshalf[s] != 0; all pointers null where no system has tiles)
+  uint16_t *lidx;         // [n][icap] window slot of each ILP entry with dz >= 0 (parallel to ilist)
+  uint16_t *segf, *segb;  // [n][NSEG] entries per z-bin offset, front and back segments
+  int *win;               // [windows][WSTR] window tables
+  int *winmax;            // [nsys] largest window of each system (atoms)
+  const int *syshalf;     // [nsys] 1: the system's ILP pairs run in the half-pair kernel
+  unsigned long long *acc;  // [6][n] partner sums (fx fy fz Sx Sy Sz) in fixed point
+  double *ipart;          // [npass][n][half_pstr] each atom's own sums per pass (z-bin offset)
+  int npass;              // passes of the half-pair kernel: z-bin offsets 0 .. npass - 1
+  int nsys;
+  int half_on;            // this evaluation runs the half-pair kernel for the half systems
 };
 
 // ---------------------------------------------------------------------------------------
@@ -241,10 +287,39 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Workgroups dealt to the XCDs in chunks: within each run of 8 K workgroups, XCD x takes K
+// consecutive ones, so the chunks follow the launch order (lower z bins of a stack first, the
+// short workgroups of the top bins last) while each XCD still works on neighbouring tiles.
+// Bijective on [0, gridDim.x) (a last partial chunk keeps the launch order).
+__device__ __forceinline__ int xcd_chunk_block(int K) {
+  const int b = blockIdx.x;
+  const int base = b / (8 * K) * (8 * K);
+  if ((int)gridDim.x - base < 8 * K) return b;
+  return base + (b % 8) * K + (b - base) / 8;
+}
+
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Wave sums of eight values at once: each of the first three exchanges halves the values a lane
+// carries (it keeps one half and adds its partner's copy of that half), the last three finish
+// within groups of eight lanes.  Lane l returns the sum of v[l >> 3]; ten exchanges instead of
+// the 48 of eight separate butterflies.
+__device__ __forceinline__ double wsum8(const double v[8], int lane) {
+  const bool u32 = (lane & 32) != 0, u16 = (lane & 16) != 0, u8 = (lane & 8) != 0;
+  double a[4], h[2];
+#pragma unroll
+  for (int k = 0; k < 4; k++) a[k] = (u32 ? v[4 + k] : v[k]) + __shfl_xor(u32 ? v[k] : v[4 + k], 32, 64);
+#pragma unroll
+  for (int k = 0; k < 2; k++) h[k] = (u16 ? a[2 + k] : a[k]) + __shfl_xor(u16 ? a[k] : a[2 + k], 16, 64);
+  double c = (u8 ? h[1] : h[0]) + __shfl_xor(u8 ? h[0] : h[1], 8, 64);
+  c += __shfl_xor(c, 4, 64);
+  c += __shfl_xor(c, 2, 64);
+  c += __shfl_xor(c, 1, 64);
+  return c;
 }
 
 // half selection of a full list by tag parity (pair_tersoff.cpp:150-158, ilp.cpp:488-496):
@@ -456,6 +531,82 @@ __device__ __forceinline__ int wrap_bin(int u, int nc, int &shift) {
   return w;
 }
 
+// Reach of the list build's stencil around an atom's bin: rows in y, bins in x.  The build and
+// the half-pair window tables use these same expressions, so every stencil run of an atom lies
+// inside the window of its tile.
+__device__ __forceinline__ int stencil_ry(const SysDev &s, double cut) { return (int)ceil(cut * s.inv_bs[1]); }
+__device__ __forceinline__ int stencil_rx(const SysDev &s, double cut) { return (int)ceil((cut + 2e-6) * s.inv_bs[0]); }
+
+// the last system whose numbered items (rows, tiles) start at or before r
+__device__ __forceinline__ int sys_of_tile(const SysDev *st, int nsys, int r) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (st[mid].tile_off <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Window tables of the half-pair ILP kernel, one wavefront per window (system, partner z bin,
+// tile).  A window is the rectangle of bins of one z bin that holds every stencil run of every
+// atom of the tile (rx bins and ry rows beyond the tile on each side, unwrapped: a periodic
+// bin reached twice appears twice, as two images).  win[w][b] = atoms in the window's bins
+// before bin b (row-major), win[w][bins] = its size; winmax[s] = the largest window of s.
+__global__ void k_win_table(int nwin, int nsys, const SysDev *st, const int *cell_start, double cut, int *win,
+                            int *winmax) {
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= nwin) return;
+  const int si = sys_of_tile(st, nsys, w);
+  const SysDev &s = st[si];
+  if (s.tile[0] <= 0) return;
+  const int loc = w - s.tile_off;
+  const int tx = loc % s.ntile[0], ty = (loc / s.ntile[0]) % s.ntile[1], zb = loc / (s.ntile[0] * s.ntile[1]);
+  const int rx = stencil_rx(s, cut), ry = stencil_ry(s, cut);
+  const int wxb = s.tile[0] + 2 * rx, wyb = s.tile[1] + 2 * ry, nb = wxb * wyb;
+  const int x0 = tx * s.tile[0] - rx, y0 = ty * s.tile[1] - ry;
+  int cnt[4], sum = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int b = 4 * lane + q;
+    cnt[q] = 0;
+    if (b < nb) {
+      const int k = b / wxb, u = b - k * wxb;
+      int uy = y0 + k, ux = x0 + u, sh;
+      bool ok = true;
+      if (s.per[1]) uy = wrap_bin(uy, s.nc[1], sh);
+      else ok = uy >= 0 && uy < s.nc[1];
+      if (s.per[0]) ux = wrap_bin(ux, s.nc[0], sh);
+      else ok = ok && ux >= 0 && ux < s.nc[0];
+      if (ok) {
+        const int c = s.cell_off + (zb * s.nc[1] + uy) * s.nc[0] + ux;
+        cnt[q] = cell_start[c + 1] - cell_start[c];
+      }
+    }
+    sum += cnt[q];
+  }
+  int scan = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int a = __shfl_up(scan, o, 64);
+    if (lane >= o) scan += a;
+  }
+  int run = scan - sum;
+  int *row = win + (size_t)w * WSTR;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int b = 4 * lane + q;
+    if (b < nb) row[b] = run;
+    run += cnt[q];
+  }
+  const int total = __shfl(scan, 63, 64);
+  if (lane == 0) {
+    row[nb] = total;
+    atomicMax(winmax + si, total);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Neighbor build (npair_full_bin_ghost.cpp:35-155 semantics: every image of every atom with
 // rsq <= (cut+skin)^2, evaluated in the reference's operation order with no fused
@@ -490,6 +641,11 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // offset of its first candidate in the wave's candidate sequence
   __shared__ uint32_t run_word[4][RUN_SLOTS];
   __shared__ int run_off[4][RUN_SLOTS + 1];
+  // half-pair tables: the window slot of a run's first atom (runs with dz > 0), the run's z-bin
+  // offset dz, and the wave's ILP entry counts per dz (front, back)
+  __shared__ uint16_t run_loc[4][RUN_SLOTS];
+  __shared__ int8_t run_dz[4][RUN_SLOTS];
+  __shared__ int seg_cnt[4][2][64];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + wv);
   const int lane = threadIdx.x & 63;
@@ -522,17 +678,33 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     if (lane == 0) atomicOr(P.err, EB_GRID);
     return;
   }
+  // the tile of i and the origin of its windows (unwrapped bins), when the system has tiles
+  const bool tiled = P.win != nullptr && s.tile[0] > 0;
+  const int rxs = stencil_rx(s, cut);
+  int win0 = 0, wx0 = 0, wy0 = 0, wxb = 1;
+  if (tiled) {
+    const int tx = c[0] / s.tile[0], ty = c[1] / s.tile[1];
+    win0 = s.tile_off + ty * s.ntile[0] + tx;  // + z bin x ntile[0] ntile[1]
+    wx0 = tx * s.tile[0] - rxs;
+    wy0 = ty * s.tile[1] - ry;
+    wxb = s.tile[0] + 2 * rxs;
+    seg_cnt[wv][0][lane] = 0;
+    seg_cnt[wv][1][lane] = 0;
+  }
   // ---- lane l: stencil rows l, l + 64 -> up to three runs each, laid end to end ----
   int total = 0, nruns = 0;  // candidates and runs laid out so far (wave-uniform)
   const float inv_wy = 1.0f / (float)wy;
   for (int it = 0; it < nrows; it += 64) {
     const int row = it + lane;
-    int nrun = 0, rb[RUNS_PER_ROW], rc_[RUNS_PER_ROW];
+    int nrun = 0, rb[RUNS_PER_ROW], rc_[RUNS_PER_ROW], rl[RUNS_PER_ROW];
     uint32_t rs[RUNS_PER_ROW];
+    int rdz = 0;
     if (row < nrows) {
       // row = qz * wy + qy (row < 128, wy <= 13: the float quotient is exact after truncation)
       const int qz = (int)(((float)row + 0.5f) * inv_wy), qy = row - qz * wy;
       int uy = c[1] + dylo + qy, uz = c[2] + dzlo + qz;
+      const int uy_unw = uy;
+      rdz = dzlo + qz;
       const double gy = fmax(0.0, bin_gap(pos[1], uy, s.nc[1], s.per[1], s.glo[1], bsy) - slack);
       const double gz = fmax(0.0, bin_gap(pos[2], uz, s.nc[2], s.per[2], s.glo[2], bsz) - slack);
       const double d2 = gy * gy + gz * gz;
@@ -543,7 +715,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
         const double w = sqrt(P.cutneighsq - d2) + slack;
         // bins in reach along x; in a periodic dimension at most nc + 2 of them (w < L/2), so
         // at most three images; in a non-periodic one the edge bins hold what lies beyond
-        const int rx = (int)ceil(cut * s.inv_bs[0]) + 1;
+        const int rx = rxs;
         int xlo = (int)fmax(floor((pos[0] - w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] - rx));
         int xhi = (int)fmin(floor((pos[0] + w - s.glo[0]) * s.inv_bs[0]), (double)(c[0] + rx));
         const int2 rm = row_mol[s.row_off + uz * s.nc[1] + uy];
@@ -564,6 +736,12 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
           xhi = min(max(xhi, 0), s.nc[0] - 1);
         }
         const int rowbase = s.cell_off + (uz * s.nc[1] + uy) * s.nc[0];
+        // window slots of this row (dz >= 0: the half-pair kernel computes the pair from i's side
+        // -- for dz = 0 only when j > i -- and its partner terms land in the window of (i's tile,
+        // uz))
+        const int *wrow = nullptr;
+        if (tiled && rdz >= 0)
+          wrow = P.win + (size_t)(win0 + uz * s.ntile[0] * s.ntile[1]) * WSTR + (uy_unw - wy0) * wxb;
         int u = xlo;
         while (u <= xhi && nrun < RUNS_PER_ROW) {
           int sx = 0, wx = u, uend = xhi;
@@ -574,6 +752,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
           const int a0 = cell_start[rowbase + wx], a1 = cell_start[rowbase + wx + (uend - u) + 1];
           rb[nrun] = a0;
           rc_[nrun] = a1 - a0;
+          rl[nrun] = wrow ? wrow[u - wx0] : 0;
           rs[nrun] = (uint32_t)((sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4)) << IDX_BITS;
           nrun++;
           u = uend + 1;
@@ -595,6 +774,8 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
       if (rc_[k] <= 0) continue;
       run_word[wv][slot] = (uint32_t)rb[k] | rs[k];
       run_off[wv][slot] = off;
+      run_loc[wv][slot] = (uint16_t)rl[k];
+      run_dz[wv][slot] = (int8_t)rdz;
       off += rc_[k];
       slot++;
     }
@@ -619,6 +800,8 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // every type i can meet has the ILP with ti: the per-candidate type test is moot
   const bool every_tj = (ilp_mask | ~((1u << P.ntypes) - 1u)) == 0xffffffffu;
   int r = 0;
+  // entry counts of the current z-bin offset (wave-uniform)
+  int cur_d = -64, cur_f = 0, cur_b = 0;
   for (int v0 = 0; v0 < total; v0 += 64) {
     // lanes past the end repeat the last candidate and take no entry, so the candidate test
     // below runs without a divergent branch
@@ -652,6 +835,37 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     const int pi = below(bi), po = below(bo), ps = below(bs), psb = below(bsb);
     if (in_i && ni + pi < P.icap) irow[ni + pi] = ent;
     if (in_o && no + po < P.icap) irow[P.icap - 1 - (no + po)] = ent;
+    if (tiled) {
+      // the entry's window slot (dz > 0) and the per-dz entry counts; dz does not decrease along
+      // the candidate sequence, so a chunk spans one or two values
+      const int dzv = run_dz[wv][r];
+      const uint16_t loc = dzv >= 0 ? (uint16_t)(run_loc[wv][r] + (v - run_off[wv][r])) : (uint16_t)0xffffu;
+      uint16_t *lrow = P.lidx + (size_t)i * P.icap;
+      if (in_i && ni + pi < P.icap) lrow[ni + pi] = loc;
+      if (in_o && no + po < P.icap) lrow[P.icap - 1 - (no + po)] = loc;
+      const int d_lo = __builtin_amdgcn_readfirstlane(dzv);
+      const int d_hi = __builtin_amdgcn_readfirstlane(__shfl(dzv, 63, 64));
+      if (d_lo == cur_d && d_hi == cur_d) {
+        cur_f += (int)__popcll(bi);
+        cur_b += (int)__popcll(bo);
+      } else {
+        for (int d = d_lo; d <= d_hi; d++) {
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(dzv == d);
+          if (d != cur_d) {
+            // dz never decreases, so the finished value's counts are final
+            if (cur_d > -64 && lane == 0) {
+              seg_cnt[wv][0][cur_d + 32] = cur_f;
+              seg_cnt[wv][1][cur_d + 32] = cur_b;
+            }
+            cur_d = d;
+            cur_f = 0;
+            cur_b = 0;
+          }
+          cur_f += (int)__popcll(bi & m);
+          cur_b += (int)__popcll(bo & m);
+        }
+      }
+    }
     // short list is stored transposed ([slot][atom]) so the per-step thread-per-atom
     // readers coalesce; it is split like the ILP rows (front from slot 0, back from scap-1)
     if (in_s && ns + ps < P.scap) P.slist[(size_t)(ns + ps) * P.n + i] = ent;
@@ -660,6 +874,27 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     no += __popcll(bo);
     ns += __popcll(bs);
     nsb += __popcll(bsb);
+  }
+  if (tiled) {
+    if (cur_d > -64 && lane == 0) {
+      seg_cnt[wv][0][cur_d + 32] = cur_f;
+      seg_cnt[wv][1][cur_d + 32] = cur_b;
+    }
+    // seg[q] = entries with dz <= q - 1: lane 31 + q after an inclusive scan over dz = lane - 32
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int cf = seg_cnt[wv][0][lane], cb = seg_cnt[wv][1][lane];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int a = __shfl_up(cf, o, 64), b = __shfl_up(cb, o, 64);
+      if (lane >= o) { cf += a; cb += b; }
+    }
+    const int q = lane - 31;
+    if (q >= 0 && q < NSEG) {
+      P.segf[(size_t)i * NSEG + q] = (uint16_t)min(cf, 0xffff);
+      P.segb[(size_t)i * NSEG + q] = (uint16_t)min(cb, 0xffff);
+    }
   }
   if (lane == 0) {
     const int front = ni < P.icap ? ni : P.icap;
@@ -991,6 +1226,8 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
   stage_table(s_ip, P.ip, MAXT * MAXT);
   __syncthreads();
   if (i >= P.n) return;
+  // atoms of systems the half-pair kernel takes in this evaluation
+  if (P.half_on && P.syshalf[P.sys[i]]) return;
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
   const double4 xi = P.x[i];
@@ -1244,6 +1481,397 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
       P.vatom[c * n + i] = va[c];
       P.T[(size_t)i * 6 + c] = tt[c];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Half-pair ILP (calc_FvdW + calc_FRep, ilp.cpp:449-675, each unordered pair once).  The pull
+// kernel above evaluates every pair from both of its atoms; here one workgroup takes the atoms
+// of one tile (tile[0] x tile[1] bins) of one z bin and evaluates each pair once, from the atom
+// whose z bin is lower (from the lower atom index within one z bin).  The terms acting on i go
+// into FP64 lane sums as in the pull kernel; the partner's terms (-F_ij and -Tap a_ji del) are
+// added in exact 64-bit fixed point to LDS sums over a window (the bins of the partner z bin
+// around the tile, one slot per atom image, slot from the list build), and each window goes out
+// to the per-atom sums acc[] with integer atomics when the pass over its z bin ends.  Integer
+// sums do not depend on the order of the adds, so results are reproducible bit for bit and do
+// not depend on the other systems of a launch.  k_ilp_combine then adds acc into fil and S.
+
+__device__ __forceinline__ unsigned long long fx_of(double v) {
+  return (unsigned long long)__double_as_longlong(fma(v, FX_SCALE, FX_MAGIC)) - FX_MAGIC_BITS;
+}
+
+// values of an atom's partial sums per pass: fx fy fz Sx Sy Sz, then e_vdw e_rep, then the virial
+__host__ __device__ constexpr int half_pstr(bool EF, bool VF) { return 6 + (EF || VF ? 2 : 0) + (VF ? 6 : 0); }
+
+template <typename R, bool EF, bool VF, bool SYM>
+__global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void k_ilp_half(DevPtrs<R> P, int wcap) {
+  using M = MathOps<R>;
+  constexpr int PSTR = half_pstr(EF, VF);
+  extern __shared__ unsigned long long s_acc[];  // [6][wcap]: fx fy fz Sx Sy Sz of the window's atoms
+  __shared__ IlpPair<R> s_ip[MAXT * MAXT];
+  constexpr int HALF_WAVES = HalfWaves<R>::n;
+  __shared__ uint32_t s_q[HALF_WAVES][128];
+  __shared__ int s_bin[WSTR];
+  __shared__ int s_row[4][2];
+  stage_table(s_ip, P.ip, MAXT * MAXT);
+  for (int p = threadIdx.x; p < 6 * wcap; p += blockDim.x) s_acc[p] = 0;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = xcd_chunk_block(16);
+  const int si = sys_of_tile(P.sys_tab, P.nsys, b);
+  const SysDev &s = P.sys_tab[si];
+  const int ntt = s.ntile[0] * s.ntile[1];
+  const int loc = b - s.tile_off;
+  const bool mine = s.tile[0] > 0 && P.syshalf[si] && loc >= 0 && loc < ntt * s.nc[2];
+  const int tx = mine ? loc % s.ntile[0] : 0, ty = mine ? (loc / s.ntile[0]) % s.ntile[1] : 0;
+  const int za = mine ? loc / ntt : 0;
+  const int x0 = tx * s.tile[0], x1 = min(x0 + s.tile[0], s.nc[0]);
+  if (threadIdx.x < 4) {
+    int a0 = 0, cnt = 0;
+    const int yy = ty * s.tile[1] + (int)threadIdx.x;
+    if (mine && (int)threadIdx.x < s.tile[1] && yy < s.nc[1]) {
+      const int base = s.cell_off + (za * s.nc[1] + yy) * s.nc[0];
+      a0 = P.cell_start[base + x0];
+      cnt = P.cell_start[base + x1] - a0;
+    }
+    s_row[threadIdx.x][0] = a0;
+    s_row[threadIdx.x][1] = cnt;
+  }
+  __syncthreads();
+  const int n0 = s_row[0][1], n1 = n0 + s_row[1][1], n2 = n1 + s_row[2][1], ntile_atoms = n2 + s_row[3][1];
+  if (ntile_atoms == 0) return;  // the whole workgroup (nothing here or not a half system)
+  auto tile_atom = [&](int k) {
+    return k < n0 ? s_row[0][0] + k : k < n1 ? s_row[1][0] + (k - n0) : k < n2 ? s_row[2][0] + (k - n1)
+                                                                                 : s_row[3][0] + (k - n2);
+  };
+  const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
+  const int far = *P.farflag;
+  const double cut = sqrt(P.cutneighsq);
+  const int rx = stencil_rx(s, cut), ry = stencil_ry(s, cut);
+  const int wxb = s.tile[0] + 2 * rx, nbins = wxb * (s.tile[1] + 2 * ry);
+  const int rz = (int)ceil(cut * s.inv_bs[2]);
+  const int dzmax = min(rz, NSEG - 2);
+  const size_t n = (size_t)P.n;
+  const int icap = P.icap;
+  constexpr bool LANE32 = sizeof(R) == 4 && !VF;
+  R cmax = 0;  // largest |C|, |Tap a_ij|, |Tap a_ji| of this lane (fixed-point range check)
+  // an atom's sums of one pass go to its own slot of ipart (pass-major), stored once by lanes
+  // 0..PSTR-1 (no read back: k_ilp_combine adds the passes in order); passes without work
+  // store zeros
+  auto put = [&](int i, int dz, double v) {
+    if (lane < PSTR) P.ipart[((size_t)dz * n + (size_t)i) * PSTR + lane] = v;
+  };
+  // the sums of eight values (wsum8) stored by lanes 0, 8, ..: value c + off at slot c + off
+  auto put8 = [&](int i, int dz, double v, int off) {
+    const int c = off + (lane >> 3);
+    if ((lane & 7) == 0 && c < PSTR) P.ipart[((size_t)dz * n + (size_t)i) * PSTR + c] = v;
+  };
+  for (int dz = 0; dz < P.npass; dz++) {
+    int zb = za + dz;
+    bool live = dz <= dzmax;
+    if (zb >= s.nc[2]) {
+      if (s.per[2]) zb -= s.nc[2];
+      else live = false;
+    }
+    const int w = live ? s.tile_off + zb * ntt + ty * s.ntile[0] + tx : 0;
+    // does any atom of the tile have entries at this dz?
+    int any = 0;
+    if (live) {
+      const int *wt = P.win + (size_t)w * WSTR;
+      for (int q = threadIdx.x; q <= nbins; q += blockDim.x) s_bin[q] = wt[q];
+      for (int k = wv; k < ntile_atoms; k += HALF_WAVES) {
+        const int i = tile_atom(k);
+        const uint16_t *sf = P.segf + (size_t)i * NSEG, *sb = P.segb + (size_t)i * NSEG;
+        any |= (sf[dz + 1] > sf[dz]) || (far && sb[dz + 1] > sb[dz]);
+      }
+    }
+    if (!__syncthreads_or(any)) {
+      for (int k = wv; k < ntile_atoms; k += HALF_WAVES) put(tile_atom(k), dz, 0.0);
+      continue;
+    }
+    for (int k = wv; k < ntile_atoms; k += HALF_WAVES) {
+      const int i = __builtin_amdgcn_readfirstlane(tile_atom(k));
+      const uint16_t *sf = P.segf + (size_t)i * NSEG, *sb = P.segb + (size_t)i * NSEG;
+      const int f0 = sf[dz], f1 = sf[dz + 1];
+      const int b0 = far ? (int)sb[dz] : 0, b1 = far ? (int)sb[dz + 1] : 0;
+      if (f1 <= f0 && b1 <= b0) {
+        put(i, dz, 0.0);
+        continue;
+      }
+      const double4 xi = P.x[i];
+      const int ti = (int)xi.w - 1;
+      const auto ni4 = P.nrm[i];
+      const R nix = ni4.x, niy = ni4.y, niz = ni4.z;
+      const int itag = P.tag[i];
+      const uint32_t *row = P.ilist + (size_t)i * icap;
+      const uint16_t *lrow = P.lidx + (size_t)i * icap;
+      double fx = 0, fy = 0, fz = 0, sx = 0, sy = 0, sz = 0, ev = 0, er = 0;
+      double wv6[6] = {0, 0, 0, 0, 0, 0};
+      R fxl = 0, fyl = 0, fzl = 0, sxl = 0, syl = 0, szl = 0;
+      using N4 = typename std::conditional<sizeof(R) == 8, double4, float4>::type;
+      auto terms = [&](const uint32_t e, const uint32_t slot, const double4 xj, const N4 nj4) {
+        const int j = e_idx(e);
+        if (dz == 0 && j <= i) return;  // within one z bin the lower index takes the pair
+        double gx, gy, gz;
+        if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
+          gx = xj.x;
+          gy = xj.y;
+          gz = xj.z;
+        } else {
+          ghost(xj, e, prd, gx, gy, gz);
+        }
+        const double dxd = xi.x - gx, dyd = xi.y - gy, dzd = xi.z - gz;
+        const R dx = (R)dxd, dy = (R)dyd, dzz = (R)dzd;
+        R rsq;
+        if (!ilp_cut_test<R>(dxd, dyd, dzd, P.ilp_cutsq, rsq)) return;
+        const int tj = (int)xj.w - 1;
+        const IlpPair<R> &pij = s_ip[ti * MAXT + tj];
+        const IlpPair<R> &pji = s_ip[tj * MAXT + ti];
+        const R rinv = M::rsqrt(rsq);
+        const R r = rsq * rinv;
+        const Tap<R> tp = calc_tap<R>(r, P.tap_poly, P.rcut, P.tap_flag);
+        // van der Waals with the parameters of the atom the reference gives the pair to
+        const IlpPair<R> *pv = &pij;
+        if (!SYM && !pij.vsym && tag_skip(itag, P.tag[j], xi.x, xi.y, xi.z, gx, gy, gz)) pv = &pji;
+        const R r2inv = rinv * rinv;
+        const R edamp = M::ex(-pv->d * (r * pv->inv_seff - (R)1));
+        const R TSinv = M::rcp((R)1 + edamp);
+        const R w6 = r2inv * r2inv * r2inv * TSinv;
+        const R Vvdw = -pv->C6 * w6;
+        const R fpv = r2inv * w6 * (pv->C6_d_seff * r * edamp * TSinv - (R)6 * pv->C6);
+        // repulsion (i,j) with n_i and (j,i) with n_j
+        const R exp0 = M::ex(-pij.lambda * (r - pij.z0));
+        const R pn1 = nix * dx + niy * dy + niz * dzz;
+        const R rd1 = (rsq - pn1 * pn1) * pij.d2inv;
+        const R frho1 = M::ex(-rd1) * pij.C;
+        const R V1 = exp0 * (pij.half_eps + frho1);
+        const R fq1 = (R)2 * exp0 * frho1 * pij.d2inv;
+        const R a1 = pn1 * fq1;
+        const R exp0b = (SYM || pij.esym) ? exp0 : M::ex(-pji.lambda * (r - pji.z0));
+        const R pn2 = -(nj4.x * dx + nj4.y * dy + nj4.z * dzz);
+        const R rd2 = (rsq - pn2 * pn2) * pji.d2inv;
+        const R frho2 = M::ex(-rd2) * pji.C;
+        const R V2 = exp0b * (pji.half_eps + frho2);
+        const R fq2 = (R)2 * exp0b * frho2 * pji.d2inv;
+        const R a2 = pn2 * fq2;
+        const R ta1 = a1 * tp.t, ta2 = a2 * tp.t;
+        const R Cd = tp.t * (fpv + rinv * (pij.lambda * V1 + pji.lambda * V2) + fq1 + fq2) -
+                     tp.dt * rinv * (Vvdw + V1 + V2);
+        // force of the pair on i: C del + Tap (a_ji n_j - a_ij n_i); on j its negative
+        const R Fx = Cd * dx + (ta2 * nj4.x - ta1 * nix);
+        const R Fy = Cd * dy + (ta2 * nj4.y - ta1 * niy);
+        const R Fz = Cd * dzz + (ta2 * nj4.z - ta1 * niz);
+        if (LANE32) {
+          fxl += Fx; fyl += Fy; fzl += Fz;
+          sxl += ta1 * dx; syl += ta1 * dy; szl += ta1 * dzz;
+        } else {
+          fx += (double)Fx; fy += (double)Fy; fz += (double)Fz;
+          sx += (double)(ta1 * dx); sy += (double)(ta1 * dy); sz += (double)(ta1 * dzz);
+        }
+        // j's terms: -F and S_j += Tap a_ji del_ji = -Tap a_ji del
+        unsigned long long *pa = s_acc + slot;
+        __hip_atomic_fetch_add(pa, fx_of(-(double)Fx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + wcap, fx_of(-(double)Fy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 2 * wcap, fx_of(-(double)Fz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 3 * wcap, fx_of(-(double)(ta2 * dx)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 4 * wcap, fx_of(-(double)(ta2 * dy)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 5 * wcap, fx_of(-(double)(ta2 * dzz)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        cmax = fmax(cmax, fmax(fabs(Cd), fmax(fabs(ta1), fabs(ta2))));
+        if (EF) {
+          ev += (double)(Vvdw * tp.t);
+          er += (double)(tp.t * V1) + (double)(tp.t * V2);
+        }
+        if (VF) {
+          // del (x) F_ij: the pair's share of the F.r virial (both atoms' halves)
+          wv6[0] += dxd * (double)Fx;
+          wv6[1] += dyd * (double)Fy;
+          wv6[2] += dzd * (double)Fz;
+          wv6[3] += dxd * (double)Fy;
+          wv6[4] += dxd * (double)Fz;
+          wv6[5] += dyd * (double)Fz;
+        }
+      };
+      // front entries of this dz, pipelined as in the pull kernel
+      {
+        const int cnt = f1 - f0;
+        const uint32_t *fr = row + f0;
+        const uint16_t *fl = lrow + f0;
+        uint32_t e_n1 = 0, e_n2 = 0;
+        uint32_t l_n1 = 0, l_n2 = 0;
+        double4 x_n1 = {0, 0, 0, 0};
+        N4 n_n1 = {0, 0, 0, 0};
+        if (lane < cnt) { e_n1 = fr[lane]; l_n1 = fl[lane]; }
+        if (lane + 64 < cnt) { e_n2 = fr[lane + 64]; l_n2 = fl[lane + 64]; }
+        // FP32: the partner's normal is fetched a chunk ahead with its position (FP64: in the
+        // chunk, which measured faster at the FP64 kernel's register budget)
+        constexpr bool NPRE = sizeof(R) == 4;
+        if (lane < cnt) {
+          x_n1 = P.x[e_idx(e_n1)];
+          if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+        }
+        for (int k2 = lane; k2 < cnt; k2 += 64) {
+          const uint32_t e = e_n1, l = l_n1;
+          const double4 xj = x_n1;
+          const N4 nj = n_n1;
+          e_n1 = e_n2;
+          l_n1 = l_n2;
+          if (k2 + 64 < cnt) {
+            x_n1 = P.x[e_idx(e_n1)];
+            if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+          }
+          if (k2 + 128 < cnt) { e_n2 = fr[k2 + 128]; l_n2 = fl[k2 + 128]; }
+          terms(e, l, xj, NPRE ? nj : P.nrm[e_idx(e)]);
+        }
+      }
+      // back entries of this dz (appended from slot icap - 1 downwards): tested first, the ones
+      // within rc queued and run in batches of 64
+      if (b1 > b0) {
+        int qn = 0;
+        auto drain = [&]() {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          const int take = qn < 64 ? qn : 64;
+          const uint32_t qs = s_q[wv][lane];
+          const uint32_t rest = lane + 64 < qn ? s_q[wv][lane + 64] : 0u;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          if (lane + 64 < qn) s_q[wv][lane] = rest;
+          qn -= take;
+          if (lane < take) {
+            const uint32_t e = row[qs];
+            terms(e, lrow[qs], P.x[e_idx(e)], P.nrm[e_idx(e)]);
+          }
+        };
+        for (int k0 = b0; k0 < b1; k0 += 64) {
+          const int k2 = k0 + lane;
+          bool in = false;
+          uint32_t slot = 0;
+          if (k2 < b1) {
+            slot = (uint32_t)(icap - 1 - k2);
+            const uint32_t e = row[slot];
+            const double4 xj = P.x[e_idx(e)];
+            double gx, gy, gz;
+            ghost(xj, e, prd, gx, gy, gz);
+            R r2;
+            in = ilp_cut_test<R>(xi.x - gx, xi.y - gy, xi.z - gz, P.ilp_cutsq, r2);
+          }
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+          const int below =
+              (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          if (in) s_q[wv][qn + below] = slot;
+          qn += (int)__popcll(m);
+          if (qn >= 64) drain();
+        }
+        if (qn > 0) drain();
+      }
+      if (LANE32) {
+        fx += (double)fxl; fy += (double)fyl; fz += (double)fzl;
+        sx += (double)sxl; sy += (double)syl; sz += (double)szl;
+      }
+      {
+        const double v8[8] = {fx, fy, fz, sx, sy, sz, ev, er};
+        put8(i, dz, wsum8(v8, lane), 0);
+      }
+      if (VF) {
+        const double w8[8] = {wv6[0], wv6[1], wv6[2], wv6[3], wv6[4], wv6[5], 0.0, 0.0};
+        put8(i, dz, wsum8(w8, lane), 8);
+      }
+    }
+    // the window's partner sums go out to acc[] (integer adds: any order gives the same sums)
+    __syncthreads();
+    const int wsize = s_bin[nbins];
+    for (int p = threadIdx.x; p < wsize; p += blockDim.x) {
+      unsigned long long vq[6];
+      bool nz = false;
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        vq[c] = s_acc[c * wcap + p];
+        nz |= vq[c] != 0ull;
+        s_acc[c * wcap + p] = 0ull;
+      }
+      if (!nz) continue;
+      // the bin of slot p: the last bin whose first slot is at or before p (it holds p)
+      int lo = 0, hi = nbins - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_bin[mid] <= p) lo = mid;
+        else hi = mid - 1;
+      }
+      const int kr = lo / wxb, u = lo - kr * wxb;
+      int uy = ty * s.tile[1] - ry + kr, ux = tx * s.tile[0] - rx + u, sh;
+      if (s.per[1]) uy = wrap_bin(uy, s.nc[1], sh);
+      if (s.per[0]) ux = wrap_bin(ux, s.nc[0], sh);
+      const int atom = P.cell_start[s.cell_off + (zb * s.nc[1] + uy) * s.nc[0] + ux] + (p - s_bin[lo]);
+#pragma unroll
+      for (int c = 0; c < 6; c++)
+        if (vq[c] != 0ull)
+          __hip_atomic_fetch_add(P.acc + (size_t)c * n + atom, vq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+  // fixed-point range: |term| < 2^7 needs the coefficients below FX_COEF_MAX
+  double cm = (double)cmax;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cm = fmax(cm, __shfl_xor(cm, o, 64));
+  if (lane == 0 && !(cm < FX_COEF_MAX)) atomicOr(P.err, EB_FIXED);
+}
+
+// The ILP force, S and tallies of every atom of a half system: its pass sums in pass order plus
+// the partner sums (which are then cleared for the next evaluation); with the virial, S is now
+// whole, so the virial of the normal-derivative forces of the atom's normal is added here (as the
+// pull kernel's tail adds it).
+template <typename R, bool EF, bool VF>
+__global__ void k_ilp_combine(DevPtrs<R> P) {
+  constexpr int PSTR = half_pstr(EF, VF);
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= P.n) return;
+  if (!P.syshalf[P.sys[a]]) return;
+  const size_t n = (size_t)P.n;
+  double v[PSTR];
+#pragma unroll
+  for (int c = 0; c < PSTR; c++) v[c] = 0.0;
+  for (int dz = 0; dz < P.npass; dz++) {
+    const double *q = P.ipart + ((size_t)dz * n + (size_t)a) * PSTR;
+#pragma unroll
+    for (int c = 0; c < PSTR; c++) v[c] += q[c];
+  }
+  long long q[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    q[c] = (long long)P.acc[c * n + a];
+    P.acc[c * n + a] = 0ull;
+  }
+  const double4 f = {v[0] + (double)q[0] * FX_INV, v[1] + (double)q[1] * FX_INV, v[2] + (double)q[2] * FX_INV, 0.0};
+  const double4 S4 = {v[3] + (double)q[3] * FX_INV, v[4] + (double)q[4] * FX_INV, v[5] + (double)q[5] * FX_INV, 0.0};
+  P.fil[a] = f;
+  P.S[a] = S4;
+  if (EF || VF) {
+    P.th[0 * n + a] = v[6];
+    P.th[1 * n + a] = v[7];
+  }
+  if (VF) {
+    const uint32_t *nrow = P.nnb + (size_t)a * 4;
+    const int ci = (int)nrow[0];
+    double w[6] = {v[8], v[9], v[10], v[11], v[12], v[13]};
+    for (int kk = 0; kk < ci; kk++) {
+      const R *D = P.dn + (size_t)a * 27 + kk * 9;
+      const double *vt = P.vet + (size_t)a * 9 + kk * 3;
+      double Fk[3];
+      for (int c = 0; c < 3; c++)
+        Fk[c] = -((double)D[0 * 3 + c] * S4.x + (double)D[1 * 3 + c] * S4.y + (double)D[2 * 3 + c] * S4.z);
+      w[0] += vt[0] * Fk[0];
+      w[1] += vt[1] * Fk[1];
+      w[2] += vt[2] * Fk[2];
+      w[3] += vt[0] * Fk[1];
+      w[4] += vt[0] * Fk[2];
+      w[5] += vt[1] * Fk[2];
+    }
+    for (int c = 0; c < 6; c++) P.th[(3 + c) * n + a] = w[c];
   }
 }
 

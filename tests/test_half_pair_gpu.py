@@ -1,0 +1,106 @@
+"""The half-pair ILP kernel on the GPU.  It evaluates every interlayer pair once, from the atom
+in the lower z bin (from the lower atom index within one z bin), and sums the partner's terms
+in exact 64-bit fixed point; the pull kernel evaluates every pair from both of its atoms.  Both
+are checked here against each other (FP64 1e-12 of the RMS force, mixed 1e-6 eV/A RMS) and the
+half-pair kernel against the oracle (1e-10) on geometries that reach each of its paths:
+partner z bins above the atom (stacks, bilayers, a periodic z), pairs within one z bin
+(corrugated sheets, whose atoms of two molecules share z bins), periodic images inside a
+window (a twisted bilayer in a small cell) and the back segment of the lists (atoms moved past
+the build's split margin).  LMFF_HALF=0 in the environment of a new context keeps the pull
+kernel."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from lmff import capi, structures as S
+from test_gpu_parity import _check_fp64, ctx_for, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _evaluate(s, precision, half, eflag=1, vflag=2, x=None):
+    old = os.environ.get("LMFF_HALF")
+    os.environ["LMFF_HALF"] = "1" if half else "0"
+    try:
+        with ctx_for(s, precision=precision) as ctx:
+            r = ctx.pair_compute(s, eflag, vflag)
+            if x is not None:
+                # the same list, positions moved (the list is reused)
+                r = ctx.pair_compute(s, eflag, vflag, new_list=0, x=x)
+            return r
+    finally:
+        if old is None:
+            del os.environ["LMFF_HALF"]
+        else:
+            os.environ["LMFF_HALF"] = old
+
+
+def _corrugated():
+    """An AB bilayer bent along x (amplitude 2 A, one wavelength per box): each sheet spans
+    several z bins, so atoms of the two molecules share z bins (dz = 0 pairs) and meet at
+    negative offsets too."""
+    s = S.layered_stack(30, 20, ["G", "BN"], seed=74)
+    lx = s.boxhi[0] - s.boxlo[0]
+    s.x[:, 2] += 2.0 * np.sin(2.0 * math.pi * (s.x[:, 0] - s.boxlo[0]) / lx)
+    return s
+
+
+def _periodic_z():
+    s = S.layered_stack(15, 9, ["G", "BN"] * 6, seed=75, jitter=0.05)
+    s.boxlo[2], s.boxhi[2] = -1.675, 12 * 3.35 - 1.675
+    s.periodic = (1, 1, 1)
+    return s
+
+
+CASES = {
+    "bilayer": lambda: S.layered_stack(30, 20, ["G", "BN"], seed=71),
+    "ten_layer": lambda: S.layered_stack(15, 9, ["G" if k % 2 == 0 else "BN" for k in range(10)], seed=72),
+    "periodic_z": _periodic_z,
+    "corrugated": _corrugated,
+    # 2,016 atoms in a 45 x 59 A cell: a window (18 x 9 bins) wraps onto the cell's own bins
+    "twisted_small_cell": lambda: S.twisted_bilayer(4, 1, seed=73, min_atoms=2000, max_atoms=8000),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_half_pair_equals_pull(name):
+    s = CASES[name]()
+    print(f"half-pair vs pull ILP kernel, {name}: {s.n} atoms", flush=True)
+    a = _evaluate(s, capi.LMFF_FP64, True)
+    b = _evaluate(s, capi.LMFF_FP64, False)
+    rms = math.sqrt((b["f"] ** 2).sum(1).mean())
+    assert np.abs(a["f"] - b["f"]).max() <= 1e-12 * rms
+    # the kernels add in different orders: bitwise equal forces would mean one path ran twice
+    assert not np.array_equal(a["f"], b["f"])
+    for k in ("energy", "e_vdw", "e_rep"):
+        assert abs(a[k] - b[k]) <= 1e-12 * abs(b[k]), k
+    assert np.abs(a["virial"] - b["virial"]).max() <= 1e-12 * np.abs(b["virial"]).max()
+    am = _evaluate(s, capi.LMFF_MIXED, True)
+    bm = _evaluate(s, capi.LMFF_MIXED, False)
+    assert math.sqrt(((am["f"] - bm["f"]) ** 2).sum(1).mean()) <= 1e-6
+    assert abs(am["energy"] - bm["energy"]) <= 1e-7 * abs(bm["energy"])
+
+
+@pytest.mark.parametrize("name", ["corrugated", "periodic_z", "twisted_small_cell"])
+def test_half_pair_vs_oracle(name):
+    s = CASES[name]()
+    print(f"half-pair ILP kernel vs the oracle, {name}: {s.n} atoms", flush=True)
+    r = oracle_for(s).compute(1, 2)
+    g = _evaluate(s, capi.LMFF_FP64, True)
+    _check_fp64(g, r)
+
+
+def test_half_pair_back_segment_vs_oracle():
+    """The middle sheet slid by 0.5 A after the list build raises the far flag: entries of the
+    back segment (beyond rc + 0.6 A at the build) are tested and the pairs that came within rc
+    run through the queue of the half-pair kernel."""
+    s = S.layered_stack(30, 20, ["G", "BN", "G"], seed=76)
+    x1 = s.x.copy()
+    x1[s.mol == 2] += np.array([0.4, 0.3, 0.0])
+    g = _evaluate(s, capi.LMFF_FP64, True, x=x1)
+    moved = s.copy()
+    moved.x = x1
+    r = oracle_for(moved).compute(1, 2)
+    _check_fp64(g, r)
