@@ -191,6 +191,9 @@ struct Ctx {
   // half-pair ILP: window slots of list entries, per-dz segments, window tables, per-system
   // largest window and mode, partner sums
   DBuf lidx, segf, segb, win, winmax, syshalf, acc, ipart;
+  // full ILP rows for evaluations by the pull kernel while half systems hold half rows
+  DBuf ilist_full, icnt_full, ocnt_full;
+  int icap_full = 0;
   int npass = 0;               // z-bin offsets the half-pair kernel walks (0 .. npass - 1)
   int ntiles = 0;              // tile x z-bin workgroups (= windows) over every system
   std::vector<int> half_sys;   // per system: 1 when its ILP pairs run in the half-pair kernel
@@ -244,7 +247,7 @@ struct Ctx {
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &row_mol, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
-                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart};
+                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart, &ilist_full, &icnt_full, &ocnt_full};
     for (DBuf *b : all) b->tally = &bytes_in_use;
     const char *h = std::getenv("LMFF_HALF");
     if (h && std::strcmp(h, "0") == 0) half_enabled = false;
@@ -797,18 +800,17 @@ struct Ctx {
 
   // Which systems run their ILP pairs in the half-pair kernel: those with tiles whose largest
   // window fits the kernel's LDS, whose stencil reaches at most NSEG - 2 z bins up, and whose
-  // list rows fit the 16-bit segment counts.  Decided per system from its own geometry, so a
-  // system's results do not depend on what else is packed with it.
-  void choose_half(const std::vector<int> &wm) {
-    std::vector<int> hs(sys.size(), 0);
+  // list rows fit the 16-bit segment counts.  Decided per system from its own geometry (on the
+  // device, k_half_decide, before the list build, which keeps half rows for those systems), so a
+  // system's results do not depend on what else is packed with it; here the host takes the
+  // decisions back and sizes the launches.
+  void choose_half(const std::vector<int> &wm, const std::vector<int> &hs) {
     int wc = 0, np = 0;
     const double cut = std::sqrt(cutneigh * cutneigh);
     for (size_t k = 0; k < sys.size(); k++) {
       const SysHost &q = sys[k];
       const int rz = (int)std::ceil(cut * (1.0 / q.bs[2]));
-      if (half_enabled && have_ilp && q.tile[0] > 0 && q.n > 0 && wm[k] > 0 && wm[k] <= WCAP_MAX &&
-          rz + 1 <= NSEG - 2 && icap < 0xffff) {
-        hs[k] = 1;
+      if (hs[k]) {
         wc = std::max(wc, wm[k]);
         // passes: z-bin offsets 0 .. rz (within the bins in a non-periodic z)
         np = std::max(np, q.per[2] ? rz + 1 : std::min(rz + 1, q.nc[2]));
@@ -835,9 +837,6 @@ struct Ctx {
       npass = np;
       half_any = any;
       pull_any = pull;
-      if (!hs.empty())
-        HIPCHK(hipMemcpyAsync(syshalf.p, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice, stream));
-      HIPCHK(hipStreamSynchronize(stream));
       gen++;  // the launches of a captured step change
     }
   }
@@ -848,17 +847,24 @@ struct Ctx {
       ilist.ensure((size_t)n * std::max(icap, 1) * sizeof(uint32_t));
       if (ntiles > 0) lidx.ensure((size_t)n * std::max(icap, 1) * sizeof(uint16_t));
       HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int) * 4, stream));
+      if (ntiles > 0)
+        hipLaunchKernelGGL(k_half_decide, dim3(((int)sys.size() + BS - 1) / BS), dim3(BS), 0, stream,
+                           (int)sys.size(), (const SysDev *)systab.as<SysDev>(), (const int *)winmax.as<int>(),
+                           std::sqrt(cutneigh * cutneigh), (half_enabled && have_ilp && icap < 0xffff) ? 1 : 0,
+                           syshalf.as<int>());
       auto P = ptrs<double>();
       const int nb = (int)((n + 3) / 4);
       hipLaunchKernelGGL(k_build_lists<double>, dim3(nb), dim3(BS), 0, stream, P, (const int *)cell_start.as<int>(),
                          (const int2 *)row_mol.as<int2>());
       int o[4];
       HIPCHK(hipMemcpyAsync(o, ovf.p, sizeof o, hipMemcpyDeviceToHost, stream));
-      std::vector<int> wm(sys.size(), 0);
-      if (ntiles > 0 && !sys.empty())
+      std::vector<int> wm(sys.size(), 0), hs(sys.size(), 0);
+      if (ntiles > 0 && !sys.empty()) {
         HIPCHK(hipMemcpyAsync(wm.data(), winmax.p, sizeof(int) * sys.size(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(hs.data(), syshalf.p, sizeof(int) * sys.size(), hipMemcpyDeviceToHost, stream));
+      }
       HIPCHK(hipStreamSynchronize(stream));
-      if (o[0] <= 0 && o[1] <= 0) choose_half(wm);
+      if (o[0] <= 0 && o[1] <= 0) choose_half(wm, hs);
       bool grow = false;
       if (o[0] > 0) {
         scap = ((int)(o[0] * 1.25) + 63) / 64 * 64;
@@ -897,8 +903,43 @@ struct Ctx {
   }
 
   template <typename R>
-  void force_impl(int ef, int vf, int va, int integ, int want_ke, int half) {
-    force_kernels<R>(ptrs<R>(), ef, vf, va, integ, want_ke, half);
+  void force_impl(int ef, int vf, int va, int integ, int want_ke, int half, bool full_rows) {
+    DevPtrs<R> P = ptrs<R>();
+    if (full_rows) {
+      P.ilist = ilist_full.as<uint32_t>();
+      P.icnt = icnt_full.as<int>();
+      P.ocnt = ocnt_full.as<int>();
+      P.icap = icap_full;
+    }
+    force_kernels<R>(P, ef, vf, va, integ, want_ke, half);
+  }
+
+  // Full ILP rows of every atom for an evaluation by the pull kernel while the lists of the half
+  // systems hold half rows (per-atom tallies), from the cell ranges of the last build: an atom
+  // has moved less than half the skin since then, so every pair within rc now lies in the
+  // stencil of the atoms' current bins (the rows are exact within rc, not within rc + skin).
+  void build_full_rows() {
+    if (icap_full < 2 * icap) icap_full = ((2 * icap) + 63) / 64 * 64;
+    for (int attempt = 0; attempt < 4; attempt++) {
+      ilist_full.ensure((size_t)n * icap_full * sizeof(uint32_t));
+      icnt_full.ensure((size_t)n * sizeof(int));
+      ocnt_full.ensure((size_t)n * sizeof(int));
+      HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int) * 4, stream));
+      auto P = ptrs<double>();
+      P.ilist = ilist_full.as<uint32_t>();
+      P.icnt = icnt_full.as<int>();
+      P.ocnt = ocnt_full.as<int>();
+      P.icap = icap_full;
+      P.ilp_only = 1;
+      hipLaunchKernelGGL(k_build_lists<double>, dim3((int)((n + 3) / 4)), dim3(BS), 0, stream, P,
+                         (const int *)cell_start.as<int>(), (const int2 *)row_mol.as<int2>());
+      int o[4];
+      HIPCHK(hipMemcpyAsync(o, ovf.p, sizeof o, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (o[1] <= 0) return;
+      icap_full = ((int)(o[1] * 1.15) + 64 + 63) / 64 * 64;
+    }
+    fail(LMFF_ERR_MEMORY, "neighbor list capacity did not converge");
   }
 
   template <typename R, bool EF, bool VF, bool SYM>
@@ -994,8 +1035,10 @@ struct Ctx {
     // per-atom energy or virial: the pull kernel tallies both halves of every pair as the
     // reference's ev_tally does; the half-pair kernel gives a pair's energy to one atom
     const int half = (half_any && !va && !(eflag & 2)) ? 1 : 0;
-    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va, integ, want_ke, half);
-    else force_impl<double>(ef, vf, va, integ, want_ke, half);
+    const bool full_rows = half_any && !half;
+    if (full_rows) build_full_rows();
+    if (precision == LMFF_MIXED) force_impl<float>(ef, vf, va, integ, want_ke, half, full_rows);
+    else force_impl<double>(ef, vf, va, integ, want_ke, half, full_rows);
     last_eflag = ef ? (eflag | 1) : 0;
     last_vflag = va ? 4 | 2 : (vf ? 2 : 0);
     forces_valid = true;

@@ -159,6 +159,7 @@ struct DevPtrs {
   double short_splitsq;   // short lists likewise hold entries with r^2 < short_splitsq first
   const int *farflag;     // set once an atom has moved more than half the split margin since the build
   const int *cell_start;  // [cells + 1] first sorted atom of each cell (of the last list build)
+  int ilp_only;           // list build: full ILP rows only (no short lists, segments or window slots)
   int tap_flag;
   int ntypes;             // atom types 1..ntypes
   int ters_rep_sym;       // repulsive Tersoff parameters equal in both orders of every type pair
@@ -607,6 +608,19 @@ __global__ void k_win_table(int nwin, int nsys, const SysDev *st, const int *cel
   }
 }
 
+// Which systems run their ILP pairs in the half-pair kernel (one thread per system), decided
+// before the list build from each system's own geometry: tiles, every window within the LDS
+// window of the kernel, the stencil within NSEG - 2 z bins.  allow = 0 keeps every system in the
+// pull kernel.
+__global__ void k_half_decide(int nsys, const SysDev *st, const int *winmax, double cut, int allow, int *syshalf) {
+  const int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nsys) return;
+  const SysDev &s = st[si];
+  const int rz = (int)ceil(cut * s.inv_bs[2]);
+  syshalf[si] = (allow && s.tile[0] > 0 && s.end > s.begin && winmax[si] > 0 && winmax[si] <= WCAP_MAX &&
+                 rz + 1 <= NSEG - 2) ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------------------
 // Neighbor build (npair_full_bin_ghost.cpp:35-155 semantics: every image of every atom with
 // rsq <= (cut+skin)^2, evaluated in the reference's operation order with no fused
@@ -643,8 +657,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   __shared__ int run_off[4][RUN_SLOTS + 1];
   // half-pair tables: the window slot of a run's first atom (runs with dz > 0), the run's z-bin
   // offset dz, and the wave's ILP entry counts per dz (front, back)
-  __shared__ uint16_t run_loc[4][RUN_SLOTS];
-  __shared__ int8_t run_dz[4][RUN_SLOTS];
+  __shared__ uint32_t run_loc[4][RUN_SLOTS];  // window slot | (dz + 32) << 16
   __shared__ int seg_cnt[4][2][64];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + wv);
@@ -679,7 +692,11 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     return;
   }
   // the tile of i and the origin of its windows (unwrapped bins), when the system has tiles
-  const bool tiled = P.win != nullptr && s.tile[0] > 0;
+  const bool tiled = P.win != nullptr && s.tile[0] > 0 && !P.ilp_only;
+  // half rows: a system in the half-pair kernel keeps only the ILP entries with dz >= 0 (the
+  // partner in a lower z bin computes the rest); the rows below i's z bin are scanned only for
+  // the short list
+  const bool halfrows = tiled && P.syshalf[P.sys[i]] != 0;
   const int rxs = stencil_rx(s, cut);
   int win0 = 0, wx0 = 0, wy0 = 0, wxb = 1;
   if (tiled) {
@@ -721,8 +738,9 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
         const int2 rm = row_mol[s.row_off + uz * s.nc[1] + uy];
         if (rm.x > rm.y) {
           xhi = xlo - 1;  // no atoms in this row of bins
-        } else if (rm.x == mi && rm.y == mi) {
-          // only atoms of i's own molecule: just the short lists can take entries from this row
+        } else if ((rm.x == mi && rm.y == mi) || (halfrows && rdz < 0)) {
+          // only atoms of i's own molecule (or, in half rows, a row below i's z bin): just the
+          // short lists can take entries from this row
           if (d2 <= P.shortsq) {
             const double ws = sqrt(P.shortsq - d2) + slack;
             xlo = max(xlo, (int)floor((pos[0] - ws - s.glo[0]) * s.inv_bs[0]));
@@ -774,8 +792,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
       if (rc_[k] <= 0) continue;
       run_word[wv][slot] = (uint32_t)rb[k] | rs[k];
       run_off[wv][slot] = off;
-      run_loc[wv][slot] = (uint16_t)rl[k];
-      run_dz[wv][slot] = (int8_t)rdz;
+      run_loc[wv][slot] = ((uint32_t)rl[k] & 0xffffu) | ((uint32_t)(rdz + 32) << 16);
       off += rc_[k];
       slot++;
     }
@@ -820,10 +837,16 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     const bool take = live && !((ent >> IDX_BITS) == 0x15u && j == i);
     bool type_ok = true;
     if (!every_tj) type_ok = ((ilp_mask >> ((int)xj.w - 1)) & 1u) != 0;
-    const bool ilp = take && (rsq <= P.cutneighsq) && (mj != mi) && type_ok;
+    uint32_t rlw = 0;
+    int dzv = 0;
+    if (tiled) {
+      rlw = run_loc[wv][r];
+      dzv = (int)(rlw >> 16) - 32;
+    }
+    const bool ilp = take && (rsq <= P.cutneighsq) && (mj != mi) && type_ok && !(halfrows && dzv < 0);
     const bool near_i = rsq < P.ilp_splitsq;
     const bool in_i = ilp && near_i, in_o = ilp && !near_i;
-    const bool sht = take && (rsq <= P.shortsq);
+    const bool sht = take && (rsq <= P.shortsq) && !P.ilp_only;
     const bool near_s = rsq < P.short_splitsq;
     const bool in_s = sht && near_s, in_sb = sht && !near_s;
     const unsigned long long bi = __builtin_amdgcn_ballot_w64(in_i), bo = __builtin_amdgcn_ballot_w64(in_o),
@@ -838,13 +861,12 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     if (tiled) {
       // the entry's window slot (dz > 0) and the per-dz entry counts; dz does not decrease along
       // the candidate sequence, so a chunk spans one or two values
-      const int dzv = run_dz[wv][r];
-      const uint16_t loc = dzv >= 0 ? (uint16_t)(run_loc[wv][r] + (v - run_off[wv][r])) : (uint16_t)0xffffu;
+      const uint16_t loc = dzv >= 0 ? (uint16_t)(rlw + (uint32_t)(v - run_off[wv][r])) : (uint16_t)0xffffu;
       uint16_t *lrow = P.lidx + (size_t)i * P.icap;
       if (in_i && ni + pi < P.icap) lrow[ni + pi] = loc;
       if (in_o && no + po < P.icap) lrow[P.icap - 1 - (no + po)] = loc;
       const int d_lo = __builtin_amdgcn_readfirstlane(dzv);
-      const int d_hi = __builtin_amdgcn_readfirstlane(__shfl(dzv, 63, 64));
+      const int d_hi = __builtin_amdgcn_readlane(dzv, 63);
       if (d_lo == cur_d && d_hi == cur_d) {
         cur_f += (int)__popcll(bi);
         cur_b += (int)__popcll(bo);
@@ -903,7 +925,7 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     ni += no;
     const int sfront = ns < P.scap ? ns : P.scap;
     const int sback = nsb < P.scap - sfront ? nsb : P.scap - sfront;
-    P.scnt[i] = sfront | (sback << 16);
+    if (!P.ilp_only) P.scnt[i] = sfront | (sback << 16);
     ns += nsb;
     // only an overflowing row reports its size (a per-wave atomic on one address would
     // serialize every wave of the launch)
@@ -2788,15 +2810,22 @@ __global__ void k_pair_stats(DevPtrs<R> P, unsigned long long *out) {
   const int nfront = P.icnt[i], nback = P.ocnt[i];
   const int cnt = nfront + nback, back_off = P.icap - nback - nfront;
   const uint32_t *row = P.ilist + (size_t)i * P.icap;
-  unsigned long long np = 0;
+  // half rows hold each pair with dz > 0 once (counted for both of its ordered pairs) and the
+  // pairs within one z bin from both atoms
+  const bool half = P.syshalf && P.syshalf[P.sys[i]] && !P.ilp_only;
+  const int f1 = half ? P.segf[(size_t)i * NSEG + 1] : 0, b1 = half ? P.segb[(size_t)i * NSEG + 1] : 0;
+  unsigned long long np = 0, ne = 0;
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = row[ilp_slot(k, nfront, back_off)];
+    // back entries lie in memory in reverse order of their append index
+    const unsigned w = (!half || (k < nfront ? k < f1 : (nback - 1 - (k - nfront)) < b1)) ? 1u : 2u;
     double gx, gy, gz;
     ghost(P.x[e_idx(e)], e, s.prd, gx, gy, gz);
     const double dx = xi.x - gx, dy = xi.y - gy, dz = xi.z - gz;
-    if (dx * dx + dy * dy + dz * dz < P.ilp_cutsq) np++;
+    ne += w;
+    if (dx * dx + dy * dy + dz * dz < P.ilp_cutsq) np += w;
   }
-  atomicAdd(&out[0], (unsigned long long)cnt);
+  atomicAdd(&out[0], ne);
   atomicAdd(&out[1], np);
   atomicAdd(&out[2], (unsigned long long)P.tcnt[i]);
   atomicAdd(&out[3], (unsigned long long)((P.scnt[i] & 0xffff) + (P.scnt[i] >> 16)));

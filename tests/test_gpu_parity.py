@@ -186,15 +186,19 @@ def test_neighbor_sets_bit_exact(geometry):
     print(f"neighbor sets, {geometry}: {s.n} atoms", flush=True)
     o = oracle_for(s)
     offs, ent, sh = o.neighbor_list()
+    want = {(i, int(j), *map(int, sh[e])) for i in range(s.n)
+            for e, j in zip(range(offs[i], offs[i + 1]), ent[offs[i]:offs[i + 1]]) if s.mol[j] != s.mol[i]}
     with ctx_for(s) as ctx:
         g = ctx.pair_compute(s, 1, 2)
-        for i in np.random.default_rng(2).choice(s.n, 64, replace=False):
-            rows = ctx.ilp_list(0, int(i))
-            want = sorted((int(j), *map(int, sh[e])) for e, j in zip(range(offs[i], offs[i + 1]),
-                                                                       ent[offs[i]:offs[i + 1]])
-                          if s.mol[j] != s.mol[i])
-            got = sorted(map(tuple, rows.tolist()))
-            assert got == want, i
+        # every atom's ILP row: the rows of a half-pair system hold each pair once (from the atom
+        # in the lower z bin; within one z bin from both atoms), so the entries with their mirror
+        # images (j, i, -shift) must make up the reference's full list exactly
+        got = set()
+        for i in range(s.n):
+            for j, sx, sy, sz in ctx.ilp_list(0, i).tolist():
+                got.add((i, j, sx, sy, sz))
+                got.add((j, i, -sx, -sy, -sz))
+    assert got == want
     # the short lists (Tersoff, normals) through the forces of every atom
     _check_fp64(g, o.compute(1, 2))
 
