@@ -280,14 +280,20 @@ def precision_runs(s, steps, warmup, drift_steps, profile=True, host_repeat=1):
 
 
 def roofline_ilp(stats, ilp_ms, natoms, tag):
-    """Roofline of the dominant kernel, the FP64 ILP pair kernel: algorithmic FLOP per launch
-    (120 per ordered pair within the cutoff: exp, sqrt and divide count 1 each) over its mean
-    launch time from HIP events on the engine's stream.  Algorithmic bytes per launch: the
-    4-byte list entries streamed plus 128 B per atom (x, normal, force and S in and out)."""
+    """Roofline of the dominant kernel, the FP64 ILP pair evaluation: the half-pair kernel
+    k_ilp_half<double> (each pair once, partner terms in fixed point) with k_ilp_combine (the
+    partner sums into each atom), timed together by the HIP events around the ILP step on the
+    engine's stream.  Algorithmic FLOP per launch: 120 per ordered pair within the cutoff (exp,
+    sqrt and divide count 1 each): the repulsion and taper of each direction (about 102) and half
+    of the pair's van der Waals term, which is what the half-pair kernel evaluates once per
+    unordered pair (both repulsion directions, one van der Waals term).  Algorithmic bytes per
+    launch: 4 B per (ordered) list entry plus 128 B per atom (x, normal, force and S in and out)."""
     flop = stats["ilp_pairs"] * ILP_PAIR_FLOP
     achieved = flop / (ilp_ms * 1e-3) / 1e12
-    traffic = profiled_traffic("k_ilp<double, false, false, false", tag)
-    return {"bound": "valu", "kernel": "k_ilp<double> (FP64 ILP pair, force-only variant)",
+    t_half = profiled_traffic("k_ilp_half<double, false, false", tag)
+    t_comb = profiled_traffic("k_ilp_combine<double, false, false", tag)
+    traffic = t_half + t_comb if t_half is not None and t_comb is not None else None
+    return {"bound": "valu", "kernel": "k_ilp_half<double> + k_ilp_combine<double> (FP64 ILP pairs, force-only variants)",
             "achieved": achieved, "peak": FP64_VALU_PEAK, "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK,
             "traffic": traffic,
             "traffic_source": (f"profiles/hbm_traffic_{tag}.md (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "

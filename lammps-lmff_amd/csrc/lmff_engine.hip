@@ -666,6 +666,8 @@ struct Ctx {
     P.acc = acc.as<unsigned long long>();
     P.ipart = ipart.as<double>();
     P.npass = npass;
+    P.fx_scale = FX_SCALE;
+    P.fx_magic = FX_MAGIC;
     P.nsys = (int)sys.size();
     P.half_on = 0;
     scalars(P);

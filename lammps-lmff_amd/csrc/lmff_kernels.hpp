@@ -177,6 +177,7 @@ struct DevPtrs {
   unsigned long long *acc;  // [6][n] partner sums (fx fy fz Sx Sy Sz) in fixed point
   double *ipart;          // [npass][n][half_pstr] each atom's own sums per pass (z-bin offset)
   int npass;              // passes of the half-pair kernel: z-bin offsets 0 .. npass - 1
+  double fx_scale, fx_magic;  // FX_SCALE and FX_MAGIC (launch arguments, see fx_of)
   int nsys;
   int half_on;            // this evaluation runs the half-pair kernel for the half systems
 };
@@ -1518,8 +1519,11 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
 // sums do not depend on the order of the adds, so results are reproducible bit for bit and do
 // not depend on the other systems of a launch.  k_ilp_combine then adds acc into fil and S.
 
-__device__ __forceinline__ unsigned long long fx_of(double v) {
-  return (unsigned long long)__double_as_longlong(fma(v, FX_SCALE, FX_MAGIC)) - FX_MAGIC_BITS;
+// scale and magic are passed at run time (equal to FX_SCALE and FX_MAGIC), so that the compiler
+// keeps them in registers across the pair loop: one fma and an add to the high word per value
+// (as compile-time constants the magic was rebuilt by two moves before every fma)
+__device__ __forceinline__ unsigned long long fx_of(double v, double scale, double magic) {
+  return (unsigned long long)__double_as_longlong(fma(v, scale, magic)) - FX_MAGIC_BITS;
 }
 
 // values of an atom's partial sums per pass: fx fy fz Sx Sy Sz, then e_vdw e_rep, then the virial
@@ -1577,6 +1581,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
   const int icap = P.icap;
   constexpr bool LANE32 = sizeof(R) == 4 && !VF;
   R cmax = 0;  // largest |C|, |Tap a_ij|, |Tap a_ji| of this lane (fixed-point range check)
+  const double fxs = P.fx_scale, fxm = P.fx_magic;
   // an atom's sums of one pass go to its own slot of ipart (pass-major), stored once by lanes
   // 0..PSTR-1 (no read back: k_ilp_combine adds the passes in order); passes without work
   // store zeros
@@ -1692,14 +1697,14 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
         }
         // j's terms: -F and S_j += Tap a_ji del_ji = -Tap a_ji del
         unsigned long long *pa = s_acc + slot;
-        __hip_atomic_fetch_add(pa, fx_of(-(double)Fx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + wcap, fx_of(-(double)Fy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 2 * wcap, fx_of(-(double)Fz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 3 * wcap, fx_of(-(double)(ta2 * dx)), __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(pa, fx_of(-(double)Fx, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + wcap, fx_of(-(double)Fy, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 2 * wcap, fx_of(-(double)Fz, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(pa + 3 * wcap, fx_of(-(double)(ta2 * dx), fxs, fxm), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 4 * wcap, fx_of(-(double)(ta2 * dy)), __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(pa + 4 * wcap, fx_of(-(double)(ta2 * dy), fxs, fxm), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 5 * wcap, fx_of(-(double)(ta2 * dzz)), __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(pa + 5 * wcap, fx_of(-(double)(ta2 * dzz), fxs, fxm), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
         cmax = fmax(cmax, fmax(fabs(Cd), fmax(fabs(ta1), fabs(ta2))));
         if (EF) {
