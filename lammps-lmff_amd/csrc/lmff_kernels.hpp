@@ -1538,7 +1538,9 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
   constexpr int HALF_WAVES = HalfWaves<R>::n;
   __shared__ uint32_t s_q[HALF_WAVES][128];
   __shared__ int s_bin[WSTR];
+  __shared__ int s_batom[WSTR];  // first atom of each window bin (bins that hold atoms)
   __shared__ int s_row[4][2];
+  __shared__ int s_next;  // the next tile atom of the pass to hand out
   stage_table(s_ip, P.ip, MAXT * MAXT);
   for (int p = threadIdx.x; p < 6 * wcap; p += blockDim.x) s_acc[p] = 0;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1603,9 +1605,21 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
     const int w = live ? s.tile_off + zb * ntt + ty * s.ntile[0] + tx : 0;
     // does any atom of the tile have entries at this dz?
     int any = 0;
+    if (threadIdx.x == 0) s_next = HALF_WAVES;
     if (live) {
       const int *wt = P.win + (size_t)w * WSTR;
-      for (int q = threadIdx.x; q <= nbins; q += blockDim.x) s_bin[q] = wt[q];
+      for (int q = threadIdx.x; q <= nbins; q += blockDim.x) {
+        const int w0 = wt[q];
+        s_bin[q] = w0;
+        if (q < nbins && wt[q + 1] > w0) {
+          // a bin that holds atoms lies in the grid (k_win_table counts no other)
+          const int kr = q / wxb, u = q - kr * wxb;
+          int uy = ty * s.tile[1] - ry + kr, ux = tx * s.tile[0] - rx + u, sh;
+          if (s.per[1]) uy = wrap_bin(uy, s.nc[1], sh);
+          if (s.per[0]) ux = wrap_bin(ux, s.nc[0], sh);
+          s_batom[q] = P.cell_start[s.cell_off + (zb * s.nc[1] + uy) * s.nc[0] + ux];
+        }
+      }
       for (int k = wv; k < ntile_atoms; k += HALF_WAVES) {
         const int i = tile_atom(k);
         const uint16_t *sf = P.segf + (size_t)i * NSEG, *sb = P.segb + (size_t)i * NSEG;
@@ -1616,7 +1630,16 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
       for (int k = wv; k < ntile_atoms; k += HALF_WAVES) put(tile_atom(k), dz, 0.0);
       continue;
     }
-    for (int k = wv; k < ntile_atoms; k += HALF_WAVES) {
+    // the waves take the tile's atoms one at a time from a shared counter, so that a wave with
+    // long rows does not hold the others at the barrier that ends the pass (which wave takes an
+    // atom does not change any result: an atom's own sums come from its row alone, and the
+    // partner sums are integers)
+    auto next_atom = [&]() {
+      int t = 0;
+      if (lane == 0) t = atomicAdd(&s_next, 1);
+      return __builtin_amdgcn_readfirstlane(t);
+    };
+    for (int k = wv; k < ntile_atoms; k = next_atom()) {
       const int i = __builtin_amdgcn_readfirstlane(tile_atom(k));
       const uint16_t *sf = P.segf + (size_t)i * NSEG, *sb = P.segb + (size_t)i * NSEG;
       const int f0 = sf[dz], f1 = sf[dz + 1];
@@ -1829,11 +1852,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
         if (s_bin[mid] <= p) lo = mid;
         else hi = mid - 1;
       }
-      const int kr = lo / wxb, u = lo - kr * wxb;
-      int uy = ty * s.tile[1] - ry + kr, ux = tx * s.tile[0] - rx + u, sh;
-      if (s.per[1]) uy = wrap_bin(uy, s.nc[1], sh);
-      if (s.per[0]) ux = wrap_bin(ux, s.nc[0], sh);
-      const int atom = P.cell_start[s.cell_off + (zb * s.nc[1] + uy) * s.nc[0] + ux] + (p - s_bin[lo]);
+      const int atom = s_batom[lo] + (p - s_bin[lo]);
 #pragma unroll
       for (int c = 0; c < 6; c++)
         if (vq[c] != 0ull)
