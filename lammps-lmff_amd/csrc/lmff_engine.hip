@@ -41,6 +41,19 @@ namespace {
 constexpr int CHUNK = 4096;
 constexpr int BS = 256;
 constexpr int HALF_TILE_X = 6, HALF_TILE_Y = 3;  // bins per half-pair ILP tile
+
+// Systems below this many atoms keep the pull ILP kernel (no tiles): a half-pair workgroup makes
+// one pass per partner z bin, each with its loads, barriers and flush in a row, so a small system
+// waits on that chain (about 90 us per step for a 3,000- to 76,000-atom bilayer against 11-100 us
+// in the pull kernel), and a packed stream of such systems also ran slower with it.
+// LMFF_HALF_MIN_ATOMS in the environment sets another bound (0: every system that fits).
+int64_t half_min_atoms() {
+  static const int64_t v = [] {
+    const char *e = std::getenv("LMFF_HALF_MIN_ATOMS");
+    return e ? std::max<int64_t>(0, std::atoll(e)) : (int64_t)100000;
+  }();
+  return v;
+}
 constexpr int MAX_DEVICES = 64;
 
 // device address of `scratch` per GPU, looked up once per device with that device current
@@ -1883,7 +1896,7 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
   const int rx = (int)std::ceil((cutneigh + 2e-6) / s.bs[0]) + 1, ry = (int)std::ceil(cutneigh / s.bs[1]) + 1;
   s.tile[0] = HALF_TILE_X;
   s.tile[1] = HALF_TILE_Y;
-  if ((s.tile[0] + 2 * rx) * (s.tile[1] + 2 * ry) >= WSTR) s.tile[0] = s.tile[1] = 0;
+  if ((s.tile[0] + 2 * rx) * (s.tile[1] + 2 * ry) >= WSTR || n < half_min_atoms()) s.tile[0] = s.tile[1] = 0;
   s.ntile[0] = s.tile[0] ? (s.nc[0] + s.tile[0] - 1) / s.tile[0] : 0;
   s.ntile[1] = s.tile[1] ? (s.nc[1] + s.tile[1] - 1) / s.tile[1] : 0;
 }

@@ -692,12 +692,13 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
     if (lane == 0) atomicOr(P.err, EB_GRID);
     return;
   }
-  // the tile of i and the origin of its windows (unwrapped bins), when the system has tiles
-  const bool tiled = P.win != nullptr && s.tile[0] > 0 && !P.ilp_only;
+  // the tile of i and the origin of its windows (unwrapped bins), for a system in the half-pair
+  // kernel (the pull kernel reads neither window slots nor segments)
+  const bool tiled = P.win != nullptr && s.tile[0] > 0 && !P.ilp_only && P.syshalf[P.sys[i]] != 0;
   // half rows: a system in the half-pair kernel keeps only the ILP entries with dz >= 0 (the
   // partner in a lower z bin computes the rest); the rows below i's z bin are scanned only for
   // the short list
-  const bool halfrows = tiled && P.syshalf[P.sys[i]] != 0;
+  const bool halfrows = tiled;
   const int rxs = stencil_rx(s, cut);
   int win0 = 0, wx0 = 0, wy0 = 0, wxb = 1;
   if (tiled) {
