@@ -1072,7 +1072,9 @@ struct Ctx {
     if (e[0] & EB_SHORT) fail(LMFF_ERR_SHORT, "more than %d Tersoff neighbours within cutmax", MAXSH);
     if (e[0] & EB_GRID) fail(LMFF_ERR_STATE, "bin grid too coarse for the neighbor stencil");
     if (e[0] & EB_FIXED)
-      fail(LMFF_ERR_NONFINITE, "ILP pair terms beyond the fixed-point range (atoms of different molecules too close)");
+      fail(LMFF_ERR_NONFINITE,
+           "ILP pair terms beyond the fixed-point range of the half-pair kernel (atoms of different molecules "
+           "closer than about 1.7 A); LMFF_HALF=0 in the environment evaluates such configurations");
   }
 
   // ---------------------------------------------------------------- thermo
