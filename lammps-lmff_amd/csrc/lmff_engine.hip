@@ -556,7 +556,7 @@ struct Ctx {
     // per ILP list entry (uint16), window tables (WSTR ints per 6 x 3-bin tile and z bin, about
     // 60 atoms per table in a stack of sheets with an empty z bin between some of them)
     const double half = 48.0 + 4.0 * NSEG + 2.0 * (double)icap_ + 4.0 * WSTR / 60.0 +
-                        8.0 * half_pstr(true, true) * std::min(8, std::max(2, nlayers + 1));  // + pass sums
+                        8.0 * half_pstr(true, true);  // + the atoms' own sums
     return d4 + i32 + f64 + u32 + sort + half + 4.0 * ((double)scap_ + (double)icap_);
   }
   // ILP row capacity the list build settles on for the worst atom of an AB-stacked pile of
@@ -839,10 +839,10 @@ struct Ctx {
     wc = (wc + 63) / 64 * 64;
     if (any) {
       half_attrs();  // here, never inside a graph capture
-      // pass sums at the widest layout (with the virial), so that no later evaluation moves the
-      // buffer under a captured graph
-      if (ipart.bytes < sizeof(double) * (size_t)np * (size_t)n * half_pstr(true, true)) {
-        ipart.ensure(sizeof(double) * (size_t)np * (size_t)n * half_pstr(true, true));
+      // the atoms' own sums at the widest layout (with the virial), so that no later evaluation
+      // moves the buffer under a captured graph
+      if (ipart.bytes < sizeof(double) * (size_t)n * half_pstr(true, true)) {
+        ipart.ensure(sizeof(double) * (size_t)n * half_pstr(true, true));
         gen++;
       }
     }

@@ -175,7 +175,7 @@ struct DevPtrs {
   int *winmax;            // [nsys] largest window of each system (atoms)
   const int *syshalf;     // [nsys] 1: the system's ILP pairs run in the half-pair kernel
   unsigned long long *acc;  // [6][n] partner sums (fx fy fz Sx Sy Sz) in fixed point
-  double *ipart;          // [npass][n][half_pstr] each atom's own sums per pass (z-bin offset)
+  double *ipart;          // [n][half_pstr] each atom's own sums, added pass by pass
   int npass;              // passes of the half-pair kernel: z-bin offsets 0 .. npass - 1
   double fx_scale, fx_magic;  // FX_SCALE and FX_MAGIC (launch arguments, see fx_of)
   int nsys;
@@ -1585,16 +1585,20 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
   constexpr bool LANE32 = sizeof(R) == 4 && !VF;
   R cmax = 0;  // largest |C|, |Tap a_ij|, |Tap a_ji| of this lane (fixed-point range check)
   const double fxs = P.fx_scale, fxm = P.fx_magic;
-  // an atom's sums of one pass go to its own slot of ipart (pass-major), stored once by lanes
-  // 0..PSTR-1 (no read back: k_ilp_combine adds the passes in order); passes without work
-  // store zeros
-  auto put = [&](int i, int dz, double v) {
-    if (lane < PSTR) P.ipart[((size_t)dz * n + (size_t)i) * PSTR + lane] = v;
+  // an atom's own sums go to its slot of ipart: the first pass (dz = 0) stores them (zeros when
+  // it has no entries there), every later pass with entries for the atom adds its sums to them.
+  // One workgroup makes all passes of its tile's atoms, and the barrier at the end of a pass
+  // orders the passes, so the sums are added in pass order whichever wave takes the atom.
+  auto put0 = [&](int i, int dz) {
+    if (dz == 0 && lane < PSTR) P.ipart[(size_t)i * PSTR + lane] = 0.0;
   };
-  // the sums of eight values (wsum8) stored by lanes 0, 8, ..: value c + off at slot c + off
+  // the sums of eight values (wsum8), lanes 0, 8, ..: value c + off at slot c + off
   auto put8 = [&](int i, int dz, double v, int off) {
     const int c = off + (lane >> 3);
-    if ((lane & 7) == 0 && c < PSTR) P.ipart[((size_t)dz * n + (size_t)i) * PSTR + c] = v;
+    if ((lane & 7) == 0 && c < PSTR) {
+      double *q = P.ipart + (size_t)i * PSTR + c;
+      *q = dz == 0 ? v : *q + v;
+    }
   };
   for (int dz = 0; dz < P.npass; dz++) {
     int zb = za + dz;
@@ -1628,7 +1632,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
       }
     }
     if (!__syncthreads_or(any)) {
-      for (int k = wv; k < ntile_atoms; k += HALF_WAVES) put(tile_atom(k), dz, 0.0);
+      for (int k = wv; k < ntile_atoms; k += HALF_WAVES) put0(tile_atom(k), dz);
       continue;
     }
     // the waves take the tile's atoms one at a time from a shared counter, so that a wave with
@@ -1646,7 +1650,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
       const int f0 = sf[dz], f1 = sf[dz + 1];
       const int b0 = far ? (int)sb[dz] : 0, b1 = far ? (int)sb[dz + 1] : 0;
       if (f1 <= f0 && b1 <= b0) {
-        put(i, dz, 0.0);
+        put0(i, dz);
         continue;
       }
       const double4 xi = P.x[i];
@@ -1881,12 +1885,7 @@ __global__ void k_ilp_combine(DevPtrs<R> P) {
   const size_t n = (size_t)P.n;
   double v[PSTR];
 #pragma unroll
-  for (int c = 0; c < PSTR; c++) v[c] = 0.0;
-  for (int dz = 0; dz < P.npass; dz++) {
-    const double *q = P.ipart + ((size_t)dz * n + (size_t)a) * PSTR;
-#pragma unroll
-    for (int c = 0; c < PSTR; c++) v[c] += q[c];
-  }
+  for (int c = 0; c < PSTR; c++) v[c] = P.ipart[(size_t)a * PSTR + c];
   long long q[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) {
