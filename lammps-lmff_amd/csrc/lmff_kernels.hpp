@@ -1527,7 +1527,7 @@ __device__ __forceinline__ unsigned long long fx_of(double v, double scale, doub
   return (unsigned long long)__double_as_longlong(fma(v, scale, magic)) - FX_MAGIC_BITS;
 }
 
-// values of an atom's partial sums per pass: fx fy fz Sx Sy Sz, then e_vdw e_rep, then the virial
+// values of an atom's own sums: fx fy fz Sx Sy Sz, then e_vdw e_rep, then the virial
 __host__ __device__ constexpr int half_pstr(bool EF, bool VF) { return 6 + (EF || VF ? 2 : 0) + (VF ? 6 : 0); }
 
 template <typename R, bool EF, bool VF, bool SYM>
