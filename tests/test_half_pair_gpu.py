@@ -54,6 +54,14 @@ def _periodic_z():
     return s
 
 
+def _open_box():
+    """A G/BN bilayer in a box open in x and y as well (no periodic images anywhere): windows
+    stop at the grid's edges and the edge bins hold the atoms beyond them."""
+    s = S.layered_stack(24, 14, ["G", "BN"], seed=78, jitter=0.04)
+    s.periodic = (0, 0, 0)
+    return s
+
+
 CASES = {
     "bilayer": lambda: S.layered_stack(30, 20, ["G", "BN"], seed=71),
     "ten_layer": lambda: S.layered_stack(15, 9, ["G" if k % 2 == 0 else "BN" for k in range(10)], seed=72),
@@ -61,6 +69,7 @@ CASES = {
     "corrugated": _corrugated,
     # 2,016 atoms in a 45 x 59 A cell: a window (18 x 9 bins) wraps onto the cell's own bins
     "twisted_small_cell": lambda: S.twisted_bilayer(4, 1, seed=73, min_atoms=2000, max_atoms=8000),
+    "open_box": _open_box,
 }
 
 
@@ -83,7 +92,7 @@ def test_half_pair_equals_pull(name):
     assert abs(am["energy"] - bm["energy"]) <= 1e-7 * abs(bm["energy"])
 
 
-@pytest.mark.parametrize("name", ["corrugated", "periodic_z", "twisted_small_cell"])
+@pytest.mark.parametrize("name", ["corrugated", "periodic_z", "twisted_small_cell", "open_box"])
 def test_half_pair_vs_oracle(name):
     s = CASES[name]()
     print(f"half-pair ILP kernel vs the oracle, {name}: {s.n} atoms", flush=True)
