@@ -1059,12 +1059,21 @@ struct Ctx {
     forces_valid = true;
   }
 
-  void check_errors() {
+  // retry_ok: an overflow of the half-pair kernel's fixed-point partner sums (and nothing
+  // else) moves this context to the pull kernel for good (full rows at the next build, which it
+  // requests) and returns true, so that the caller repeats its evaluation; otherwise every error
+  // bit ends the call with its message.
+  bool check_errors(bool retry_ok = false) {
     int e[4];
     HIPCHK(hipMemcpyAsync(e, errw.p, sizeof e, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    if (e[0] == 0) return;
+    if (e[0] == 0) return false;
     HIPCHK(hipMemset(errw.p, 0, sizeof(int) * 4));
+    if (retry_ok && e[0] == EB_FIXED && half_enabled) {
+      half_enabled = false;
+      need_rebuild = true;
+      return true;
+    }
     if (e[0] & EB_NONFINITE) fail(LMFF_ERR_NONFINITE, "Non-numeric atom coords - simulation unstable");
     if (e[0] & EB_NORMAL_COUNT)
       fail(LMFF_ERR_NORMAL, "There are too many neighbors for some atoms, please check your configuration");
@@ -1075,6 +1084,7 @@ struct Ctx {
       fail(LMFF_ERR_NONFINITE,
            "ILP pair terms beyond the fixed-point range of the half-pair kernel (atoms of different molecules "
            "closer than about 1.7 A); LMFF_HALF=0 in the environment evaluates such configurations");
+    return false;
   }
 
   // ---------------------------------------------------------------- thermo
@@ -1124,14 +1134,17 @@ struct Ctx {
     auto eval = [&]() {  // energy, then f.f, f.h, f.g, max|h_c|, max|f|^2, max|f_c| of the new forces
       ago++;
       if (rebuild_due()) rebuild();
-      force(1, 0);
-      reduce(3, out.as<double>());
-      hipLaunchKernelGGL(k_cg_stats, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p,
-                         (const double4 *)cg_g.p, (const double4 *)cg_h.p, th.as<double>());
-      reduce(6, out.as<double>() + 3, 3);
-      HIPCHK(hipMemcpyAsync(hst, out.p, sizeof hst, hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      check_errors();
+      for (int attempt = 0;; attempt++) {
+        if (attempt > 0) rebuild();  // full rows for the pull kernel
+        force(1, 0);
+        reduce(3, out.as<double>());
+        hipLaunchKernelGGL(k_cg_stats, dim3(nb), dim3(BS), 0, stream, (int)n, (const double4 *)f.p,
+                           (const double4 *)cg_g.p, (const double4 *)cg_h.p, th.as<double>());
+        reduce(6, out.as<double>() + 3, 3);
+        HIPCHK(hipMemcpyAsync(hst, out.p, sizeof hst, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (!check_errors(attempt == 0)) break;
+      }
       return hst[0] + hst[1] + hst[2];
     };
     int64_t neval = 0;
@@ -1278,14 +1291,17 @@ struct Ctx {
     std::vector<double> h(16 * (size_t)nsys);
     // energies (columns 0-2 of the thermo block) and the FIRE statistics of the current state
     auto measure = [&]() {
-      force(1, 0);
-      reduce(3, out.as<double>());
-      hipLaunchKernelGGL(k_fire_stats, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n, (const double4 *)v.p,
-                         (const double4 *)f.p, th.as<double>());
-      reduce(6, out.as<double>() + 3 * (size_t)nsys, 3);
-      HIPCHK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 9 * nsys, hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      check_errors();
+      for (int attempt = 0;; attempt++) {
+        if (attempt > 0) rebuild();  // full rows for the pull kernel
+        force(1, 0);
+        reduce(3, out.as<double>());
+        hipLaunchKernelGGL(k_fire_stats, dim3((n + BS - 1) / BS), dim3(BS), 0, stream, (int)n,
+                           (const double4 *)v.p, (const double4 *)f.p, th.as<double>());
+        reduce(6, out.as<double>() + 3 * (size_t)nsys, 3);
+        HIPCHK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 9 * nsys, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (!check_errors(attempt == 0)) break;
+      }
     };
     auto energy = [&](int k) { return h[3 * k] + h[3 * k + 1] + h[3 * k + 2]; };
     auto stat = [&](int k, int c) { return h[3 * (size_t)nsys + 6 * k + c]; };
@@ -1461,13 +1477,15 @@ struct Ctx {
   }
 
   void evaluate(int eflag, int vflag) {
-    relayout();
-    if (need_rebuild) rebuild();
-    force(eflag, vflag);
-    if (eflag || vflag) {
-      if (eflag || (vflag % 4)) reduce(9, sums.as<double>());
+    for (int attempt = 0;; attempt++) {
+      relayout();
+      if (need_rebuild) rebuild();
+      force(eflag, vflag);
+      if (eflag || vflag) {
+        if (eflag || (vflag % 4)) reduce(9, sums.as<double>());
+      }
+      if (!check_errors(attempt == 0)) break;  // a repeat runs the pull kernel
     }
-    check_errors();
   }
 
   void run(int64_t nsteps, int64_t thermo_every, int use_graph, lmff_thermo *out, int64_t out_cap) {
