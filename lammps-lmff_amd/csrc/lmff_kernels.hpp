@@ -1422,8 +1422,10 @@ __global__ __launch_bounds__(256, (EF || VF || VA) ? 1 : LMFF_ILP_MIN_WAVES) voi
     const uint32_t e = e_n1;
     const double4 xj = x_n1;
     e_n1 = e_n2;
-    if (k + 64 < nfront) x_n1 = P.x[e_idx(e_n1)];
-    if (k + 128 < nfront) e_n2 = row[k + 128];
+    // unconditional (past the row's end a lane reloads an entry it holds or atom 0, in bounds):
+    // no branch, so the waits inside the pair terms can leave these loads in flight
+    x_n1 = P.x[e_idx(e_n1)];
+    e_n2 = row[k + 128 < nfront ? k + 128 : nfront - 1];
     pair_terms(e, xj);
   }
   // back segment (walked once the far flag is up): only a few of its entries have come within
@@ -1771,15 +1773,21 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
         for (int k2 = lane; k2 < cnt; k2 += 64) {
           const uint32_t e = e_n1, l = l_n1;
           const double4 xj = x_n1;
-          const N4 nj = n_n1;
+          // FP64: this chunk's normal is loaded before the prefetches, and the prefetches are
+          // unconditional (past the row's end a lane reloads an entry it already holds or atom
+          // 0, both in bounds), so that waiting for the normal leaves them in flight (vector
+          // loads complete in issue order, and a wait after a branch counts every load)
+          const N4 nj = NPRE ? n_n1 : P.nrm[e_idx(e)];
           e_n1 = e_n2;
           l_n1 = l_n2;
-          if (k2 + 64 < cnt) {
-            x_n1 = P.x[e_idx(e_n1)];
-            if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+          x_n1 = P.x[e_idx(e_n1)];
+          if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+          {
+            const int kn = k2 + 128 < cnt ? k2 + 128 : cnt - 1;
+            e_n2 = fr[kn];
+            l_n2 = fl[kn];
           }
-          if (k2 + 128 < cnt) { e_n2 = fr[k2 + 128]; l_n2 = fl[k2 + 128]; }
-          terms(e, l, xj, NPRE ? nj : P.nrm[e_idx(e)]);
+          terms(e, l, xj, nj);
         }
       }
       // back entries of this dz (appended from slot icap - 1 downwards): tested first, the ones
