@@ -109,9 +109,16 @@ def timed_run(ctx, steps, warmup, profile):
 def drift_run(ctx, sid, natoms, nsteps, dt_ps=0.001, every=10):
     """NVE energy drift: total energy (pair + kinetic) every `every` steps over an nsteps run,
     fitted by least squares against time; slope in eV/atom/ps with its standard error, and the
-    RMS of the residual fluctuation."""
-    rec = ctx.run(nsteps, every, use_graph=True)[0]
-    ms_total, _, _ = ctx.profile_read()
+    RMS of the residual fluctuation.  Run in segments of at most 1,000 steps (the trajectory
+    and its step count carry on from one to the next), each reported on stderr."""
+    rec, ms_total, done = [], 0.0, 0
+    while done < nsteps:
+        seg = min(1000, nsteps - done)
+        rec += ctx.run(seg, every, use_graph=True)[0]
+        ms_total += ctx.profile_read()[0]
+        done += seg
+        if nsteps > 1000:
+            log(f"  drift run: {done} of {nsteps} steps")
     t = np.array([r["step"] for r in rec], dtype=np.float64) * dt_ps
     e = np.array([r["pe"] + r["ke"] for r in rec]) / natoms
     A = np.stack([t - t.mean(), np.ones_like(t)], 1)

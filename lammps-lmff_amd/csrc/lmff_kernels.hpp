@@ -2082,11 +2082,12 @@ __device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const doubl
   t = (int)xj.w - 1;
 }
 
-// Per step 3: the Tersoff bonds of i, one lane per bond (four lanes per atom, lane q takes
-// bonds q, q+4, ...).  For bond (i, j): zeta_ij and b_ij (zeta 599, force_zeta 621-640), the
+// Per step 3: the Tersoff bonds of i, one lane per bond (three lanes per atom, the bond count of
+// an atom of a graphene or hBN sheet; lane q takes bonds q, q+3, ...; a wave holds 21 atoms and
+// its last lane idles).  For bond (i, j): zeta_ij and b_ij (zeta 599, force_zeta 621-640), the
 // three-body forces of every triplet (i, j, k) (attractive 642, ters_zetaterm_d 738-795), and
 // the repulsive pair with i's half (584-597; pair_tersoff.cpp:141-262 for the loop structure).
-// Every triplet is evaluated once.  What acts on i is summed over the quad into ft[i]; the
+// Every triplet is evaluated once.  What acts on i is summed over the atom's lanes into ft[i]; the
 // force on j and on every k is stored per bond in fb[i][jj][kk] (kk == jj: the force on j),
 // where the gather kernel picks it up.  The global virial of the bond is kept at i; per-atom
 // virial shares (v_tally3 thirds, ev_tally halves) are stored in wb for the gather.
@@ -2094,6 +2095,9 @@ __device__ __forceinline__ void nb_del(const double4 *x, uint32_t e, const doubl
 // staged in LDS, which shortens the chains of dependent loads that bound this kernel.  Three
 // resident waves per SIMD (a few bytes of spill) measured 10 % faster than two.
 constexpr int TP_LDS = 64;
+// lanes per atom in the bonds kernel and the atoms a wave holds
+constexpr int BOND_LANES = 3;
+constexpr int BOND_ATOMS_PER_WAVE = 64 / BOND_LANES;
 
 template <typename R, bool EF, bool VF, bool VA, bool LDSP>
 __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
@@ -2108,10 +2112,11 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   }
   const int *t3 = LDSP ? s_t3 : P.t3;
   const TersP<double> *tp = LDSP ? s_tp : P.tp;
-  const int gt = xcd_block() * blockDim.x + threadIdx.x;
-  const int i = gt >> 2;
-  const int quad = gt & 3;
-  if (i >= P.n) return;  // the whole quad leaves together
+  const int wl = threadIdx.x & 63;
+  const int la = wl / BOND_LANES;               // atom within the wave
+  const int lane = wl - BOND_LANES * la;        // lane within the atom
+  const int i = ((xcd_block() * blockDim.x + threadIdx.x) >> 6) * BOND_ATOMS_PER_WAVE + la;
+  if (la >= BOND_ATOMS_PER_WAVE || i >= P.n) return;  // all of an atom's lanes leave together
   const SysDev &s = P.sys_tab[P.sys[i]];
   const double prd[3] = {s.prd[0], s.prd[1], s.prd[2]};
   const double4 xi = P.x[i];
@@ -2120,11 +2125,11 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   const int itag = P.tag[i];
   const uint32_t *row = P.tsh + (size_t)i * MAXSH;
   const size_t n = (size_t)P.n;
-  // the quad stages i's neighbours once (x_k - x_i, r^2, r, type per slot); both loops of
+  // the atom's lanes stage i's neighbours once (x_k - x_i, r^2, r, type per slot); both loops of
   // every bond read them from here instead of gathering and recomputing them per triplet
-  __shared__ double s_nb[64][MAXSH][6];
-  double(*nbr)[6] = s_nb[threadIdx.x >> 2];
-  for (int kk = quad; kk < ns; kk += 4) {
+  __shared__ double s_nb[4 * BOND_ATOMS_PER_WAVE][MAXSH][6];
+  double(*nbr)[6] = s_nb[(threadIdx.x >> 6) * BOND_ATOMS_PER_WAVE + la];
+  for (int kk = lane; kk < ns; kk += BOND_LANES) {
     double d[3], rsq;
     int t;
     nb_del(P.x, row[kk], prd, xi, d, rsq, t);
@@ -2135,7 +2140,7 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     nbr[kk][4] = M::sq(rsq);
     nbr[kk][5] = (double)t;
   }
-  // read by the other lanes of this quad only (one wave)
+  // read by the other lanes of this atom only (one wave)
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -2144,7 +2149,7 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
   double vg[6] = {0, 0, 0, 0, 0, 0};  // global virial of i's bonds (VF)
   double vo[6] = {0, 0, 0, 0, 0, 0};  // i's own per-atom share (VA)
   const double third = 1.0 / 3.0;
-  for (int jj = quad; jj < ns; jj += 4) {
+  for (int jj = lane; jj < ns; jj += BOND_LANES) {
     const uint32_t ej = row[jj];
     const size_t slot = (size_t)i * MAXSH + jj;
     const double dj[3] = {nbr[jj][0], nbr[jj][1], nbr[jj][2]};
@@ -2281,19 +2286,18 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     if (VA)
       for (int c = 0; c < 6; c++) P.wb[bond_slot(i, jj, jj, ns) * 6 + c] = vj[c];
   }
-  // quad reduction, the same order in every lane
-#pragma unroll
-  for (int o = 1; o <= 2; o <<= 1) {
-    F.x += __shfl_xor(F.x, o, 64);
-    F.y += __shfl_xor(F.y, o, 64);
-    F.z += __shfl_xor(F.z, o, 64);
-    if (EF) et += __shfl_xor(et, o, 64);
-    if (VF)
-      for (int c = 0; c < 6; c++) vg[c] += __shfl_xor(vg[c], o, 64);
-    if (VA)
-      for (int c = 0; c < 6; c++) vo[c] += __shfl_xor(vo[c], o, 64);
-  }
-  if (quad != 0) return;
+  // sum over the atom's lanes in lane order (the same in every lane)
+  const int l0 = BOND_LANES * la;
+  auto lsum = [&](double v) { return (__shfl(v, l0, 64) + __shfl(v, l0 + 1, 64)) + __shfl(v, l0 + 2, 64); };
+  F.x = lsum(F.x);
+  F.y = lsum(F.y);
+  F.z = lsum(F.z);
+  if (EF) et = lsum(et);
+  if (VF)
+    for (int c = 0; c < 6; c++) vg[c] = lsum(vg[c]);
+  if (VA)
+    for (int c = 0; c < 6; c++) vo[c] = lsum(vo[c]);
+  if (lane != 0) return;
   F.w = 0.0;
   P.ft[i] = F;
   if (EF) P.th[2 * n + i] = et;
