@@ -1023,14 +1023,10 @@ struct Ctx {
     }
     mark_end(2, mk);
     mark_begin(3, mk);
-    const int nb4 = (int)((4 * n + BS - 1) / BS);  // four lanes per atom (the gather)
-    {
-      // the bonds kernel: BOND_ATOMS_PER_WAVE atoms per wave, four waves per block
-      const int waves = (int)((n + BOND_ATOMS_PER_WAVE - 1) / BOND_ATOMS_PER_WAVE);
-      const int nbb = (waves + BS / 64 - 1) / (BS / 64);
-      if (ntp <= TP_LDS) launch_bonds<R, true>(P, ef, vf, va, nbb);
-      else launch_bonds<R, false>(P, ef, vf, va, nbb);
-    }
+    // the bonds and gather kernels: BOND_ATOMS_PER_WAVE atoms per wave, four waves per block
+    const int nb4 = (int)(((n + BOND_ATOMS_PER_WAVE - 1) / BOND_ATOMS_PER_WAVE + BS / 64 - 1) / (BS / 64));
+    if (ntp <= TP_LDS) launch_bonds<R, true>(P, ef, vf, va, nb4);
+    else launch_bonds<R, false>(P, ef, vf, va, nb4);
     mark_end(3, mk);
     mark_begin(4, mk);
     if (integ) {

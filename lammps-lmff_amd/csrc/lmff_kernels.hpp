@@ -2307,14 +2307,15 @@ __global__ __launch_bounds__(256, 3) void k_tersoff_bonds(DevPtrs<R> P) {
     for (int c = 0; c < 6; c++) P.vatom[c * n + i] += vo[c];
 }
 
-// Per step 4: the force on atom a, by a quad of lanes.  Lane 0 adds the ILP pair sum and a's
-// own Tersoff bonds (both from the kernels before) and a's own d n_a / d x_a term.  The quad
-// then walks one list of items, lane q taking items q, q+4, ...: first a's reverse Tersoff
+// Per step 4: the force on atom a, by BOND_LANES lanes (21 atoms per wave, as in the bonds
+// kernel).  Lane 0 adds the ILP pair sum and a's own Tersoff bonds (both from the kernels
+// before) and a's own d n_a / d x_a term.  The atom's lanes then walk one list of items, lane q
+// taking items q, q+3, ...: first a's reverse Tersoff
 // partners i (what i's bonds left for a: fb[i][slot of a][jj] over jj, a as j when jj is a's
 // own slot, as k otherwise), then a's reverse normal candidates m (the normal-derivative force
 // on a as a normal neighbour of m).  Every item is a short chain of dependent loads; spreading
-// them over the quad keeps several chains in flight per atom.  The quad is summed with a
-// fixed-order shuffle butterfly.  With INT (the Verlet loop), lane 0 then applies post_force
+// them over the lanes keeps several chains in flight per atom.  The lanes are summed in lane
+// order.  With INT (the Verlet loop), lane 0 then applies post_force
 // (Langevin) and final_integrate to a with the same arithmetic as k_final_integrate, which
 // saves a launch and a re-read of f and v per step.  In that loop k_initial_integrate has
 // already advanced the step counter, so the noise is keyed by step - 1, the step
@@ -2326,7 +2327,7 @@ constexpr uint32_t PHILOX_STEP_WORD = 0x4c4d4646u;   // "LMFF"
 constexpr uint32_t PHILOX_SETUP_WORD = 0x53455455u;  // "SETU"
 
 template <typename R, bool EF, bool VA, bool INT>
-__device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad, int want_ke) {
+__device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int lane, int want_ke) {
   const size_t n = (size_t)P.n;
   double4 F = {0.0, 0.0, 0.0, 0.0};
   double et = 0.0;
@@ -2334,7 +2335,7 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
   double vk[6] = {0, 0, 0, 0, 0, 0};   // ILP k-term halves landing on a (per-atom only)
   const int rt = P.rtcnt[a], rn = P.rncnt[a];
   const double *prd = P.sys_tab[P.sys[a]].prd;  // a's reverse candidates share a's system
-  if (quad == 0) {
+  if (lane == 0) {
     const double4 f0 = P.fil[a], f1 = P.ft[a];
     F = {f0.x + f1.x, f0.y + f1.y, f0.z + f1.z, 0.0};
     // ---- ILP: d n_a / d x_a term (2-neighbour normals only) ----
@@ -2348,7 +2349,7 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
       }
     }
   }
-  for (int q = quad; q < rt + rn; q += 4) {
+  for (int q = lane; q < rt + rn; q += BOND_LANES) {
     if (q < rt) {
       // ---- Tersoff: what the bonds of i left for a ----
       const int i = (int)P.rtc[(size_t)a * MAXSH + q];
@@ -2434,20 +2435,19 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
       }
     }
   }
-  // quad reduction, the same order in every lane
-#pragma unroll
-  for (int o = 1; o <= 2; o <<= 1) {
-    F.x += __shfl_xor(F.x, o, 64);
-    F.y += __shfl_xor(F.y, o, 64);
-    F.z += __shfl_xor(F.z, o, 64);
-    if (EF) et += __shfl_xor(et, o, 64);
-    if (VA)
-      for (int c = 0; c < 6; c++) {
-        va[c] += __shfl_xor(va[c], o, 64);
-        vk[c] += __shfl_xor(vk[c], o, 64);
-      }
-  }
-  if (quad != 0) return;
+  // sum over the atom's lanes in lane order (the same in every lane)
+  const int l0 = ((int)(threadIdx.x & 63) / BOND_LANES) * BOND_LANES;
+  auto lsum = [&](double v) { return (__shfl(v, l0, 64) + __shfl(v, l0 + 1, 64)) + __shfl(v, l0 + 2, 64); };
+  F.x = lsum(F.x);
+  F.y = lsum(F.y);
+  F.z = lsum(F.z);
+  if (EF) et = lsum(et);
+  if (VA)
+    for (int c = 0; c < 6; c++) {
+      va[c] = lsum(va[c]);
+      vk[c] = lsum(vk[c]);
+    }
+  if (lane != 0) return;
   F.w = 0.0;
   if (INT) {
     // post_force (Langevin) + final_integrate, as k_final_integrate does them
@@ -2483,10 +2483,11 @@ __device__ __forceinline__ void gather_atom(const DevPtrs<R> &P, int a, int quad
 
 template <typename R, bool EF, bool VA, bool INT>
 __global__ __launch_bounds__(256) void k_gather(DevPtrs<R> P, int want_ke) {
-  const int gt = xcd_block() * blockDim.x + threadIdx.x;
-  const int a = gt >> 2;
-  const int quad = gt & 3;
-  if (a < P.n) gather_atom<R, EF, VA, INT>(P, a, quad, want_ke);  // the whole quad or none of it
+  const int wl = threadIdx.x & 63;
+  const int la = wl / BOND_LANES;
+  const int a = ((xcd_block() * blockDim.x + threadIdx.x) >> 6) * BOND_ATOMS_PER_WAVE + la;
+  // all of an atom's lanes or none of them
+  if (la < BOND_ATOMS_PER_WAVE && a < P.n) gather_atom<R, EF, VA, INT>(P, a, wl - BOND_LANES * la, want_ke);
 }
 
 // ---------------------------------------------------------------------------------------
