@@ -54,6 +54,13 @@ int64_t half_min_atoms() {
   }();
   return v;
 }
+// window atoms the half-pair kernel keeps in LDS at most: WCAP_MAX, or LMFF_HALF_WCAP (a
+// multiple of 64 from 64 up, read at every list build) to send more slots to the global sums
+int half_wcap_limit() {
+  const char *e = std::getenv("LMFF_HALF_WCAP");
+  if (!e) return WCAP_MAX;
+  return std::min(WCAP_MAX, std::max(64, std::atoi(e) / 64 * 64));
+}
 constexpr int MAX_DEVICES = 64;
 
 // device address of `scratch` per GPU, looked up once per device with that device current
@@ -151,6 +158,7 @@ struct SysHost {
 
 void check_box(const double lo[3], const double hi[3], const int per[3], double cutneigh);
 void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh);
+void grid_from_extent(SysHost &s, const double mn[3], const double mx[3], int64_t n, double cutneigh);
 
 struct Ctx {
   int device = 0;
@@ -204,6 +212,8 @@ struct Ctx {
   // half-pair ILP: window slots of list entries, per-dz segments, window tables, per-system
   // largest window and mode, partner sums
   DBuf lidx, segf, segb, win, winmax, syshalf, acc, ipart;
+  DBuf extent_d;  // per re-fitted system: the atoms' extent (k_extent)
+  PinnedBuf regrid_host;  // the extents read back, the system table's grid fields written
   // full ILP rows for evaluations by the pull kernel while half systems hold half rows
   DBuf ilist_full, icnt_full, ocnt_full;
   int icap_full = 0;
@@ -212,6 +222,7 @@ struct Ctx {
   std::vector<int> half_sys;   // per system: 1 when its ILP pairs run in the half-pair kernel
   bool half_any = false, pull_any = true;
   int wcap = 0;                // window atoms the half-pair kernel's LDS holds (largest half window)
+  bool half_ovf = false;       // some half window is larger than wcap (the kernel's OVF variant)
   bool half_enabled = true;    // LMFF_HALF=0 in the environment keeps every system in the pull kernel
   PinnedBuf hb_host;   // and the forces on their way back
   int ntp = 0;  // Tersoff parameter records
@@ -260,7 +271,7 @@ struct Ctx {
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &row_mol, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
-                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart, &ilist_full, &icnt_full, &ocnt_full};
+                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart, &ilist_full, &icnt_full, &ocnt_full, &extent_d};
     for (DBuf *b : all) b->tally = &bytes_in_use;
     const char *h = std::getenv("LMFF_HALF");
     if (h && std::strcmp(h, "0") == 0) half_enabled = false;
@@ -504,6 +515,7 @@ struct Ctx {
     half_any = false;
     pull_any = true;
     wcap = 0;
+    half_ovf = false;
     if (n) HIPCHK(hipMemcpyAsync(f.p, fsave.p, n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
     restore_f = had_new ? keep_f : std::vector<std::pair<int, int>>{};
     upload_systab();
@@ -522,6 +534,15 @@ struct Ctx {
     sums.ensure(sizeof(double) * std::max<size_t>(sys.size(), 1) * 16);
     if (!ch.empty()) HIPCHK(hipMemcpy(chunks.p, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(cbeg.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+    size_cells();
+    layout_dirty = false;
+    need_rebuild = true;
+    forces_valid = forces_valid && !had_new;
+    gen++;
+  }
+
+  // cell ranges, row molecule ranges and the sort's key bits and scratch for ncells cells
+  void size_cells() {
     cell_start.ensure(sizeof(int) * (size_t)(ncells + 1));
     row_mol.ensure(sizeof(int2) * (size_t)std::max(nrows_cells, 1));
     end_bit = 1;
@@ -531,10 +552,89 @@ struct Ctx {
                                               (int *)nullptr, (int)std::max<int64_t>(n, 1), 0, end_bit, stream));
     sort_tmp.ensure(tb);
     sort_tmp_bytes = tb;
-    layout_dirty = false;
-    need_rebuild = true;
-    forces_valid = forces_valid && !had_new;
-    gen++;
+  }
+
+  // Re-fits the bins of the non-periodic sides of every system of at least half_min_atoms()
+  // atoms to its atoms' current extent (grid_from_extent), at the start of every list build: a
+  // stack whose sheets expand or ripple would otherwise pile its outer sheets into the edge
+  // bins, whose windows then outgrow the half-pair kernel's LDS.  The cell, row and tile offsets
+  // of every system follow; a change in their counts invalidates captured steps.
+  void regrid() {
+    std::vector<int> which;
+    for (size_t k = 0; k < sys.size(); k++) {
+      const SysHost &s = sys[k];
+      if (s.n > 0 && s.n >= half_min_atoms() && !(s.per[0] && s.per[1] && s.per[2])) which.push_back((int)k);
+    }
+    if (which.empty()) return;
+    const size_t m = which.size();
+    // minima of every re-fitted system first, then the maxima
+    const size_t eb = sizeof(unsigned long long) * 3 * m;
+    extent_d.ensure(2 * eb);
+    regrid_host.ensure(std::max(2 * eb, sizeof(SysDev) * sys.size()));
+    HIPCHK(hipMemsetAsync(extent_d.p, 0xff, eb, stream));
+    HIPCHK(hipMemsetAsync((char *)extent_d.p + eb, 0, eb, stream));
+    unsigned long long *emin = extent_d.as<unsigned long long>(), *emax = emin + 3 * m;
+    for (size_t q = 0; q < m; q++) {
+      const SysHost &s = sys[which[q]];
+      const int blocks = (int)std::min<int64_t>(256, (s.n + BS - 1) / BS);
+      hipLaunchKernelGGL(k_extent, dim3(blocks), dim3(BS), 0, stream, s.begin, s.end, (const double4 *)x.p,
+                         emin + 3 * q, emax + 3 * q);
+    }
+    HIPCHK(hipMemcpyAsync(regrid_host.p, extent_d.p, 2 * eb, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<unsigned long long> ex((const unsigned long long *)regrid_host.p,
+                                       (const unsigned long long *)regrid_host.p + 6 * m);
+    auto value = [](unsigned long long k) {
+      const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+      double v;
+      std::memcpy(&v, &b, sizeof v);
+      return v;
+    };
+    bool changed = false, counts = false;
+    for (size_t q = 0; q < m; q++) {
+      SysHost &s = sys[which[q]];
+      double mn[3], mx[3];
+      for (int d = 0; d < 3; d++) {
+        mn[d] = value(ex[3 * q + d]);
+        mx[d] = value(ex[3 * m + 3 * q + d]);
+        if (!std::isfinite(mn[d]) || !std::isfinite(mx[d])) return;  // k_wrap_key reports the atom
+      }
+      const SysHost old = s;
+      grid_from_extent(s, mn, mx, s.n, cutneigh);
+      for (int d = 0; d < 3; d++) {
+        changed |= s.nc[d] != old.nc[d] || s.glo[d] != old.glo[d] || s.bs[d] != old.bs[d];
+        counts |= s.nc[d] != old.nc[d];
+      }
+      counts |= s.tile[0] != old.tile[0] || s.ntile[0] != old.ntile[0] || s.ntile[1] != old.ntile[1];
+    }
+    if (!changed) return;
+    if (counts) {
+      int nc = 0, nr = 0, nt = 0;
+      for (auto &s : sys) {
+        s.cell_off = nc;
+        nc += s.ncells;
+        s.row_off = nr;
+        nr += s.nc[1] * s.nc[2];
+        s.tile_off = nt;
+        if (s.tile[0] > 0) nt += s.ntile[0] * s.ntile[1] * s.nc[2];
+      }
+      ncells = nc;
+      nrows_cells = nr;
+      ntiles = nt;
+      size_cells();
+      win.ensure(sizeof(int) * (size_t)std::max(ntiles, 1) * WSTR);
+      gen++;  // launch sizes and buffers of a captured step change
+    }
+    // the grid fields of the device's system table (nc up to begin: bins, offsets, tiles; the
+    // step counters the device advances are left alone), from page-locked staging that no
+    // other copy uses before the next list build
+    SysDev *img = (SysDev *)regrid_host.p;
+    const size_t f0 = offsetof(SysDev, nc), f1 = offsetof(SysDev, begin);
+    for (size_t k = 0; k < sys.size(); k++) {
+      img[k] = sysdev_of(sys[k]);
+      HIPCHK(hipMemcpyAsync((char *)systab.p + k * sizeof(SysDev) + f0, (const char *)(img + k) + f0, f1 - f0,
+                            hipMemcpyHostToDevice, stream));
+    }
   }
 
   static void adopt(DBuf &dst, DBuf &src) {
@@ -610,36 +710,38 @@ struct Ctx {
 
   void upload_systab() {
     std::vector<SysDev> t(sys.size());
-    for (size_t k = 0; k < sys.size(); k++) {
-      const auto &s = sys[k];
-      SysDev &d = t[k];
-      for (int a = 0; a < 3; a++) {
-        d.lo[a] = s.lo[a];
-        d.prd[a] = s.hi[a] - s.lo[a];
-        d.per[a] = s.per[a];
-        d.nc[a] = s.nc[a];
-        d.glo[a] = s.glo[a];
-        d.inv_bs[a] = 1.0 / s.bs[a];
-      }
-      d.cell_off = s.cell_off;
-      d.row_off = s.row_off;
-      d.tile[0] = s.tile[0];
-      d.tile[1] = s.tile[1];
-      d.ntile[0] = s.ntile[0];
-      d.ntile[1] = s.ntile[1];
-      d.tile_off = s.tile_off;
-      d.begin = s.begin;
-      d.end = s.end;
-      d.dtv = s.dt;
-      d.dtf = 0.5 * s.dt * FTM2V;
-      d.style = s.style;
-      d.t_target = s.t_target;
-      d.damp = s.damp;
-      d.seed = s.seed;
-      d.step = s.step;
-    }
+    for (size_t k = 0; k < sys.size(); k++) t[k] = sysdev_of(sys[k]);
     systab.ensure(sizeof(SysDev) * std::max<size_t>(t.size(), 1));
     if (!t.empty()) HIPCHK(hipMemcpy(systab.p, t.data(), sizeof(SysDev) * t.size(), hipMemcpyHostToDevice));
+  }
+
+  static SysDev sysdev_of(const SysHost &s) {
+    SysDev d{};
+    for (int a = 0; a < 3; a++) {
+      d.lo[a] = s.lo[a];
+      d.prd[a] = s.hi[a] - s.lo[a];
+      d.per[a] = s.per[a];
+      d.nc[a] = s.nc[a];
+      d.glo[a] = s.glo[a];
+      d.inv_bs[a] = 1.0 / s.bs[a];
+    }
+    d.cell_off = s.cell_off;
+    d.row_off = s.row_off;
+    d.tile[0] = s.tile[0];
+    d.tile[1] = s.tile[1];
+    d.ntile[0] = s.ntile[0];
+    d.ntile[1] = s.ntile[1];
+    d.tile_off = s.tile_off;
+    d.begin = s.begin;
+    d.end = s.end;
+    d.dtv = s.dt;
+    d.dtf = 0.5 * s.dt * FTM2V;
+    d.style = s.style;
+    d.t_target = s.t_target;
+    d.damp = s.damp;
+    d.seed = s.seed;
+    d.step = s.step;
+    return d;
   }
 
   void sync_steps_from_device() {
@@ -760,6 +862,7 @@ struct Ctx {
     }
     size_t mk = 0;
     mark_begin(0, mk);
+    regrid();
     const int nb = (int)((n + BS - 1) / BS);
     hipLaunchKernelGGL(k_wrap_key, dim3(nb), dim3(BS), 0, stream, (int)n, x.as<double4>(), img.as<int>(),
                        sysid.as<int>(), systab.as<SysDev>(), key.as<uint32_t>(), val.as<int>(), errw.as<int>());
@@ -813,9 +916,10 @@ struct Ctx {
     HIPCHK(hipMemcpyAsync(b.p, tmpi.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
   }
 
-  // Which systems run their ILP pairs in the half-pair kernel: those with tiles whose largest
-  // window fits the kernel's LDS, whose stencil reaches at most NSEG - 2 z bins up, and whose
-  // list rows fit the 16-bit segment counts.  Decided per system from its own geometry (on the
+  // Which systems run their ILP pairs in the half-pair kernel: those with tiles, whose stencil
+  // reaches at most NSEG - 2 z bins up, and whose list rows fit the 16-bit segment counts.  The
+  // kernel's LDS holds the largest window up to WCAP_MAX atoms (LMFF_HALF_WCAP in the
+  // environment lowers that limit); slots beyond it add to the global sums directly.  Decided per system from its own geometry (on the
   // device, k_half_decide, before the list build, which keeps half rows for those systems), so a
   // system's results do not depend on what else is packed with it; here the host takes the
   // decisions back and sizes the launches.
@@ -836,7 +940,9 @@ struct Ctx {
       if (hs[k]) any = true;
       else if (sys[k].n > 0) pull = true;
     }
-    wc = (wc + 63) / 64 * 64;
+    const int wmax = wc;
+    wc = std::min((wc + 63) / 64 * 64, half_wcap_limit());
+    const bool ovf = any && wmax > wc;
     if (any) {
       half_attrs();  // here, never inside a graph capture
       // the atoms' own sums at the widest layout (with the virial), so that no later evaluation
@@ -846,9 +952,10 @@ struct Ctx {
         gen++;
       }
     }
-    if (hs != half_sys || wc != wcap || any != half_any || pull != pull_any || np != npass) {
+    if (hs != half_sys || wc != wcap || any != half_any || pull != pull_any || np != npass || ovf != half_ovf) {
       half_sys = hs;
       wcap = wc;
+      half_ovf = ovf;
       npass = np;
       half_any = any;
       pull_any = pull;
@@ -960,7 +1067,12 @@ struct Ctx {
   template <typename R, bool EF, bool VF, bool SYM>
   void launch_half_one(const DevPtrs<R> &P) {
     const size_t lds = sizeof(unsigned long long) * 6 * (size_t)wcap;
-    hipLaunchKernelGGL((k_ilp_half<R, EF, VF, SYM>), dim3(ntiles), dim3(64 * HalfWaves<R>::n), lds, stream, P, wcap);
+    if (half_ovf)
+      hipLaunchKernelGGL((k_ilp_half<R, EF, VF, SYM, true>), dim3(ntiles), dim3(64 * HalfWaves<R>::n), lds, stream, P,
+                         wcap);
+    else
+      hipLaunchKernelGGL((k_ilp_half<R, EF, VF, SYM, false>), dim3(ntiles), dim3(64 * HalfWaves<R>::n), lds, stream, P,
+                         wcap);
   }
   template <typename R, bool SYM>
   void launch_half(const DevPtrs<R> &P, int ef, int vf) {
@@ -971,8 +1083,10 @@ struct Ctx {
   // the half-pair kernel's window sums take more than the default 64 KB of LDS per workgroup
   template <typename R, bool EF, bool VF, bool SYM>
   static void half_attr_one() {
-    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM, false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
   }
   template <typename R>
   static void half_attr_prec() {
@@ -1872,6 +1986,29 @@ void check_box(const double lo[3], const double hi[3], const int per[3], double 
 // rounding).  A non-periodic side spans the atoms' extent, or the box bounds when x is null
 // (atoms beyond the bins fall into the edge bins, which reach to infinity).
 void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) {
+    mn[d] = 1e300;
+    mx[d] = -1e300;
+    if (!x) {
+      mn[d] = s.lo[d];
+      mx[d] = s.hi[d];
+    } else {
+      for (int64_t i = 0; i < n; i++) {
+        mn[d] = std::min(mn[d], x[3 * i + d]);
+        mx[d] = std::max(mx[d], x[3 * i + d]);
+      }
+      if (n == 0) mn[d] = mx[d] = 0.0;
+    }
+  }
+  grid_from_extent(s, mn, mx, n, cutneigh);
+}
+
+// The grid of setup_grid from the extent [mn, mx] of the atoms along the non-periodic sides
+// (periodic sides take the box).  Also what each list build of a running system re-fits its
+// non-periodic sides with, so that the edge bins of a stack that expands or ripples do not
+// gather the atoms of two sheets.
+void grid_from_extent(SysHost &s, const double mn_[3], const double mx_[3], int64_t n, double cutneigh) {
   // bins of at least a sixth of the list cutoff in x and z and a third in y (the list build's
   // stencil then has at most 7 rows in y and 13 in a periodic z); a non-periodic z (the normal
   // of a stack) splits the atoms' extent into bins of at most a sixth instead, so that sheets
@@ -1886,17 +2023,7 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
       s.glo[d] = s.lo[d];
       s.bs[d] = L / s.nc[d];
     } else {
-      double mn = 1e300, mx = -1e300;
-      if (!x) {
-        mn = s.lo[d];
-        mx = s.hi[d];
-      } else {
-        for (int64_t i = 0; i < n; i++) {
-          mn = std::min(mn, x[3 * i + d]);
-          mx = std::max(mx, x[3 * i + d]);
-        }
-        if (n == 0) mn = mx = 0.0;
-      }
+      const double mn = mn_[d], mx = mx_[d];
       const double ext = mx - mn;
       s.glo[d] = mn;
       if (d == 2 && ext >= half) {

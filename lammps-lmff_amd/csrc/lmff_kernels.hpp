@@ -61,7 +61,9 @@ constexpr int NSEG = 16;
 // window tables: per window (tile, partner z bin) the exclusive prefix of atoms over its bins
 // (row-major over (tile[1] + 2 ry) rows of (tile[0] + 2 rx) bins), the total at [bins]
 constexpr int WSTR = 256;
-// window atoms the half-pair kernel holds in LDS (six int64 sums each: 72 KB)
+// window atoms the half-pair kernel holds in LDS (six int64 sums each: 72 KB); the partner sums
+// of slots beyond it (windows of rippled or crowded sheets) go to the global sums with device
+// atomics instead
 constexpr int WCAP_MAX = 1536;
 // wavefronts per half-pair workgroup and the waves per SIMD the kernel is compiled for: two
 // workgroups fit a CU's LDS, so 8 waves give 4 per SIMD (128 VGPRs; 10-wave workgroups for FP32,
@@ -340,6 +342,47 @@ __device__ __forceinline__ bool tag_skip(int itag, int jtag, double xi, double y
 // ---------------------------------------------------------------------------------------
 // Rebuild: wrap (Domain::pbc, domain.cpp:501-560), cell key, sort, permute, cell ranges
 
+// order-preserving integer image of a double (a larger value gives a larger integer)
+__device__ __forceinline__ unsigned long long ord_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// The extent of atoms [begin, end) per dimension as ord_key images: emin[d], emax[d] (the caller
+// sets them to ~0 and 0 first).  256 threads per block, grid-stride.
+__global__ __launch_bounds__(256) void k_extent(int begin, int end, const double4 *x, unsigned long long *emin,
+                                                unsigned long long *emax) {
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int p = begin + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += gridDim.x * blockDim.x) {
+    const double4 q = x[p];
+    mn[0] = fmin(mn[0], q.x); mn[1] = fmin(mn[1], q.y); mn[2] = fmin(mn[2], q.z);
+    mx[0] = fmax(mx[0], q.x); mx[1] = fmax(mx[1], q.y); mx[2] = fmax(mx[2], q.z);
+  }
+  __shared__ double s_m[4][6];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[d] = fmin(mn[d], __shfl_xor(mn[d], o, 64));
+      mx[d] = fmax(mx[d], __shfl_xor(mx[d], o, 64));
+    }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; d++) {
+      s_m[wv][d] = mn[d];
+      s_m[wv][3 + d] = mx[d];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    double v = s_m[0][c];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = c < 3 ? fmin(v, s_m[w][c]) : fmax(v, s_m[w][c]);
+    if (v > -1e300 && v < 1e300) {  // a block with atoms
+      if (c < 3) atomicMin(emin + c, ord_key(v));
+      else atomicMax(emax + (c - 3), ord_key(v));
+    }
+  }
+}
+
 // Domain::pbc (LMFF/domain.cpp:501): wraps into the periodic box and counts the crossing in
 // the atom's image flags, packed as LAMMPS' 32-bit imageint (10 bits per dimension, offset
 // IMG_MAX; x in bits 0-9, y in 10-19, z in 20-29)
@@ -611,15 +654,17 @@ __global__ void k_win_table(int nwin, int nsys, const SysDev *st, const int *cel
 }
 
 // Which systems run their ILP pairs in the half-pair kernel (one thread per system), decided
-// before the list build from each system's own geometry: tiles, every window within the LDS
-// window of the kernel, the stencil within NSEG - 2 z bins.  allow = 0 keeps every system in the
-// pull kernel.
+// before the list build from each system's own geometry: tiles, atoms in some window and fewer
+// than 0xffff in every one, the stencil within NSEG - 2 z bins (a window larger than the
+// kernel's LDS holds is allowed: the slots beyond it add to the global sums directly).  allow = 0 keeps every system in the pull
+// kernel.
 __global__ void k_half_decide(int nsys, const SysDev *st, const int *winmax, double cut, int allow, int *syshalf) {
   const int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nsys) return;
   const SysDev &s = st[si];
   const int rz = (int)ceil(cut * s.inv_bs[2]);
-  syshalf[si] = (allow && s.tile[0] > 0 && s.end > s.begin && winmax[si] > 0 && winmax[si] <= WCAP_MAX &&
+  // window slots are 16-bit in the lists (0xffff: none)
+  syshalf[si] = (allow && s.tile[0] > 0 && s.end > s.begin && winmax[si] > 0 && winmax[si] < 0xffff &&
                  rz + 1 <= NSEG - 2) ? 1 : 0;
 }
 
@@ -696,9 +741,9 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
   // the tile of i and the origin of its windows (unwrapped bins), for a system in the half-pair
   // kernel (the pull kernel reads neither window slots nor segments)
   const bool tiled = P.win != nullptr && s.tile[0] > 0 && !P.ilp_only && P.syshalf[P.sys[i]] != 0;
-  // half rows: a system in the half-pair kernel keeps only the ILP entries with dz >= 0 (the
-  // partner in a lower z bin computes the rest); the rows below i's z bin are scanned only for
-  // the short list
+  // half rows: a system in the half-pair kernel keeps only the ILP entries with dz > 0, and with
+  // dz = 0 those of a higher index (the partner in a lower z bin, or of the lower index, computes
+  // the rest); the rows below i's z bin are scanned only for the short list
   const bool halfrows = tiled;
   const int rxs = stencil_rx(s, cut);
   int win0 = 0, wx0 = 0, wy0 = 0, wxb = 1;
@@ -846,7 +891,10 @@ __global__ __launch_bounds__(256) void k_build_lists(DevPtrs<R> P, const int *ce
       rlw = run_loc[wv][r];
       dzv = (int)(rlw >> 16) - 32;
     }
-    const bool ilp = take && (rsq <= P.cutneighsq) && (mj != mi) && type_ok && !(halfrows && dzv < 0);
+    // half rows: only partners in z bins at or above i's, and within i's own z bin only those
+    // of a higher index (the pair is the lower index's)
+    const bool ilp = take && (rsq <= P.cutneighsq) && (mj != mi) && type_ok &&
+                     !(halfrows && (dzv < 0 || (dzv == 0 && j < i)));
     const bool near_i = rsq < P.ilp_splitsq;
     const bool in_i = ilp && near_i, in_o = ilp && !near_i;
     const bool sht = take && (rsq <= P.shortsq) && !P.ilp_only;
@@ -1533,7 +1581,8 @@ __device__ __forceinline__ unsigned long long fx_of(double v, double scale, doub
 // values of an atom's own sums: fx fy fz Sx Sy Sz, then e_vdw e_rep, then the virial
 __host__ __device__ constexpr int half_pstr(bool EF, bool VF) { return 6 + (EF || VF ? 2 : 0) + (VF ? 6 : 0); }
 
-template <typename R, bool EF, bool VF, bool SYM>
+// OVF: some window holds more atoms than wcap (the LDS); without it every slot is below wcap
+template <typename R, bool EF, bool VF, bool SYM, bool OVF>
 __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void k_ilp_half(DevPtrs<R> P, int wcap) {
   using M = MathOps<R>;
   constexpr int PSTR = half_pstr(EF, VF);
@@ -1638,6 +1687,9 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
       for (int k = wv; k < ntile_atoms; k += HALF_WAVES) put0(tile_atom(k), dz);
       continue;
     }
+    // a window larger than the LDS holds (OVF kernels only): the slots from wcap on add to acc[]
+    // directly
+    const bool wovf = OVF && __builtin_amdgcn_readfirstlane(s_bin[nbins]) > wcap;
     // the waves take the tile's atoms one at a time from a shared counter, so that a wave with
     // long rows does not hold the others at the barrier that ends the pass (which wave takes an
     // atom does not change any result: an atom's own sums come from its row alone, and the
@@ -1667,9 +1719,11 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
       double wv6[6] = {0, 0, 0, 0, 0, 0};
       R fxl = 0, fyl = 0, fzl = 0, sxl = 0, syl = 0, szl = 0;
       using N4 = typename std::conditional<sizeof(R) == 8, double4, float4>::type;
-      auto terms = [&](const uint32_t e, const uint32_t slot, const double4 xj, const N4 nj4) {
-        const int j = e_idx(e);
-        if (dz == 0 && j <= i) return;  // within one z bin the lower index takes the pair
+      // mode: where j's terms go -- 0 the window's LDS sums (slot < wcap), 1 the global sums
+      // (slot >= wcap), 2 decided per entry
+      auto terms = [&](auto mode, const uint32_t e, const uint32_t slot, const double4 xj, const N4 nj4) {
+        constexpr int MODE = decltype(mode)::value;
+        const int j = e_idx(e);  // within i's own z bin (dz = 0) the rows hold only j > i
         double gx, gy, gz;
         if (__ballot((e >> IDX_BITS) != 0x15u) == 0) {
           gx = xj.x;
@@ -1727,16 +1781,41 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
           sx += (double)(ta1 * dx); sy += (double)(ta1 * dy); sz += (double)(ta1 * dzz);
         }
         // j's terms: -F and S_j += Tap a_ji del_ji = -Tap a_ji del
-        unsigned long long *pa = s_acc + slot;
-        __hip_atomic_fetch_add(pa, fx_of(-(double)Fx, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + wcap, fx_of(-(double)Fy, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 2 * wcap, fx_of(-(double)Fz, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 3 * wcap, fx_of(-(double)(ta2 * dx), fxs, fxm), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 4 * wcap, fx_of(-(double)(ta2 * dy), fxs, fxm), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(pa + 5 * wcap, fx_of(-(double)(ta2 * dzz), fxs, fxm), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        auto lds_add = [&]() {
+          unsigned long long *pa = s_acc + slot;
+          __hip_atomic_fetch_add(pa, fx_of(-(double)Fx, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(pa + wcap, fx_of(-(double)Fy, fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(pa + 2 * wcap, fx_of(-(double)Fz, fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(pa + 3 * wcap, fx_of(-(double)(ta2 * dx), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(pa + 4 * wcap, fx_of(-(double)(ta2 * dy), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(pa + 5 * wcap, fx_of(-(double)(ta2 * dzz), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        // a slot beyond what the LDS holds (a window larger than wcap): straight to acc[]
+        auto global_add = [&]() {
+          unsigned long long *ga = P.acc + j;
+          __hip_atomic_fetch_add(ga, fx_of(-(double)Fx, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(ga + n, fx_of(-(double)Fy, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(ga + 2 * n, fx_of(-(double)Fz, fxs, fxm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(ga + 3 * n, fx_of(-(double)(ta2 * dx), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(ga + 4 * n, fx_of(-(double)(ta2 * dy), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(ga + 5 * n, fx_of(-(double)(ta2 * dzz), fxs, fxm), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        };
+        if constexpr (MODE == 0) {
+          lds_add();
+        } else if constexpr (MODE == 1) {
+          global_add();
+        } else {
+          if (slot < (uint32_t)wcap) lds_add();
+          else global_add();
+        }
         cmax = fmax(cmax, fmax(fabs(Cd), fmax(fabs(ta1), fabs(ta2))));
         if (EF) {
           ev += (double)(Vvdw * tp.t);
@@ -1752,8 +1831,8 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
           wv6[5] += dyd * (double)Fz;
         }
       };
-      // front entries of this dz, pipelined as in the pull kernel
-      {
+      if constexpr (!OVF) {
+        // front entries of this dz, pipelined as in the pull kernel
         const int cnt = f1 - f0;
         const uint32_t *fr = row + f0;
         const uint16_t *fl = lrow + f0;
@@ -1787,7 +1866,62 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
             e_n2 = fr[kn];
             l_n2 = fl[kn];
           }
-          terms(e, l, xj, nj);
+          terms(std::integral_constant<int, 0>{}, e, l, xj, nj);
+        }
+      } else {
+        // the same loop over entries [k0, k1) of the front segment, their partner sums going to
+        // the LDS window (mode 0) or to acc[] (mode 1)
+        auto front = [&](auto mode, const int k0, const int k1) {
+          const int cnt = k1 - k0;
+          if (cnt <= 0) return;
+          const uint32_t *fr = row + k0;
+          const uint16_t *fl = lrow + k0;
+          uint32_t e_n1 = 0, e_n2 = 0;
+          uint32_t l_n1 = 0, l_n2 = 0;
+          double4 x_n1 = {0, 0, 0, 0};
+          N4 n_n1 = {0, 0, 0, 0};
+          if (lane < cnt) { e_n1 = fr[lane]; l_n1 = fl[lane]; }
+          if (lane + 64 < cnt) { e_n2 = fr[lane + 64]; l_n2 = fl[lane + 64]; }
+          // FP32: the partner's normal is fetched a chunk ahead with its position (FP64: in the
+          // chunk, which measured faster at the FP64 kernel's register budget)
+          constexpr bool NPRE = sizeof(R) == 4;
+          if (lane < cnt) {
+            x_n1 = P.x[e_idx(e_n1)];
+            if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+          }
+          for (int k2 = lane; k2 < cnt; k2 += 64) {
+            const uint32_t e = e_n1, l = l_n1;
+            const double4 xj = x_n1;
+            // FP64: this chunk's normal is loaded before the prefetches, and the prefetches are
+            // unconditional (past the row's end a lane reloads an entry it already holds or atom
+            // 0, both in bounds), so that waiting for the normal leaves them in flight (vector
+            // loads complete in issue order, and a wait after a branch counts every load)
+            const N4 nj = NPRE ? n_n1 : P.nrm[e_idx(e)];
+            e_n1 = e_n2;
+            l_n1 = l_n2;
+            x_n1 = P.x[e_idx(e_n1)];
+            if (NPRE) n_n1 = P.nrm[e_idx(e_n1)];
+            {
+              const int kn = k2 + 128 < cnt ? k2 + 128 : cnt - 1;
+              e_n2 = fr[kn];
+              l_n2 = fl[kn];
+            }
+            terms(mode, e, l, xj, nj);
+          }
+        };
+        if (!wovf) {
+          front(std::integral_constant<int, 0>{}, f0, f1);
+        } else {
+          // the entries of one dz come in ascending window slots (the build scans the window's
+          // bins in its row-major order), so those at or beyond wcap are the segment's tail
+          int lo = f0, hi = f1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)lrow[mid] < wcap) lo = mid + 1;
+            else hi = mid;
+          }
+          front(std::integral_constant<int, 0>{}, f0, lo);
+          front(std::integral_constant<int, 1>{}, lo, f1);
         }
       }
       // back entries of this dz (appended from slot icap - 1 downwards): tested first, the ones
@@ -1808,7 +1942,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
           qn -= take;
           if (lane < take) {
             const uint32_t e = row[qs];
-            terms(e, lrow[qs], P.x[e_idx(e)], P.nrm[e_idx(e)]);
+            terms(std::integral_constant<int, OVF ? 2 : 0>{}, e, lrow[qs], P.x[e_idx(e)], P.nrm[e_idx(e)]);
           }
         };
         for (int k0 = b0; k0 < b1; k0 += 64) {
@@ -1848,7 +1982,7 @@ __global__ __launch_bounds__(64 * HalfWaves<R>::n, HalfWaves<R>::per_simd) void 
     }
     // the window's partner sums go out to acc[] (integer adds: any order gives the same sums)
     __syncthreads();
-    const int wsize = s_bin[nbins];
+    const int wsize = OVF ? min(s_bin[nbins], wcap) : s_bin[nbins];
     for (int p = threadIdx.x; p < wsize; p += blockDim.x) {
       unsigned long long vq[6];
       bool nz = false;

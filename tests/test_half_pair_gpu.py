@@ -133,3 +133,82 @@ def test_half_pair_fixed_point_range_falls_back_to_the_pull_kernel():
     assert np.abs(a["f"] - b["f"]).max() <= 1e-12 * rms
     assert abs(a["energy"] - b["energy"]) <= 1e-12 * abs(b["energy"])
     _check_fp64(a, oracle_for(s).compute(1, 2))
+
+
+def _with_env(key, value, fn):
+    old = os.environ.get(key)
+    os.environ[key] = value
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ[key]
+        else:
+            os.environ[key] = old
+
+
+@pytest.mark.parametrize("name", ["ten_layer", "corrugated", "twisted_small_cell", "open_box"])
+def test_half_pair_window_beyond_lds(name):
+    """LMFF_HALF_WCAP=64 keeps 64 atoms of every window in the kernel's LDS; the partner sums of
+    every later slot go straight to the global fixed-point sums (the kernel's OVF variant, which
+    splits each row segment where its window slots reach 64).  The partner sums are integers
+    either way; the atom's own sums are floating-point sums over the lanes, which the split
+    regroups, so the results equal the default's to rounding (FP64 1e-12 of the RMS force, mixed
+    1e-6 eV/A RMS) and the oracle's to the FP64 bar.  Periodic images inside a window and
+    windows cut by the edges of an open box are among the cases."""
+    s = CASES[name]()
+    print(f"half-pair windows beyond the LDS, {name}: {s.n} atoms", flush=True)
+    a = _evaluate(s, capi.LMFF_FP64, True)
+    b = _with_env("LMFF_HALF_WCAP", "64", lambda: _evaluate(s, capi.LMFF_FP64, True))
+    rms = math.sqrt((a["f"] ** 2).sum(1).mean())
+    assert np.abs(a["f"] - b["f"]).max() <= 1e-12 * rms
+    for k in ("energy", "e_vdw", "e_rep"):
+        assert abs(a[k] - b[k]) <= 1e-12 * abs(a[k]), k
+    assert np.abs(a["virial"] - b["virial"]).max() <= 1e-12 * np.abs(a["virial"]).max()
+    _check_fp64(b, oracle_for(s).compute(1, 2))
+    am = _evaluate(s, capi.LMFF_MIXED, True)
+    bm = _with_env("LMFF_HALF_WCAP", "64", lambda: _evaluate(s, capi.LMFF_MIXED, True))
+    assert math.sqrt(((am["f"] - bm["f"]) ** 2).sum(1).mean()) <= 1e-6
+    assert abs(am["energy"] - bm["energy"]) <= 1e-7 * abs(am["energy"])
+
+
+def _resident(s, x=None, steps=0):
+    """Forces and potential energy of a resident system, after an optional move of its atoms (the
+    next list build re-fits the bins of the open z side), then an optional NVE run."""
+    with ctx_for(s) as ctx:
+        sid = ctx.add_system(s)
+        ctx.integrator(sid, capi.LMFF_NVE, dt=0.001)
+        ctx.evaluate(1, 2)
+        if x is not None:
+            ctx.set(sid, x=x)
+            ctx.evaluate(1, 2)
+        pe = ctx.thermo(sid)["pe"]
+        f = ctx.get(sid)[2]
+        if steps:
+            ctx.run(steps, steps, use_graph=True)
+        return f, pe, ctx.get(sid)[0]
+
+
+def test_half_pair_bins_follow_a_moving_stack():
+    """A resident 4-layer stack whose top sheet then moves up by 4 A while its bottom sheet ripples
+    by 1.5 A: the next list build re-fits the z bins to the new extent (two bins more), and the
+    forces equal those of a context that starts from the moved positions and the oracle's; 30
+    Verlet steps (captured steps with the new tile count) follow the same trajectory."""
+    s = S.layered_stack(15, 9, ["G", "BN", "G", "BN"], seed=79)
+    x1 = s.x.copy()
+    x1[s.mol == 4, 2] += 4.0
+    lx = s.boxhi[0] - s.boxlo[0]
+    bot = s.mol == 1
+    x1[bot, 2] += 1.5 * np.sin(2.0 * math.pi * (s.x[bot, 0] - s.boxlo[0]) / lx)
+    moved = s.copy()
+    moved.x = x1
+    print(f"resident stack moved after set-up: {s.n} atoms", flush=True)
+    fa, pea, xa = _resident(s, x1, steps=30)
+    fb, peb, xb = _resident(moved, steps=30)
+    r = oracle_for(moved).compute(1, 2)
+    rms = math.sqrt((r["f"] ** 2).sum(1).mean())
+    assert np.abs(fa - fb).max() <= 1e-12 * rms
+    assert np.abs(fa - r["f"]).max() <= 1e-10 * rms
+    assert abs(pea - r["energy"]) <= 1e-10 * abs(r["energy"])
+    assert abs(pea - peb) <= 1e-12 * abs(peb)
+    assert np.abs(xa - xb).max() <= 1e-9
