@@ -154,6 +154,10 @@ struct SysHost {
   unsigned long long seed = 1;
   long long step = 0;
   std::vector<int> chunk_ids;
+  // sheets of a stack (open z): 0 not looked at yet, 1 its molecules are the sheets (sheet_ids,
+  // sorted), -1 not a stack of at most MAX_SHEETS molecules
+  int sheet_state = 0;
+  std::vector<int> sheet_ids;
 };
 
 void check_box(const double lo[3], const double hi[3], const int per[3], double cutneigh);
@@ -214,6 +218,8 @@ struct Ctx {
   DBuf lidx, segf, segb, win, winmax, syshalf, acc, ipart;
   DBuf extent_d;  // per re-fitted system: the atoms' extent (k_extent)
   PinnedBuf regrid_host;  // the extents read back, the system table's grid fields written
+  DBuf sheet_d;  // per sheet of the re-fitted stacks: z sum and count (k_sheet_z), then the ids
+  PinnedBuf sheet_host;
   // full ILP rows for evaluations by the pull kernel while half systems hold half rows
   DBuf ilist_full, icnt_full, ocnt_full;
   int icap_full = 0;
@@ -271,7 +277,7 @@ struct Ctx {
                    &rnc, &rncnt, &eb, &fb, &wb, &ft,
                    &fil, &S, &T, &th, &vatom, &cell_start, &row_mol, &sort_tmp, &systab, &chunks, &cbeg, &partial, &sums,
                    &rmass, &ip_d, &ip_f, &tp_d, &tp_f, &t3, &errw, &ovf, &dflag, &fsave, &farflag,
-                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart, &ilist_full, &icnt_full, &ocnt_full, &extent_d};
+                   &cg_x0, &cg_g, &cg_h, &hb_in, &hb_out, &lidx, &segf, &segb, &win, &winmax, &syshalf, &acc, &ipart, &ilist_full, &icnt_full, &ocnt_full, &extent_d, &sheet_d};
     for (DBuf *b : all) b->tally = &bytes_in_use;
     const char *h = std::getenv("LMFF_HALF");
     if (h && std::strcmp(h, "0") == 0) half_enabled = false;
@@ -567,6 +573,25 @@ struct Ctx {
     }
     if (which.empty()) return;
     const size_t m = which.size();
+    // the sheets of a stack: its molecules, when there are 2 .. MAX_SHEETS of them (looked up
+    // once per system; the ILP tells layers apart by molecule, so a stack has one per sheet)
+    std::vector<int> sheet_off(m, -1);
+    int nsheet = 0;
+    for (size_t q = 0; q < m; q++) {
+      SysHost &s = sys[which[q]];
+      if (s.per[2]) continue;
+      if (s.sheet_state == 0) {
+        auto ids = download<int>(mol, (size_t)s.begin, (size_t)s.n);
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        s.sheet_state = (ids.size() >= 2 && ids.size() <= (size_t)MAX_SHEETS) ? 1 : -1;
+        if (s.sheet_state == 1) s.sheet_ids = ids;
+      }
+      if (s.sheet_state == 1) {
+        sheet_off[q] = nsheet;
+        nsheet += (int)s.sheet_ids.size();
+      }
+    }
     // minima of every re-fitted system first, then the maxima
     const size_t eb = sizeof(unsigned long long) * 3 * m;
     extent_d.ensure(2 * eb);
@@ -574,16 +599,33 @@ struct Ctx {
     HIPCHK(hipMemsetAsync(extent_d.p, 0xff, eb, stream));
     HIPCHK(hipMemsetAsync((char *)extent_d.p + eb, 0, eb, stream));
     unsigned long long *emin = extent_d.as<unsigned long long>(), *emax = emin + 3 * m;
+    const size_t sb = sizeof(unsigned long long) * 2 * (size_t)nsheet;
+    if (nsheet) {
+      sheet_d.ensure(sb + sizeof(int) * (size_t)nsheet);
+      sheet_host.ensure(sb + sizeof(int) * (size_t)nsheet);
+      int *hid = (int *)((char *)sheet_host.p + sb);
+      for (size_t q = 0; q < m; q++)
+        if (sheet_off[q] >= 0)
+          std::copy(sys[which[q]].sheet_ids.begin(), sys[which[q]].sheet_ids.end(), hid + sheet_off[q]);
+      HIPCHK(hipMemsetAsync(sheet_d.p, 0, sb, stream));
+      HIPCHK(hipMemcpyAsync((char *)sheet_d.p + sb, hid, sizeof(int) * (size_t)nsheet, hipMemcpyHostToDevice, stream));
+    }
     for (size_t q = 0; q < m; q++) {
       const SysHost &s = sys[which[q]];
       const int blocks = (int)std::min<int64_t>(256, (s.n + BS - 1) / BS);
       hipLaunchKernelGGL(k_extent, dim3(blocks), dim3(BS), 0, stream, s.begin, s.end, (const double4 *)x.p,
                          emin + 3 * q, emax + 3 * q);
+      if (sheet_off[q] >= 0)
+        hipLaunchKernelGGL(k_sheet_z, dim3(blocks), dim3(BS), 0, stream, s.begin, s.end, (const double4 *)x.p,
+                           (const int *)mol.p, (const int *)((char *)sheet_d.p + sb) + sheet_off[q],
+                           (int)s.sheet_ids.size(), sheet_d.as<unsigned long long>() + 2 * sheet_off[q]);
     }
     HIPCHK(hipMemcpyAsync(regrid_host.p, extent_d.p, 2 * eb, hipMemcpyDeviceToHost, stream));
+    if (nsheet) HIPCHK(hipMemcpyAsync(sheet_host.p, sheet_d.p, sb, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     std::vector<unsigned long long> ex((const unsigned long long *)regrid_host.p,
                                        (const unsigned long long *)regrid_host.p + 6 * m);
+    const unsigned long long *zs = (const unsigned long long *)sheet_host.p;
     auto value = [](unsigned long long k) {
       const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
       double v;
@@ -601,6 +643,32 @@ struct Ctx {
       }
       const SysHost old = s;
       grid_from_extent(s, mn, mx, s.n, cutneigh);
+      if (sheet_off[q] >= 0) {
+        // one z bin per sheet, centred on the sheets' mean heights, when they are evenly spaced
+        // (within a factor 1.5) and far enough apart for the list build's stencil (at most 8 bins
+        // of reach): each sheet keeps its bin while it ripples by less than half the spacing
+        const int nl = (int)s.sheet_ids.size();
+        std::vector<double> zm(nl);
+        bool ok = true;
+        for (int r = 0; r < nl; r++) {
+          const unsigned long long c = zs[2 * (sheet_off[q] + r) + 1];
+          ok = ok && c > 0;
+          zm[r] = c ? (double)(long long)zs[2 * (sheet_off[q] + r)] / 1048576.0 / (double)c : 0.0;
+        }
+        std::sort(zm.begin(), zm.end());
+        double dmin = 1e300, dmax = 0.0;
+        for (int r = 0; r + 1 < nl; r++) {
+          dmin = std::min(dmin, zm[r + 1] - zm[r]);
+          dmax = std::max(dmax, zm[r + 1] - zm[r]);
+        }
+        const double bsz = ok ? (zm[nl - 1] - zm[0]) / (nl - 1) : 0.0;
+        if (ok && dmin > 0 && dmax <= 1.5 * dmin && bsz >= cutneigh / 8.0) {
+          s.nc[2] = nl;
+          s.bs[2] = bsz;
+          s.glo[2] = zm[0] - 0.5 * bsz;
+          s.ncells = s.nc[0] * s.nc[1] * s.nc[2];
+        }
+      }
       for (int d = 0; d < 3; d++) {
         changed |= s.nc[d] != old.nc[d] || s.glo[d] != old.glo[d] || s.bs[d] != old.bs[d];
         counts |= s.nc[d] != old.nc[d];
@@ -918,8 +986,9 @@ struct Ctx {
 
   // Which systems run their ILP pairs in the half-pair kernel: those with tiles, whose stencil
   // reaches at most NSEG - 2 z bins up, and whose list rows fit the 16-bit segment counts.  The
-  // kernel's LDS holds the largest window up to WCAP_MAX atoms (LMFF_HALF_WCAP in the
-  // environment lowers that limit); slots beyond it add to the global sums directly.  Decided per system from its own geometry (on the
+  // kernel's LDS holds the largest window up to the size that keeps two workgroups per CU
+  // (half_lds_cap; LMFF_HALF_WCAP in the environment lowers that limit); slots beyond it add to
+  // the global sums directly.  Decided per system from its own geometry (on the
   // device, k_half_decide, before the list build, which keeps half rows for those systems), so a
   // system's results do not depend on what else is packed with it; here the host takes the
   // decisions back and sizes the launches.
@@ -941,10 +1010,10 @@ struct Ctx {
       else if (sys[k].n > 0) pull = true;
     }
     const int wmax = wc;
-    wc = std::min((wc + 63) / 64 * 64, half_wcap_limit());
+    if (any) half_attrs();  // here, never inside a graph capture
+    wc = std::min({(wc + 63) / 64 * 64, half_wcap_limit(), half_lds_cap});
     const bool ovf = any && wmax > wc;
     if (any) {
-      half_attrs();  // here, never inside a graph capture
       // the atoms' own sums at the widest layout (with the virial), so that no later evaluation
       // moves the buffer under a captured graph
       if (ipart.bytes < sizeof(double) * (size_t)n * half_pstr(true, true)) {
@@ -1081,27 +1150,35 @@ struct Ctx {
     else launch_half_one<R, false, false, SYM>(P);
   }
   // the half-pair kernel's window sums take more than the default 64 KB of LDS per workgroup
+  // (and returns the largest static LDS of the variants)
   template <typename R, bool EF, bool VF, bool SYM>
-  static void half_attr_one() {
-    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM, false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_ilp_half<R, EF, VF, SYM, true>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
+  static size_t half_attr_one() {
+    size_t st = 0;
+    for (const void *k : {(const void *)k_ilp_half<R, EF, VF, SYM, false>, (const void *)k_ilp_half<R, EF, VF, SYM, true>}) {
+      HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(unsigned long long) * 6 * WCAP_MAX)));
+      hipFuncAttributes a{};
+      HIPCHK(hipFuncGetAttributes(&a, k));
+      st = std::max(st, (size_t)a.sharedSizeBytes);
+    }
+    return st;
   }
   template <typename R>
-  static void half_attr_prec() {
-    half_attr_one<R, false, false, false>();
-    half_attr_one<R, true, false, false>();
-    half_attr_one<R, true, true, false>();
-    half_attr_one<R, false, false, true>();
-    half_attr_one<R, true, false, true>();
-    half_attr_one<R, true, true, true>();
+  static size_t half_attr_prec() {
+    return std::max({half_attr_one<R, false, false, false>(), half_attr_one<R, true, false, false>(),
+                     half_attr_one<R, true, true, false>(), half_attr_one<R, false, false, true>(),
+                     half_attr_one<R, true, false, true>(), half_attr_one<R, true, true, true>()});
   }
   bool half_attr_done = false;
+  // window atoms whose sums leave room in a CU's 160 KB of LDS for two workgroups of the kernel
+  // (with its static LDS): a larger window would halve the kernel's occupancy
+  int half_lds_cap = 64;
   void half_attrs() {
     if (half_attr_done) return;
-    half_attr_prec<double>();
-    half_attr_prec<float>();
+    const size_t st = std::max(half_attr_prec<double>(), half_attr_prec<float>());
+    const size_t per_wg = (size_t)160 * 1024 / 2;
+    const size_t room = per_wg > st ? per_wg - st : 0;
+    half_lds_cap = std::max(64, std::min(WCAP_MAX, (int)(room / (6 * sizeof(unsigned long long))) / 64 * 64));
     half_attr_done = true;
   }
 

@@ -61,9 +61,10 @@ constexpr int NSEG = 16;
 // window tables: per window (tile, partner z bin) the exclusive prefix of atoms over its bins
 // (row-major over (tile[1] + 2 ry) rows of (tile[0] + 2 rx) bins), the total at [bins]
 constexpr int WSTR = 256;
-// window atoms the half-pair kernel holds in LDS (six int64 sums each: 72 KB); the partner sums
-// of slots beyond it (windows of rippled or crowded sheets) go to the global sums with device
-// atomics instead
+// window atoms the half-pair kernel can hold in LDS (six int64 sums each: 72 KB); the host keeps
+// the windows it launches with below what leaves room for two workgroups per CU (about 1,400
+// atoms with the kernel's static LDS), and the partner sums of slots beyond that (windows of
+// rippled or crowded sheets) go to the global sums with device atomics instead
 constexpr int WCAP_MAX = 1536;
 // wavefronts per half-pair workgroup and the waves per SIMD the kernel is compiled for: two
 // workgroups fit a CU's LDS, so 8 waves give 4 per SIMD (128 VGPRs; 10-wave workgroups for FP32,
@@ -346,6 +347,32 @@ __device__ __forceinline__ bool tag_skip(int itag, int jtag, double xi, double y
 __device__ __forceinline__ unsigned long long ord_key(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Per sheet (molecule) of atoms [begin, end): the z sum in fixed point (z 2^20, exact and
+// independent of the order of the adds) and the atom count, into acc[2 r], acc[2 r + 1] for the
+// sheet whose molecule id is ids[r] (ids sorted, nl <= MAX_SHEETS; the caller clears acc).
+constexpr int MAX_SHEETS = 64;
+__global__ __launch_bounds__(256) void k_sheet_z(int begin, int end, const double4 *x, const int *mol,
+                                                 const int *ids, int nl, unsigned long long *acc) {
+  __shared__ unsigned long long s_acc[2 * MAX_SHEETS];
+  for (int k = threadIdx.x; k < 2 * nl; k += blockDim.x) s_acc[k] = 0ull;
+  __syncthreads();
+  for (int p = begin + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += gridDim.x * blockDim.x) {
+    const int m = mol[p];
+    int lo = 0, hi = nl - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ids[mid] < m) lo = mid + 1;
+      else hi = mid;
+    }
+    if (ids[lo] != m) continue;  // not one of the sheets (the host checked there is none)
+    atomicAdd(&s_acc[2 * lo], (unsigned long long)llrint(x[p].z * 1048576.0));
+    atomicAdd(&s_acc[2 * lo + 1], 1ull);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 2 * nl; k += blockDim.x)
+    if (s_acc[k] != 0ull) atomicAdd(acc + k, s_acc[k]);
 }
 
 // The extent of atoms [begin, end) per dimension as ord_key images: emin[d], emax[d] (the caller

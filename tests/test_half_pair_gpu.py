@@ -190,10 +190,11 @@ def _resident(s, x=None, steps=0):
 
 
 def test_half_pair_bins_follow_a_moving_stack():
-    """A resident 4-layer stack whose top sheet then moves up by 4 A while its bottom sheet ripples
-    by 1.5 A: the next list build re-fits the z bins to the new extent (two bins more), and the
-    forces equal those of a context that starts from the moved positions and the oracle's; 30
-    Verlet steps (captured steps with the new tile count) follow the same trajectory."""
+    """A resident 4-layer stack (one z bin per sheet, centred on the sheets) whose top sheet then
+    moves up by 4 A while its bottom sheet ripples by 1.5 A: the sheets are no longer evenly
+    spaced, so the next list build fits the z bins to the atoms' extent instead (six bins), and
+    the forces equal those of a context that starts from the moved positions and the oracle's;
+    30 Verlet steps (captured steps with the new tile count) follow the same trajectory."""
     s = S.layered_stack(15, 9, ["G", "BN", "G", "BN"], seed=79)
     x1 = s.x.copy()
     x1[s.mol == 4, 2] += 4.0
