@@ -217,6 +217,9 @@ struct Ctx {
   // largest window and mode, partner sums
   DBuf lidx, segf, segb, win, winmax, syshalf, acc, ipart;
   DBuf extent_d;  // per re-fitted system: the atoms' extent (k_extent)
+  static constexpr int REGRID_EVERY = 5;
+  bool regrid_due = true;
+  int builds_since_regrid = 0;
   PinnedBuf regrid_host;  // the extents read back, the system table's grid fields written
   DBuf sheet_d;  // per sheet of the re-fitted stacks: z sum and count (k_sheet_z), then the ids
   PinnedBuf sheet_host;
@@ -543,6 +546,7 @@ struct Ctx {
     size_cells();
     layout_dirty = false;
     need_rebuild = true;
+    regrid_due = true;
     forces_valid = forces_valid && !had_new;
     gen++;
   }
@@ -561,11 +565,16 @@ struct Ctx {
   }
 
   // Re-fits the bins of the non-periodic sides of every system of at least half_min_atoms()
-  // atoms to its atoms' current extent (grid_from_extent), at the start of every list build: a
+  // atoms to its atoms' current extent (grid_from_extent), at the start of a list build: a
   // stack whose sheets expand or ripple would otherwise pile its outer sheets into the edge
   // bins, whose windows then outgrow the half-pair kernel's LDS.  The cell, row and tile offsets
   // of every system follow; a change in their counts invalidates captured steps.
   void regrid() {
+    // sheets breathe and ripple over hundreds of steps: re-fit at every REGRID_EVERY-th build, and
+    // at the first build after a layout change or positions from the host
+    if (!regrid_due && ++builds_since_regrid < REGRID_EVERY) return;
+    regrid_due = false;
+    builds_since_regrid = 0;
     std::vector<int> which;
     for (size_t k = 0; k < sys.size(); k++) {
       const SysHost &s = sys[k];
@@ -2082,7 +2091,7 @@ void setup_grid(SysHost &s, const double *x, int64_t n, double cutneigh) {
 }
 
 // The grid of setup_grid from the extent [mn, mx] of the atoms along the non-periodic sides
-// (periodic sides take the box).  Also what each list build of a running system re-fits its
+// (periodic sides take the box).  Also what the list builds of a running system re-fit its
 // non-periodic sides with, so that the edge bins of a stack that expands or ripples do not
 // gather the atoms of two sheets.
 void grid_from_extent(SysHost &s, const double mn_[3], const double mx_[3], int64_t n, double cutneigh) {
@@ -2471,6 +2480,7 @@ int lmff_system_set(lmff_ctx *ctx, int sys_id, const double *xi, const double *v
     upd(c.v, vi);
     if (xi) {
       c.need_rebuild = true;
+      c.regrid_due = true;
       c.forces_valid = false;
     }
   });
@@ -2638,6 +2648,7 @@ int lmff_pair_compute(lmff_ctx *ctx, int64_t nlocal, const double *x, const int3
                              (const long long *)dmol, c.x.as<double4>() + b, c.v.as<double4>() + b, c.tag.as<int>() + b,
                              c.mol.as<int>() + b, c.orig.as<int>() + b, c.img.as<int>() + b);
           c.need_rebuild = true;
+          c.regrid_due = true;
           c.forces_valid = false;
         }
       }
